@@ -1,0 +1,173 @@
+/* mdft_hip.h -- C ABI of the MI355X (gfx950) excess-functional hot path.
+ *
+ * Plain pointers and sizes only.  Every entry point returns 0 on success and
+ * a nonzero status on failure; mdft_last_error() then returns the message of
+ * the calling thread's last failure (the reference throws std::runtime_error
+ * from Impl::Throw_If, common/MDFT_Asserts.hpp:15-28; the C++ facade in
+ * mini_mdft_amd/cpp rethrows it).  No exception crosses this boundary.
+ *
+ * Device pointers are HIP device memory; `stream` is a hipStream_t (NULL =
+ * the default stream).  Calls are stream ordered and do not synchronise,
+ * except where a host scalar is returned (ff).  A handle is not thread safe:
+ * one handle per stream.
+ *
+ * Layouts are the reference's LayoutRight views:
+ *   orientation fields o, xi, vexc, df : (nx*ny*nz, ntheta, nphi, npsi) real
+ *   projections p                     : (np, nx, ny, nz) complex (re, im)
+ *
+ * Precision forms (MDFT_FP64 / MDFT_FP32 / MDFT_MIXED) fix the element types:
+ *   form        o / xi / df   p               arithmetic
+ *   MDFT_FP64   double        complex double  FP64 everywhere
+ *   MDFT_FP32   float         complex float   FP32 (reference ScalarType=float)
+ *   MDFT_MIXED  double        complex float   projections: BF16x3 MFMA with
+ *                                             FP32 accumulation; FFT, rotation,
+ *                                             OZ in FP32; R(q), interpolation
+ *                                             weights, ff in FP64
+ */
+#ifndef MDFT_HIP_H
+#define MDFT_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDFT_FP64 0
+#define MDFT_FP32 1
+#define MDFT_MIXED 2
+
+/* element type codes of the free functions */
+#define MDFT_DOUBLE 0
+#define MDFT_FLOAT 1
+
+typedef struct mdft_opt_s* mdft_opt_t;
+typedef struct mdft_conv_s* mdft_conv_t;
+typedef struct mdft_solver_s* mdft_solver_t;
+
+const char* mdft_last_error(void);
+int mdft_version(void);
+
+/* ---- init-time tables (host, FP64) ------------------------------------ */
+
+/* common/MDFT_Math_Utils.hpp:24-73 (gauss_legendre): nodes descending,
+ * weights summing to 1. */
+int mdft_gauss_legendre(int n, double* x, double* w);
+
+/* common/MDFT_Wigner.hpp:62-130 (wigner_small_d): d^m_{mu,mup}(theta). */
+int mdft_wigner_small_d(int m, int mu, int mup, double theta, double* out);
+
+/* common/MDFT_Grid.hpp:129-152 (AngularGrid sizes): sizes[0..5] = ntheta,
+ * nphi, npsi, no, np, mmax/mrso + 1. */
+int mdft_angular_sizes(int mmax, int mrso, int* sizes);
+
+/* common/MDFT_Grid.hpp:129-263 (AngularGrid) and
+ * common/MDFT_OrientationProjectionTransform.hpp:55-147
+ * (OrientationProjectionMap): theta and wtheta (ntheta), w (no), p map
+ * ((mmax+1) x (2mmax+1) x (mmax/mrso+1)), m, mup, mu, mu2 (np), and the
+ * Wigner table (ntheta x np).  Any output pointer may be NULL. */
+int mdft_angular_tables(int mmax, int mrso, double* theta, double* wtheta, double* w, int* pmap,
+                        int* m, int* mup, int* mu, int* mu2, double* wigner_d);
+
+/* common/MDFT_Grid.hpp:87-103 (SpatialGrid::kproj): k_i for a box side L. */
+int mdft_kproj(int n, double length, double* k);
+
+/* common/MDFT_Rotation.hpp:103-283
+ * (rotation_matrix_between_complex_spherical_harmonics_lu): R has
+ * (mmax+1) x (2mmax+1) x (2mmax+1) complex entries (re, im interleaved). */
+int mdft_rotation_matrix(int mmax, const double* q, double* R);
+
+/* common/MDFT_Convolution.hpp:126-176 (the (q, -q) list, gamma_p_map):
+ * the count, and the list in the reference's order (ix, iy, iz triples). */
+int mdft_qpair_count(int nx, int ny, int nz);
+int mdft_qpairs(int nx, int ny, int nz, int* ixyz);
+
+/* common/IO/MDFT_ReadCLuc.hpp:40-259 (LucData): reads np_luc tuples and
+ * nq = int(qmaxwanted/dq + 0.5) + 2 rows of a Luc c(q) file.  tuples:
+ * np_luc x 5 ints (m, n, mu, nu, khi), c: np_luc x nq complex (re, im); either
+ * may be NULL (call once with NULLs to get nq). */
+int mdft_luc_read(const char* path, int np_luc, double qmaxwanted, int* nq, double* dq, int* tuples,
+                  double* c);
+
+/* ---- OrientationProjectionTransform --------------------------------------
+ * common/MDFT_OrientationProjectionTransform.hpp:161-485: the constructor
+ * (:217-257), angl2proj (:267-380) and proj2angl (:388-484). */
+int mdft_opt_create(int nx, int ny, int nz, int mmax, int mrso, int precision, mdft_opt_t* out);
+int mdft_opt_destroy(mdft_opt_t h);
+int mdft_angl2proj(mdft_opt_t h, const void* o, void* p, void* stream);
+int mdft_proj2angl(mdft_opt_t h, const void* p, void* o, void* stream);
+
+/* ---- Convolution -----------------------------------------------------------
+ * common/MDFT_Convolution.hpp:20-549: the constructor (:106-313) reading the
+ * Luc file `luc_path` with np_luc tuples, and execute (:324-548), in place on
+ * p.  mdft_conv_create_from_table takes the tuples and values that
+ * mdft_luc_read returns instead of a path (nq_table >= the rows the grid
+ * needs). */
+int mdft_conv_create(const char* luc_path, int nx, int ny, int nz, double lx, double ly, double lz,
+                     int mmax, int mrso, int np_luc, int precision, mdft_conv_t* out);
+int mdft_conv_create_from_table(int nx, int ny, int nz, double lx, double ly, double lz, int mmax,
+                                int mrso, int np_luc, const int* tuples, const double* c, int nq_table,
+                                double dq, int precision, mdft_conv_t* out);
+/* accessors dq() and the sizes of gamma_p_map() / mnmunukhi_q() (:316-318) */
+int mdft_conv_info(mdft_conv_t h, int* nq, int* np_new, int* npairs, double* dq);
+/* mnmunukhi_q (:316-318): np_new x nq complex (re, im), signs folded in */
+int mdft_conv_table(mdft_conv_t h, double* c);
+int mdft_conv_execute(mdft_conv_t h, void* p, void* stream);
+int mdft_conv_destroy(mdft_conv_t h);
+
+/* Batched unnormalised 3D c2c FFT, in place over the last three axes of a
+ * (batch, nx, ny, nz) complex array (the KokkosFFT plans of
+ * common/MDFT_Convolution.hpp:198-208); sign -1 forward, +1 backward;
+ * dtype MDFT_DOUBLE or MDFT_FLOAT. */
+int mdft_fft3d(void* data, int batch, int nx, int ny, int nz, int sign, int dtype, void* stream);
+
+/* ---- Solvent free functions ---------------------------------------------- */
+
+/* common/MDFT_Solvent.hpp:575-596 (get_delta_rho): delta_rho = rho0 (xi^2 - 1)
+ * over n elements. */
+int mdft_delta_rho(const void* xi, void* delta_rho, size_t n, double rho0, int dtype, void* stream);
+
+/* common/MDFT_Solvent.hpp:613-660 (get_delta_f): delta_f = 2 rho0 xi vexc and
+ * *ff = sum rho0 prefactor vexc w (xi^2 - 1) (FP64 accumulation, fixed
+ * order: deterministic).  w is a device array of no doubles; ff is a host
+ * pointer and the call synchronises the stream to fill it. */
+int mdft_delta_f(const void* xi, const void* vexc, const double* w, void* delta_f, size_t nxyz, int no,
+                 double rho0, double prefactor, double* ff, int dtype, void* stream);
+
+/* ---- Solver: one c-proj functional + gradient evaluation -------------------
+ * src/mini-mdft.hpp:137-193 (Solver::energy_cproj_mrso): delta_rho ->
+ * angl2proj -> Convolution::execute -> proj2angl -> get_delta_f, with
+ * delta_rho fused into the projection load and get_delta_f into the
+ * proj2angl epilogue.  xi and df are (N, no) doubles for MDFT_FP64 and
+ * MDFT_MIXED, floats for MDFT_FP32.  The solver owns its work buffers
+ * (mdft_solver_workspace_bytes). */
+int mdft_solver_create(mdft_conv_t conv, double rho0, double prefactor, int precision,
+                       mdft_solver_t* out);
+int mdft_solver_workspace_bytes(mdft_solver_t h, size_t* bytes);
+int mdft_solver_destroy(mdft_solver_t h);
+/* ff: host pointer, the call synchronises the stream */
+int mdft_energy_cproj(mdft_solver_t h, const void* xi, void* df, double* ff, void* stream);
+/* ff_device: one device double written on the stream, no synchronisation */
+int mdft_energy_cproj_async(mdft_solver_t h, const void* xi, void* df, double* ff_device, void* stream);
+
+/* Stage timers (the Solver's per-stage Kokkos::Timer printout,
+ * src/mini-mdft.hpp:140-192), as hipEvents recorded on the work stream
+ * without synchronising between evaluations.  set_timing(1) starts (and
+ * resets) them, set_timing(0) stops them.  stage_ms waits for the recorded
+ * evaluations and returns the accumulated milliseconds of ms[0..4] =
+ * angl2proj (delta_rho fused), forward 3D FFT, rotation/OZ pass, backward 3D
+ * FFT, proj2angl + delta_f, and the number of evaluations in *count. */
+int mdft_solver_set_timing(mdft_solver_t h, int enable);
+int mdft_solver_stage_ms(mdft_solver_t h, double* ms, long* count);
+
+/* ---- utilities --------------------------------------------------------------
+ * Counter-based uniforms: x_i = lo + (hi - lo) u_i with u_i the splitmix64
+ * stream of (seed, offset + i) (same definition as mini_mdft_amd/rng.py). */
+int mdft_fill_uniform(void* x, size_t n, unsigned long long seed, unsigned long long offset, double lo,
+                      double hi, int dtype, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MDFT_HIP_H */

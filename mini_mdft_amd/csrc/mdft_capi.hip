@@ -1,0 +1,490 @@
+// The C ABI of include/mdft_hip.h: handles, argument checks, error capture
+// and dispatch by precision form.  Exceptions never cross the boundary; every
+// failure becomes a nonzero status plus the thread's last-error text (the
+// reference raises std::runtime_error via Impl::Throw_If,
+// common/MDFT_Asserts.hpp:15-28).
+#include <cfloat>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/mdft_hip.h"
+#include "mdft_conv.hpp"
+#include "mdft_gemm.hpp"
+#include "mdft_proj.hpp"
+
+using namespace mdft;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+  } catch (...) {
+    g_last_error = "unknown error";
+  }
+  return 1;
+}
+
+void check_precision(int precision) {
+  throw_if(precision != MDFT_FP64 && precision != MDFT_FP32 && precision != MDFT_MIXED, "unknown precision form");
+}
+
+void check_dtype(int dtype) { throw_if(dtype != MDFT_DOUBLE && dtype != MDFT_FLOAT, "unknown element type"); }
+
+// Chop threshold of gamma (common/MDFT_Convolution.hpp:373, :514-517): the
+// epsilon of the reference's ScalarType, i.e. DBL_EPSILON for MDFT_FP64 and
+// FLT_EPSILON for MDFT_FP32.  MDFT_MIXED approximates the FP64 form and keeps
+// its threshold.
+double chop_eps_of(int precision) { return precision == MDFT_FP32 ? FLT_EPSILON : DBL_EPSILON; }
+
+}  // namespace
+
+struct mdft_opt_s {
+  int nx, ny, nz, precision;
+  long N;
+  AngularGrid grid;
+  std::unique_ptr<ProjTables> tab;
+  std::unique_ptr<ProjGemm> gemm;  // MDFT_MIXED only
+};
+
+struct mdft_conv_s {
+  int precision;
+  std::unique_ptr<ConvDevice> dev;
+};
+
+struct mdft_solver_s {
+  mdft_conv_s* conv;  // borrowed
+  int precision;
+  double rho0, prefactor;
+  long N;
+  std::unique_ptr<ProjTables> tab;
+  std::unique_ptr<ProjGemm> gemm;
+  void* pbuf = nullptr;  // projections, np x N complex
+  double* partials = nullptr;
+  int nparts = 0;
+  double* ffdev = nullptr;
+  size_t bytes = 0;
+  std::unique_ptr<StageTimer> timer;  // set while stage timing is on
+  ~mdft_solver_s() {
+    if (pbuf) (void)hipFree(pbuf);
+    if (partials) (void)hipFree(partials);
+    if (ffdev) (void)hipFree(ffdev);
+  }
+};
+
+extern "C" {
+
+const char* mdft_last_error(void) { return g_last_error.c_str(); }
+
+int mdft_version(void) { return 1; }
+
+int mdft_gauss_legendre(int n, double* x, double* w) {
+  return guarded([&] {
+    throw_if(n <= 0 || !x || !w, "gauss_legendre: bad arguments");
+    gauss_legendre(n, x, w);
+  });
+}
+
+int mdft_wigner_small_d(int m, int mu, int mup, double theta, double* out) {
+  return guarded([&] {
+    throw_if(!out, "null output");
+    *out = wigner_small_d(m, mu, mup, theta);
+  });
+}
+
+int mdft_angular_sizes(int mmax, int mrso, int* sizes) {
+  return guarded([&] {
+    throw_if(!sizes, "null output");
+    AngularGrid g = make_angular_grid(mmax, mrso);
+    const int v[6] = {g.ntheta, g.nphi, g.npsi, g.no, g.np, g.nmu2};
+    std::memcpy(sizes, v, sizeof(v));
+  });
+}
+
+int mdft_angular_tables(int mmax, int mrso, double* theta, double* wtheta, double* w, int* pmap, int* m, int* mup,
+                        int* mu, int* mu2, double* wigner_d) {
+  return guarded([&] {
+    AngularGrid g = make_angular_grid(mmax, mrso);
+    auto put = [](auto* dst, const auto& src) {
+      if (dst) std::copy(src.begin(), src.end(), dst);
+    };
+    put(theta, g.theta);
+    put(wtheta, g.wtheta);
+    put(w, g.w);
+    put(pmap, g.pmap);
+    put(m, g.m);
+    put(mup, g.mup);
+    put(mu, g.mu);
+    put(mu2, g.mu2);
+    put(wigner_d, g.wigner_d);
+  });
+}
+
+int mdft_kproj(int n, double length, double* k) {
+  return guarded([&] {
+    throw_if(n <= 0 || !k, "kproj: bad arguments");
+    auto v = kproj(n, length);
+    std::copy(v.begin(), v.end(), k);
+  });
+}
+
+int mdft_rotation_matrix(int mmax, const double* q, double* R) {
+  return guarded([&] {
+    throw_if(mmax < 0 || mmax > 6 || !q || !R, "rotation_matrix: bad arguments (0 <= mmax <= 6)");
+    rotation_matrix(mmax, q, reinterpret_cast<cd*>(R));
+  });
+}
+
+int mdft_qpair_count(int nx, int ny, int nz) {
+  int out = -1;
+  guarded([&] {
+    throw_if(nx <= 0 || ny <= 0 || nz <= 0, "grid sizes must be positive");
+    out = static_cast<int>(qpairs_reference(nx, ny, nz).size() / 3);
+  });
+  return out;
+}
+
+int mdft_qpairs(int nx, int ny, int nz, int* ixyz) {
+  return guarded([&] {
+    throw_if(nx <= 0 || ny <= 0 || nz <= 0 || !ixyz, "qpairs: bad arguments");
+    auto v = qpairs_reference(nx, ny, nz);
+    std::copy(v.begin(), v.end(), ixyz);
+  });
+}
+
+int mdft_luc_read(const char* path, int np_luc, double qmaxwanted, int* nq, double* dq, int* tuples, double* c) {
+  return guarded([&] {
+    throw_if(!path || np_luc <= 0, "luc_read: bad arguments");
+    LucTable t = read_luc(path, np_luc, qmaxwanted);
+    if (nq) *nq = t.nq;
+    if (dq) *dq = t.dq;
+    if (tuples) std::copy(t.tuples.begin(), t.tuples.end(), tuples);
+    if (c) std::memcpy(c, t.c.data(), t.c.size() * sizeof(cd));
+  });
+}
+
+// ---------------------------------------------------------------------------
+int mdft_opt_create(int nx, int ny, int nz, int mmax, int mrso, int precision, mdft_opt_t* out) {
+  return guarded([&] {
+    throw_if(!out, "null output handle");
+    *out = nullptr;
+    check_precision(precision);
+    throw_if(nx <= 0 || ny <= 0 || nz <= 0, "The space is divided into grid nodes.");
+    auto h = std::make_unique<mdft_opt_s>();
+    h->nx = nx; h->ny = ny; h->nz = nz;
+    h->N = static_cast<long>(nx) * ny * nz;
+    h->precision = precision;
+    h->grid = make_angular_grid(mmax, mrso);
+    h->tab = std::make_unique<ProjTables>(h->grid);
+    if (precision == MDFT_MIXED) h->gemm = std::make_unique<ProjGemm>(h->grid);
+    *out = h.release();
+  });
+}
+
+int mdft_opt_destroy(mdft_opt_t h) {
+  return guarded([&] { delete h; });
+}
+
+int mdft_angl2proj(mdft_opt_t h, const void* o, void* p, void* stream) {
+  return guarded([&] {
+    throw_if(!h || !o || !p, "angl2proj: null argument");
+    hipStream_t s = as_stream(stream);
+    switch (h->precision) {
+      case MDFT_FP64:
+        angl2proj_factorized<double, double, double>(*h->tab, h->N, static_cast<const double*>(o),
+                                                     static_cast<cplx<double>*>(p), nullptr, s);
+        break;
+      case MDFT_FP32:
+        angl2proj_factorized<float, float, float>(*h->tab, h->N, static_cast<const float*>(o),
+                                                  static_cast<cplx<float>*>(p), nullptr, s);
+        break;
+      default:
+        h->gemm->angl2proj(h->N, static_cast<const double*>(o), static_cast<cplx<float>*>(p), nullptr, s);
+        break;
+    }
+  });
+}
+
+int mdft_proj2angl(mdft_opt_t h, const void* p, void* o, void* stream) {
+  return guarded([&] {
+    throw_if(!h || !o || !p, "proj2angl: null argument");
+    hipStream_t s = as_stream(stream);
+    switch (h->precision) {
+      case MDFT_FP64:
+        proj2angl_factorized<double, double, double>(*h->tab, h->N, static_cast<const cplx<double>*>(p),
+                                                     static_cast<double*>(o), nullptr, nullptr, 0.0, nullptr, s);
+        break;
+      case MDFT_FP32:
+        proj2angl_factorized<float, float, float>(*h->tab, h->N, static_cast<const cplx<float>*>(p),
+                                                  static_cast<float*>(o), nullptr, nullptr, 0.0, nullptr, s);
+        break;
+      default:
+        h->gemm->proj2angl(h->N, static_cast<const cplx<float>*>(p), static_cast<double*>(o), nullptr, nullptr,
+                           0.0, nullptr, s);
+        break;
+    }
+  });
+}
+
+// ---------------------------------------------------------------------------
+static mdft_conv_s* make_conv(int nx, int ny, int nz, double lx, double ly, double lz, int mmax, int mrso,
+                              int precision, const LucTable& luc) {
+  check_precision(precision);
+  auto h = std::make_unique<mdft_conv_s>();
+  h->precision = precision;
+  h->dev = std::make_unique<ConvDevice>(nx, ny, nz, lx, ly, lz, mmax, mrso, luc);
+  return h.release();
+}
+
+int mdft_conv_create(const char* luc_path, int nx, int ny, int nz, double lx, double ly, double lz, int mmax,
+                     int mrso, int np_luc, int precision, mdft_conv_t* out) {
+  return guarded([&] {
+    throw_if(!out || !luc_path, "conv_create: null argument");
+    *out = nullptr;
+    throw_if(nx <= 0 || ny <= 0 || nz <= 0, "The space is divided into grid nodes.");
+    throw_if(lx <= 0 || ly <= 0 || lz <= 0, "The supercell cannot have negative length.");
+    const auto kx = kproj(nx, lx), ky = kproj(ny, ly), kz = kproj(nz, lz);
+    LucTable luc = read_luc(luc_path, np_luc, qmax_of_grid(kx, ky, kz, nz));
+    *out = make_conv(nx, ny, nz, lx, ly, lz, mmax, mrso, precision, luc);
+  });
+}
+
+int mdft_conv_create_from_table(int nx, int ny, int nz, double lx, double ly, double lz, int mmax, int mrso,
+                                int np_luc, const int* tuples, const double* c, int nq_table, double dq,
+                                int precision, mdft_conv_t* out) {
+  return guarded([&] {
+    throw_if(!out || !tuples || !c, "conv_create_from_table: null argument");
+    *out = nullptr;
+    throw_if(np_luc <= 0 || nq_table <= 0 || !(dq > 0), "conv_create_from_table: bad sizes");
+    LucTable luc;
+    luc.np_luc = np_luc;
+    luc.nq = nq_table;
+    luc.dq = dq;
+    luc.tuples.assign(tuples, tuples + static_cast<size_t>(np_luc) * 5);
+    luc.c.resize(static_cast<size_t>(np_luc) * nq_table);
+    std::memcpy(luc.c.data(), c, luc.c.size() * sizeof(cd));
+    *out = make_conv(nx, ny, nz, lx, ly, lz, mmax, mrso, precision, luc);
+  });
+}
+
+int mdft_conv_info(mdft_conv_t h, int* nq, int* np_new, int* npairs, double* dq) {
+  return guarded([&] {
+    throw_if(!h, "null handle");
+    if (nq) *nq = h->dev->tab.nq;
+    if (np_new) *np_new = h->dev->tab.np_new;
+    if (npairs) *npairs = static_cast<int>(h->dev->npairs);
+    if (dq) *dq = h->dev->tab.dq;
+  });
+}
+
+int mdft_conv_table(mdft_conv_t h, double* c) {
+  return guarded([&] {
+    throw_if(!h || !c, "null argument");
+    std::memcpy(c, h->dev->tab.ctab.data(), h->dev->tab.ctab.size() * sizeof(cd));
+  });
+}
+
+int mdft_conv_execute(mdft_conv_t h, void* p, void* stream) {
+  return guarded([&] {
+    throw_if(!h || !p, "conv_execute: null argument");
+    if (h->precision == MDFT_FP64)
+      h->dev->execute<double>(static_cast<cplx<double>*>(p), chop_eps_of(h->precision), as_stream(stream));
+    else
+      h->dev->execute<float>(static_cast<cplx<float>*>(p), chop_eps_of(h->precision), as_stream(stream));
+  });
+}
+
+int mdft_conv_destroy(mdft_conv_t h) {
+  return guarded([&] { delete h; });
+}
+
+int mdft_fft3d(void* data, int batch, int nx, int ny, int nz, int sign, int dtype, void* stream) {
+  return guarded([&] {
+    throw_if(!data || batch < 0 || nx <= 0 || ny <= 0 || nz <= 0, "fft3d: bad arguments");
+    check_dtype(dtype);
+    Fft3dPlan plan(nx, ny, nz);
+    hipStream_t s = as_stream(stream);
+    if (dtype == MDFT_DOUBLE)
+      plan.execute<double>(static_cast<cplx<double>*>(data), batch, sign, s);
+    else
+      plan.execute<float>(static_cast<cplx<float>*>(data), batch, sign, s);
+    // the plan's twiddle tables are freed on return: finish the work first
+    MDFT_HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+// ---------------------------------------------------------------------------
+int mdft_delta_rho(const void* xi, void* delta_rho, size_t n, double rho0, int dtype, void* stream) {
+  return guarded([&] {
+    throw_if((!xi || !delta_rho) && n > 0, "delta_rho: null argument");
+    check_dtype(dtype);
+    if (dtype == MDFT_DOUBLE)
+      mdft::delta_rho<double>(static_cast<const double*>(xi), static_cast<double*>(delta_rho), n, rho0,
+                              as_stream(stream));
+    else
+      mdft::delta_rho<float>(static_cast<const float*>(xi), static_cast<float*>(delta_rho), n, rho0,
+                             as_stream(stream));
+  });
+}
+
+int mdft_delta_f(const void* xi, const void* vexc, const double* w, void* delta_f, size_t nxyz, int no, double rho0,
+                 double prefactor, double* ff, int dtype, void* stream) {
+  return guarded([&] {
+    throw_if(!ff || no <= 0, "delta_f: bad arguments");
+    check_dtype(dtype);
+    throw_if(nxyz > 0 && (!xi || !vexc || !w || !delta_f), "delta_f: null argument");
+    if (nxyz == 0) {
+      *ff = 0.0;
+      return;
+    }
+    hipStream_t s = as_stream(stream);
+    const int nb = delta_f_blocks(static_cast<long>(nxyz));
+    double* part = nullptr;
+    MDFT_HIP_CHECK(hipMallocAsync(reinterpret_cast<void**>(&part), (nb + 1) * sizeof(double), s));
+    if (dtype == MDFT_DOUBLE)
+      mdft::delta_f<double>(static_cast<const double*>(xi), static_cast<const double*>(vexc), w,
+                            static_cast<double*>(delta_f), static_cast<long>(nxyz), no, rho0, part, nb, s);
+    else
+      mdft::delta_f<float>(static_cast<const float*>(xi), static_cast<const float*>(vexc), w,
+                           static_cast<float*>(delta_f), static_cast<long>(nxyz), no, rho0, part, nb, s);
+    reduce_partials(part, nb, rho0 * prefactor, part + nb, s);
+    MDFT_HIP_CHECK(hipMemcpyAsync(ff, part + nb, sizeof(double), hipMemcpyDeviceToHost, s));
+    MDFT_HIP_CHECK(hipFreeAsync(part, s));
+    MDFT_HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+// ---------------------------------------------------------------------------
+int mdft_solver_create(mdft_conv_t conv, double rho0, double prefactor, int precision, mdft_solver_t* out) {
+  return guarded([&] {
+    throw_if(!out || !conv, "solver_create: null argument");
+    *out = nullptr;
+    check_precision(precision);
+    throw_if((precision == MDFT_FP64) != (conv->precision == MDFT_FP64),
+             "solver and convolution must agree on FP64 vs FP32 projections");
+    auto h = std::make_unique<mdft_solver_s>();
+    h->conv = conv;
+    h->precision = precision;
+    h->rho0 = rho0;
+    h->prefactor = prefactor;
+    const ConvTables& t = conv->dev->tab;
+    h->N = static_cast<long>(t.nx) * t.ny * t.nz;
+    h->tab = std::make_unique<ProjTables>(t.ang);
+    if (precision == MDFT_MIXED) h->gemm = std::make_unique<ProjGemm>(t.ang);
+    const size_t pbytes = static_cast<size_t>(h->N) * t.ang.np * (precision == MDFT_FP64 ? 16 : 8);
+    h->nparts = std::max(proj2angl_blocks(*h->tab, h->N), h->gemm ? h->gemm->p2a_blocks(h->N) : 0);
+    MDFT_HIP_CHECK(hipMalloc(&h->pbuf, pbytes));
+    MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&h->partials), h->nparts * sizeof(double)));
+    MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&h->ffdev), sizeof(double)));
+    h->bytes = pbytes + h->nparts * sizeof(double) + sizeof(double);
+    *out = h.release();
+  });
+}
+
+int mdft_solver_workspace_bytes(mdft_solver_t h, size_t* bytes) {
+  return guarded([&] {
+    throw_if(!h || !bytes, "null argument");
+    *bytes = h->bytes;
+  });
+}
+
+int mdft_solver_destroy(mdft_solver_t h) {
+  return guarded([&] { delete h; });
+}
+
+static void energy_impl(mdft_solver_t h, const void* xi, void* df, double* ffd, hipStream_t s) {
+  throw_if(!h || !xi || !df || !ffd, "energy_cproj: null argument");
+  const double rho0 = h->rho0;
+  const double eps = chop_eps_of(h->precision);
+  StageTimer* tm = h->timer.get();
+  if (tm) {
+    tm->begin_eval();
+    tm->mark(s, kStageStart);
+  }
+  int nb = 0;
+  switch (h->precision) {
+    case MDFT_FP64: {
+      auto* p = static_cast<cplx<double>*>(h->pbuf);
+      angl2proj_factorized<double, double, double>(*h->tab, h->N, static_cast<const double*>(xi), p, &rho0, s);
+      h->conv->dev->execute<double>(p, eps, s, tm);
+      proj2angl_factorized<double, double, double>(*h->tab, h->N, p, nullptr, static_cast<const double*>(xi),
+                                                   static_cast<double*>(df), rho0, h->partials, s);
+      nb = proj2angl_blocks(*h->tab, h->N);
+      break;
+    }
+    case MDFT_FP32: {
+      auto* p = static_cast<cplx<float>*>(h->pbuf);
+      angl2proj_factorized<float, float, float>(*h->tab, h->N, static_cast<const float*>(xi), p, &rho0, s);
+      h->conv->dev->execute<float>(p, eps, s, tm);
+      proj2angl_factorized<float, float, float>(*h->tab, h->N, p, nullptr, static_cast<const float*>(xi),
+                                                static_cast<float*>(df), rho0, h->partials, s);
+      nb = proj2angl_blocks(*h->tab, h->N);
+      break;
+    }
+    default: {
+      auto* p = static_cast<cplx<float>*>(h->pbuf);
+      h->gemm->angl2proj(h->N, static_cast<const double*>(xi), p, &rho0, s);
+      h->conv->dev->execute<float>(p, eps, s, tm);
+      nb = h->gemm->proj2angl(h->N, p, nullptr, static_cast<const double*>(xi), static_cast<double*>(df), rho0,
+                              h->partials, s);
+      break;
+    }
+  }
+  reduce_partials(h->partials, nb, rho0 * h->prefactor, ffd, s);
+  if (tm) tm->mark(s, kStageEnd);
+}
+
+int mdft_solver_set_timing(mdft_solver_t h, int enable) {
+  return guarded([&] {
+    throw_if(!h, "null handle");
+    if (enable)
+      h->timer = std::make_unique<StageTimer>();
+    else
+      h->timer.reset();
+  });
+}
+
+int mdft_solver_stage_ms(mdft_solver_t h, double* ms, long* count) {
+  return guarded([&] {
+    throw_if(!h || !ms, "null argument");
+    throw_if(!h->timer, "stage timing is off (mdft_solver_set_timing)");
+    h->timer->collect();
+    for (int i = 0; i < kNumStages; ++i) ms[i] = h->timer->acc()[i];
+    if (count) *count = h->timer->count();
+  });
+}
+
+int mdft_energy_cproj(mdft_solver_t h, const void* xi, void* df, double* ff, void* stream) {
+  return guarded([&] {
+    throw_if(!ff, "energy_cproj: null ff");
+    hipStream_t s = as_stream(stream);
+    energy_impl(h, xi, df, h ? h->ffdev : nullptr, s);
+    MDFT_HIP_CHECK(hipMemcpyAsync(ff, h->ffdev, sizeof(double), hipMemcpyDeviceToHost, s));
+    MDFT_HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int mdft_energy_cproj_async(mdft_solver_t h, const void* xi, void* df, double* ff_device, void* stream) {
+  return guarded([&] { energy_impl(h, xi, df, ff_device, as_stream(stream)); });
+}
+
+int mdft_fill_uniform(void* x, size_t n, unsigned long long seed, unsigned long long offset, double lo, double hi,
+                      int dtype, void* stream) {
+  return guarded([&] {
+    throw_if(!x && n > 0, "fill_uniform: null argument");
+    check_dtype(dtype);
+    if (dtype == MDFT_DOUBLE)
+      fill_uniform<double>(static_cast<double*>(x), n, seed, offset, lo, hi, as_stream(stream));
+    else
+      fill_uniform<float>(static_cast<float*>(x), n, seed, offset, lo, hi, as_stream(stream));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,43 @@
+// Convolution::execute on gfx950 (see mdft_conv.hip).
+#pragma once
+
+#include <memory>
+
+#include "mdft_fft.hpp"
+
+namespace mdft {
+
+struct ConvDevice {
+  ConvTables tab;
+  Fft3dPlan fft;
+  long npairs = 0;
+  double* kxyz = nullptr;     // kx | ky | kz (FP64)
+  int* pairs = nullptr;       // packed ix | iy << 10 | iz << 20, (ix, iy, iz)-ordered
+  int* ia_start = nullptr;    // np + 1
+  int* oz_src = nullptr;      // np_new: source ip of term ia | (nu2 < 0) << 16
+  int* ipinfo = nullptr;      // np: m | (mup + mmax) << 8 | mu2 << 16
+  int* pmap = nullptr;        // (mmax+1) x (2mmax+1) x nmu2
+  double2* ctab64 = nullptr;  // (nq, np_new): the OZ table, transposed
+  float2* ctab32 = nullptr;
+  double* rcoef = nullptr;    // rotation recursion coefficients a | b | c | d
+  ConvDevice(int nx, int ny, int nz, double lx, double ly, double lz, int mmax, int mrso, const LucTable& luc);
+  ~ConvDevice();
+  ConvDevice(const ConvDevice&) = delete;
+  ConvDevice& operator=(const ConvDevice&) = delete;
+
+  // the per-(q, -q) rotation / OZ / inverse-rotation pass on FFT'd data;
+  // gamma components below chop_eps in magnitude are set to 0
+  template <typename T>
+  void oz_pass(cplx<T>* p, double chop_eps, hipStream_t s) const;
+  // FFT, oz_pass, inverse FFT: in place on (np, nx, ny, nz); marks the
+  // FFT / OZ stage boundaries on `timer` when given
+  template <typename T>
+  void execute(cplx<T>* p, double chop_eps, hipStream_t s, StageTimer* timer = nullptr) const;
+};
+
+// (ix, iy, iz) triples of the reference's (q, -q) list with the same q / -q
+// designation, sorted by (ix, iy, iz) so that consecutive pairs are
+// neighbours in memory.
+std::vector<int> qpairs_memory_order(int nx, int ny, int nz);
+
+}  // namespace mdft

@@ -1,0 +1,236 @@
+// Batched unnormalised 3D c2c FFT for gfx950: one launch per axis, each a
+// mixed-radix Stockham transform of a tile of lines staged in LDS.
+//
+// Replaces the kokkos-fft plans of common/MDFT_Convolution.hpp:198-208 and
+// their execution at :328-329 (forward) and :546-547 (backward), both with
+// Normalization::none.  A tile holds W lines of length n: for the contiguous
+// axis (z) the W lines are W consecutive rows; for the strided axes (y, x) they
+// are W neighbouring z columns, so every global access is a run of W (or W*n)
+// consecutive complex values.  Each Stockham stage reads one LDS buffer and
+// writes the other (radix-4/2/3/5 butterflies, a generic odd-prime DFT up to
+// 13), with the twiddles e^{-2 pi i t/n} read from an LDS copy of an FP64
+// host table.
+#include "mdft_fft.hpp"
+
+#include <cmath>
+
+namespace mdft {
+
+namespace {
+
+constexpr int kFftThreads = 256;
+constexpr int kMaxRadix = 13;
+
+template <typename T>
+__device__ __forceinline__ cplx<T> twiddle(const cplx<T>* tw, int t, int sign) {
+  cplx<T> w = tw[t];
+  if (sign > 0) w.y = -w.y;
+  return w;
+}
+
+// in-place DFT of R values; sign -1 forward, +1 backward
+template <typename T>
+__device__ __forceinline__ void dft_small(cplx<T>* v, int R, int sign, const cplx<T>* tw, int n) {
+  if (R == 2) {
+    cplx<T> a = v[0], b = v[1];
+    v[0] = cadd<T>(a, b);
+    v[1] = csub<T>(a, b);
+  } else if (R == 4) {
+    cplx<T> s02 = cadd<T>(v[0], v[2]), d02 = csub<T>(v[0], v[2]);
+    cplx<T> s13 = cadd<T>(v[1], v[3]), d13 = csub<T>(v[1], v[3]);
+    // W = e^{sign 2 pi i/4} = sign * i
+    cplx<T> wd = mkc<T>(-static_cast<T>(sign) * d13.y, static_cast<T>(sign) * d13.x);
+    v[0] = cadd<T>(s02, s13);
+    v[2] = csub<T>(s02, s13);
+    v[1] = cadd<T>(d02, wd);
+    v[3] = csub<T>(d02, wd);
+  } else if (R == 3) {
+    const T h = static_cast<T>(0.86602540378443864676);  // sqrt(3)/2
+    cplx<T> a = v[0], t = cadd<T>(v[1], v[2]), u = csub<T>(v[1], v[2]);
+    cplx<T> m = mkc<T>(a.x - static_cast<T>(0.5) * t.x, a.y - static_cast<T>(0.5) * t.y);
+    // sign * i * h * u
+    cplx<T> r = mkc<T>(-static_cast<T>(sign) * h * u.y, static_cast<T>(sign) * h * u.x);
+    v[0] = cadd<T>(a, t);
+    v[1] = cadd<T>(m, r);
+    v[2] = csub<T>(m, r);
+  } else {
+    cplx<T> out[kMaxRadix];
+    const int step = n / R;
+    for (int s = 0; s < R; ++s) {
+      cplx<T> acc = v[0];
+      for (int r = 1; r < R; ++r) acc = cfma<T>(acc, v[r], twiddle<T>(tw, ((r * s) % R) * step, sign));
+      out[s] = acc;
+    }
+    for (int s = 0; s < R; ++s) v[s] = out[s];
+  }
+}
+
+// One block transforms a tile of W lines of length n.
+//   STRIDED = false: line l occupies data[l*n .. l*n+n); the tile is lines
+//                    [tile*W, tile*W + W) (clipped at nlines).
+//   STRIDED = true : element e of line (o, i) sits at data[(o*n + e)*S + i];
+//                    tile = (o, chunk) covers i in [chunk*W, chunk*W + W).
+template <typename T, bool STRIDED>
+__global__ __launch_bounds__(kFftThreads) void fft_axis_kernel(cplx<T>* __restrict__ data, FftStages st,
+                                                               const cplx<T>* __restrict__ tw_g, int sign,
+                                                               long nlines_or_outer, long S, int W) {
+  extern __shared__ unsigned char lds_raw[];
+  const int n = st.n;
+  const int ld = n + 1;  // padded row: avoids bank conflicts on column access
+  cplx<T>* bufA = reinterpret_cast<cplx<T>*>(lds_raw);
+  cplx<T>* bufB = bufA + W * ld;
+  cplx<T>* tw = bufB + W * ld;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < n; i += blockDim.x) tw[i] = tw_g[i];
+
+  long base;
+  int wvalid;
+  if (!STRIDED) {
+    const long l0 = static_cast<long>(blockIdx.x) * W;
+    wvalid = static_cast<int>(min(static_cast<long>(W), nlines_or_outer - l0));
+    base = l0 * n;
+    for (int t = tid; t < wvalid * n; t += blockDim.x) {
+      const int w = t / n, e = t - w * n;
+      bufA[w * ld + e] = data[base + t];
+    }
+  } else {
+    const long nchunk = (S + W - 1) / W;
+    const long o = blockIdx.x / nchunk;
+    const long i0 = (blockIdx.x - o * nchunk) * W;
+    wvalid = static_cast<int>(min(static_cast<long>(W), S - i0));
+    base = o * n * S + i0;
+    for (int t = tid; t < wvalid * n; t += blockDim.x) {
+      const int e = t / wvalid, w = t - e * wvalid;
+      bufA[w * ld + e] = data[base + static_cast<long>(e) * S + w];
+    }
+  }
+  __syncthreads();
+
+  cplx<T>* src = bufA;
+  cplx<T>* dst = bufB;
+  int Ns = 1;
+  for (int s = 0; s < st.nstages; ++s) {
+    const int R = st.radix[s];
+    const int nb = n / R;
+    const int tstep = n / (Ns * R);
+    for (int beta = tid; beta < wvalid * nb; beta += blockDim.x) {
+      const int w = beta / nb, j = beta - w * nb;
+      const int k = j % Ns;
+      cplx<T> v[kMaxRadix];
+      const cplx<T>* in = src + w * ld + j;
+      v[0] = in[0];
+      for (int r = 1; r < R; ++r) v[r] = cmul<T>(in[r * nb], twiddle<T>(tw, (r * k * tstep) % n, sign));
+      dft_small<T>(v, R, sign, tw, n);
+      cplx<T>* out = dst + w * ld + (j / Ns) * Ns * R + k;
+      for (int r = 0; r < R; ++r) out[r * Ns] = v[r];
+    }
+    __syncthreads();
+    cplx<T>* t = src;
+    src = dst;
+    dst = t;
+    Ns *= R;
+  }
+
+  if (!STRIDED) {
+    for (int t = tid; t < wvalid * n; t += blockDim.x) {
+      const int w = t / n, e = t - w * n;
+      data[base + t] = src[w * ld + e];
+    }
+  } else {
+    for (int t = tid; t < wvalid * n; t += blockDim.x) {
+      const int e = t / wvalid, w = t - e * wvalid;
+      data[base + static_cast<long>(e) * S + w] = src[w * ld + e];
+    }
+  }
+}
+
+int choose_tile_lines(int n) {
+  int W = 1024 / n;
+  if (W < 1) W = 1;
+  if (W > 64) W = 64;
+  return W;
+}
+
+template <typename T>
+void launch_axis(cplx<T>* data, const FftPlan1D& p, int sign, long nlines_or_outer, long S, bool strided,
+                 hipStream_t stream) {
+  if (p.st.n == 1 || nlines_or_outer == 0) return;
+  int W = choose_tile_lines(p.st.n);
+  if (strided && S < W) W = static_cast<int>(S);
+  const size_t lds = (2 * static_cast<size_t>(W) * (p.st.n + 1) + p.st.n) * sizeof(cplx<T>);
+  throw_if(lds > 160 * 1024, "FFT tile does not fit in LDS");
+  const cplx<T>* tw = p.twiddles<T>();
+  if (!strided) {
+    const long nblk = (nlines_or_outer + W - 1) / W;
+    throw_if(nblk > 0x7fffffffL, "too many FFT tiles");
+    hipLaunchKernelGGL((fft_axis_kernel<T, false>), dim3(static_cast<unsigned>(nblk)), dim3(kFftThreads), lds,
+                       stream, data, p.st, tw, sign, nlines_or_outer, S, W);
+  } else {
+    const long nblk = nlines_or_outer * ((S + W - 1) / W);
+    throw_if(nblk > 0x7fffffffL, "too many FFT tiles");
+    hipLaunchKernelGGL((fft_axis_kernel<T, true>), dim3(static_cast<unsigned>(nblk)), dim3(kFftThreads), lds,
+                       stream, data, p.st, tw, sign, nlines_or_outer, S, W);
+  }
+  MDFT_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+FftPlan1D::FftPlan1D(int n) {
+  throw_if(n <= 0, "FFT length must be positive");
+  throw_if(n > 4096, "FFT length above 4096 is not supported");
+  st.n = n;
+  st.nstages = 0;
+  int r = n;
+  for (int p : {4, 2, 3, 5, 7, 11, 13}) {
+    while (r % p == 0) {
+      throw_if(st.nstages >= kFftMaxStages, "too many FFT stages");
+      st.radix[st.nstages++] = p;
+      r /= p;
+    }
+  }
+  throw_if(r != 1, "FFT length must factor into primes <= 13");
+  std::vector<double2> h64(n);
+  std::vector<float2> h32(n);
+  for (int t = 0; t < n; ++t) {
+    const double a = 2.0 * M_PI * static_cast<double>(t) / static_cast<double>(n);
+    h64[t].x = std::cos(a);
+    h64[t].y = -std::sin(a);
+    h32[t].x = static_cast<float>(h64[t].x);
+    h32[t].y = static_cast<float>(h64[t].y);
+  }
+  MDFT_HIP_CHECK(hipMalloc(&tw64_, n * sizeof(double2)));
+  MDFT_HIP_CHECK(hipMalloc(&tw32_, n * sizeof(float2)));
+  MDFT_HIP_CHECK(hipMemcpy(tw64_, h64.data(), n * sizeof(double2), hipMemcpyHostToDevice));
+  MDFT_HIP_CHECK(hipMemcpy(tw32_, h32.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+}
+
+FftPlan1D::~FftPlan1D() {
+  if (tw64_) (void)hipFree(tw64_);
+  if (tw32_) (void)hipFree(tw32_);
+}
+
+template <>
+const cplx<double>* FftPlan1D::twiddles<double>() const {
+  return tw64_;
+}
+template <>
+const cplx<float>* FftPlan1D::twiddles<float>() const {
+  return tw32_;
+}
+
+Fft3dPlan::Fft3dPlan(int nx_, int ny_, int nz_) : nx(nx_), ny(ny_), nz(nz_), px(nx_), py(ny_), pz(nz_) {}
+
+template <typename T>
+void Fft3dPlan::execute(cplx<T>* data, long batch, int sign, hipStream_t stream) const {
+  throw_if(sign != -1 && sign != 1, "FFT sign must be -1 or +1");
+  const long nxl = nx, nyl = ny, nzl = nz;
+  launch_axis<T>(data, pz, sign, batch * nxl * nyl, 1, false, stream);   // z: contiguous lines
+  launch_axis<T>(data, py, sign, batch * nxl, nzl, true, stream);        // y: stride nz
+  launch_axis<T>(data, px, sign, batch, nyl * nzl, true, stream);        // x: stride ny*nz
+}
+
+template void Fft3dPlan::execute<double>(cplx<double>*, long, int, hipStream_t) const;
+template void Fft3dPlan::execute<float>(cplx<float>*, long, int, hipStream_t) const;
+
+}  // namespace mdft

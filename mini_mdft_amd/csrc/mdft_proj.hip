@@ -1,0 +1,486 @@
+// Orientation <-> projection transforms in the factorised form, for the FP64
+// and FP32 precision forms, plus delta_rho / delta_f and their deterministic
+// FP64 reductions (gfx950).
+//
+// angl2proj (common/MDFT_OrientationProjectionTransform.hpp:267-380) is an
+// unnormalised rfft2 over (phi, psi), a conjugation / (nphi npsi), the mup ->
+// kphi unscramble and a theta quadrature.  Because the field is real, that
+// equals
+//   p[ip](r) = sum_theta W[theta][ip] Y[theta][mup][mu2],
+//   Y = sum_j e^{+2 pi i mup j/nphi} sum_k e^{+2 pi i mu2 k/npsi} o[theta][j][k],
+// W = f_m w_theta d^m_{mup,mu}(theta) / (nphi npsi).  One block handles RT
+// consecutive r: the RT x no input tile is staged in LDS (coalesced), then the
+// psi sums X (one thread per (r, theta, j, mu2)), the phi sums Y (one thread
+// per (r, theta, mup, mu2)) and the theta sums (one thread per (ip, r), r
+// fastest so every ip row is written as a run of RT values).
+//
+// proj2angl (:388-484) runs the same three steps backwards: S = conj(sum_m p d
+// f_m), Z = sum_mup S e^{+2 pi i mup j/nphi}, and the c2r over psi with only
+// Re of the DC bin and twice Re of the others (npsi = 2 mmax/mrso + 1 is odd,
+// so there is no Nyquist bin).  Optionally get_delta_f
+// (common/MDFT_Solvent.hpp:613-660) is fused into the last step.
+#include "mdft_proj.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace mdft {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kRT = 8;  // r values per block
+
+struct TileDims {
+  int ntheta, nphi, npsi, nmu2, no, np, mmax;
+};
+
+template <typename TC>
+struct TabPtrs {
+  const cplx<TC>* ephi;
+  const cplx<TC>* epsi;
+  const TC* wts;
+  const int* ipinfo;
+  const int* yidx;
+};
+
+__device__ __forceinline__ int pmod(int a, int n) {
+  int r = a % n;
+  return r < 0 ? r + n : r;
+}
+
+template <typename TC>
+__device__ __forceinline__ void block_sum_store(double v, double* out) {
+  __shared__ double red[kThreads];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+template <typename TI, typename TP, typename TC>
+__global__ __launch_bounds__(kThreads) void angl2proj_kernel(const TI* __restrict__ o, cplx<TP>* __restrict__ p,
+                                                             long N, TileDims d, TabPtrs<TC> tb, int fuse,
+                                                             double rho0) {
+  extern __shared__ unsigned char lds_raw[];
+  const int cells = kRT * d.ntheta * d.nphi * d.nmu2;  // X / Y entries per tile
+  const int regA = max(cells * 2, kRT * d.no);          // in TC units
+  TC* A = reinterpret_cast<TC*>(lds_raw);               // o tile, then Y
+  cplx<TC>* X = reinterpret_cast<cplx<TC>*>(A + regA);  // psi sums
+  cplx<TC>* Y = reinterpret_cast<cplx<TC>*>(A);
+  cplx<TC>* ephi = X + cells;
+  cplx<TC>* epsi = ephi + d.nphi;
+  TC* wts = reinterpret_cast<TC*>(epsi + d.nmu2 * d.npsi);
+
+  const int tid = threadIdx.x;
+  const long r0 = static_cast<long>(blockIdx.x) * kRT;
+  const int rv = static_cast<int>(min(static_cast<long>(kRT), N - r0));
+
+  for (int i = tid; i < d.nphi; i += kThreads) ephi[i] = tb.ephi[i];
+  for (int i = tid; i < d.nmu2 * d.npsi; i += kThreads) epsi[i] = tb.epsi[i];
+  for (int i = tid; i < d.ntheta * d.np; i += kThreads) wts[i] = tb.wts[i];
+  const TI* src = o + r0 * d.no;
+  for (int i = tid; i < rv * d.no; i += kThreads) {
+    if (fuse) {
+      const double x = static_cast<double>(src[i]);
+      A[i] = static_cast<TC>(rho0 * (x * x - 1.0));
+    } else {
+      A[i] = static_cast<TC>(src[i]);
+    }
+  }
+  __syncthreads();
+
+  // X[r][theta][j][mu2] = sum_k o[r][theta][j][k] e^{+2 pi i mu2 k/npsi}
+  const int nx_items = rv * d.ntheta * d.nphi * d.nmu2;
+  for (int it = tid; it < nx_items; it += kThreads) {
+    const int mu2 = it % d.nmu2;
+    const int rtj = it / d.nmu2;  // (r*ntheta + theta)*nphi + j
+    const TC* row = A + (rtj / (d.ntheta * d.nphi)) * d.no + (rtj % (d.ntheta * d.nphi)) * d.npsi;
+    const cplx<TC>* e = epsi + mu2 * d.npsi;
+    TC re = 0, im = 0;
+    for (int k = 0; k < d.npsi; ++k) {
+      re = fma(row[k], e[k].x, re);
+      im = fma(row[k], e[k].y, im);
+    }
+    X[it] = mkc<TC>(re, im);
+  }
+  __syncthreads();
+
+  // Y[r][theta][mup][mu2] = sum_j X[r][theta][j][mu2] e^{+2 pi i mup j/nphi}
+  for (int it = tid; it < nx_items; it += kThreads) {
+    const int mu2 = it % d.nmu2;
+    const int rtm = it / d.nmu2;
+    const int imup = rtm % d.nphi;
+    const int rt = rtm / d.nphi;
+    const int mup = imup - d.mmax;
+    const cplx<TC>* xs = X + rt * d.nphi * d.nmu2 + mu2;
+    cplx<TC> acc = mkc<TC>(0, 0);
+    for (int j = 0; j < d.nphi; ++j) acc = cfma<TC>(acc, xs[j * d.nmu2], ephi[pmod(mup * j, d.nphi)]);
+    Y[it] = acc;
+  }
+  __syncthreads();
+
+  // p[ip][r] = sum_theta W[theta][ip] Y[r][theta][mup(ip)][mu2(ip)]
+  for (int it = tid; it < d.np * rv; it += kThreads) {
+    const int ip = it / rv, r = it - ip * rv;
+    const int info = tb.ipinfo[ip];
+    const int imup = info & 0xff, mu2 = (info >> 8) & 0xff;
+    const cplx<TC>* ys = Y + (r * d.ntheta * d.nphi + imup) * d.nmu2 + mu2;
+    TC re = 0, im = 0;
+    for (int th = 0; th < d.ntheta; ++th) {
+      const TC wv = wts[th * d.np + ip];
+      const cplx<TC> y = ys[th * d.nphi * d.nmu2];
+      re = fma(wv, y.x, re);
+      im = fma(wv, y.y, im);
+    }
+    p[static_cast<long>(ip) * N + r0 + r] = mkc<TP>(static_cast<TP>(re), static_cast<TP>(im));
+  }
+}
+
+template <typename TP, typename TO, typename TC>
+__global__ __launch_bounds__(kThreads) void proj2angl_kernel(const cplx<TP>* __restrict__ p, TO* __restrict__ o,
+                                                             const TO* __restrict__ xi, TO* __restrict__ df,
+                                                             const double* __restrict__ wq, long N, TileDims d,
+                                                             TabPtrs<TC> tb, double rho0,
+                                                             double* __restrict__ partials) {
+  extern __shared__ unsigned char lds_raw[];
+  const int cells = kRT * d.ntheta * d.nphi * d.nmu2;
+  const int regA = max(cells, kRT * d.np);  // complex entries
+  cplx<TC>* A = reinterpret_cast<cplx<TC>*>(lds_raw);  // p tile, then Z
+  cplx<TC>* S = A + regA;
+  cplx<TC>* ephi = S + cells;
+  cplx<TC>* epsi = ephi + d.nphi;
+  TC* wts = reinterpret_cast<TC*>(epsi + d.nmu2 * d.npsi);
+
+  const int tid = threadIdx.x;
+  const long r0 = static_cast<long>(blockIdx.x) * kRT;
+  const int rv = static_cast<int>(min(static_cast<long>(kRT), N - r0));
+
+  for (int i = tid; i < d.nphi; i += kThreads) ephi[i] = tb.ephi[i];
+  for (int i = tid; i < d.nmu2 * d.npsi; i += kThreads) epsi[i] = tb.epsi[i];
+  for (int i = tid; i < d.ntheta * d.np; i += kThreads) wts[i] = tb.wts[i];
+  for (int it = tid; it < d.np * rv; it += kThreads) {
+    const int ip = it / rv, r = it - ip * rv;
+    const cplx<TP> v = p[static_cast<long>(ip) * N + r0 + r];
+    A[ip * kRT + r] = mkc<TC>(static_cast<TC>(v.x), static_cast<TC>(v.y));
+  }
+  __syncthreads();
+
+  // S[r][theta][mup][mu2] = conj(sum_{m >= max(|mup|, mrso mu2)} p[P(m,mup,mu2)] d f_m)
+  const int n_items = rv * d.ntheta * d.nphi * d.nmu2;
+  for (int it = tid; it < n_items; it += kThreads) {
+    const int mu2 = it % d.nmu2;
+    const int rtm = it / d.nmu2;
+    const int imup = rtm % d.nphi;
+    const int rt = rtm / d.nphi;
+    const int th = rt % d.ntheta, r = rt / d.ntheta;
+    const int* ys = tb.yidx + (imup * d.nmu2 + mu2) * (d.mmax + 1);
+    TC re = 0, im = 0;
+    for (int m = 0; m <= d.mmax; ++m) {
+      const int ip = ys[m];
+      if (ip < 0) continue;
+      const TC wv = wts[th * d.np + ip];
+      const cplx<TC> v = A[ip * kRT + r];
+      re = fma(v.x, wv, re);
+      im = fma(v.y, wv, im);
+    }
+    S[it] = mkc<TC>(re, -im);
+  }
+  __syncthreads();
+
+  // Z[r][theta][j][mu2] = sum_mup S[r][theta][mup][mu2] e^{+2 pi i mup j/nphi}
+  cplx<TC>* Z = A;
+  for (int it = tid; it < n_items; it += kThreads) {
+    const int mu2 = it % d.nmu2;
+    const int rtj = it / d.nmu2;
+    const int j = rtj % d.nphi;
+    const int rt = rtj / d.nphi;
+    const cplx<TC>* ss = S + rt * d.nphi * d.nmu2 + mu2;
+    cplx<TC> acc = mkc<TC>(0, 0);
+    for (int imup = 0; imup < d.nphi; ++imup)
+      acc = cfma<TC>(acc, ss[imup * d.nmu2], ephi[pmod((imup - d.mmax) * j, d.nphi)]);
+    Z[it] = acc;
+  }
+  __syncthreads();
+
+  // o[r][theta][j][k] = Re Z[..][0] + 2 sum_{mu2 >= 1} Re(Z[..][mu2] e^{+2 pi i mu2 k/npsi})
+  double acc_ff = 0.0;
+  for (int it = tid; it < rv * d.no; it += kThreads) {
+    const int io = it % d.no, r = it / d.no;
+    const int k = io % d.npsi, tj = io / d.npsi;
+    const cplx<TC>* zs = Z + (r * d.ntheta * d.nphi + tj) * d.nmu2;
+    TC v = zs[0].x;
+    for (int mu2 = 1; mu2 < d.nmu2; ++mu2) {
+      const cplx<TC> e = epsi[mu2 * d.npsi + k];
+      const cplx<TC> z = zs[mu2];
+      v += static_cast<TC>(2) * (z.x * e.x - z.y * e.y);
+    }
+    const long g = (r0 + r) * d.no + io;
+    if (xi == nullptr) {
+      o[g] = static_cast<TO>(v);
+    } else {
+      const double x = static_cast<double>(xi[g]);
+      const double vx = static_cast<double>(static_cast<TO>(v));
+      acc_ff += vx * wq[io] * (x * x - 1.0);
+      df[g] = static_cast<TO>(2.0 * rho0 * x * vx);
+    }
+  }
+  if (xi != nullptr) block_sum_store<TC>(acc_ff, partials + blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void delta_rho_kernel(const T* __restrict__ xi, T* __restrict__ out, size_t n,
+                                                             T rho0) {
+  for (size_t i = blockIdx.x * static_cast<size_t>(kThreads) + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * kThreads) {
+    const T x = xi[i];
+    out[i] = rho0 * (x * x - static_cast<T>(1.0));
+  }
+}
+
+constexpr int kDfRows = 8;
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void delta_f_kernel(const T* __restrict__ xi, const T* __restrict__ vexc,
+                                                           const double* __restrict__ w, T* __restrict__ df,
+                                                           long nxyz, int no, double rho0,
+                                                           double* __restrict__ partials) {
+  const long r0 = static_cast<long>(blockIdx.x) * kDfRows;
+  const long rv = min(static_cast<long>(kDfRows), nxyz - r0);
+  const long base = r0 * no;
+  double acc = 0.0;
+  for (long i = threadIdx.x; i < rv * no; i += kThreads) {
+    const int io = static_cast<int>(i % no);
+    const double x = static_cast<double>(xi[base + i]);
+    const double v = static_cast<double>(vexc[base + i]);
+    acc += v * w[io] * (x * x - 1.0);
+    df[base + i] = static_cast<T>(2.0 * rho0 * x * v);
+  }
+  block_sum_store<double>(acc, partials + blockIdx.x);
+}
+
+__global__ __launch_bounds__(kThreads) void reduce_partials_kernel(const double* __restrict__ partials, int count,
+                                                                   double scale, double* __restrict__ out) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < count; i += kThreads) acc += partials[i];
+  __shared__ double red[kThreads];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = scale * red[0];
+}
+
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void fill_uniform_kernel(T* __restrict__ x, size_t n, unsigned long long key,
+                                                                unsigned long long offset, double lo, double hi) {
+#pragma clang fp contract(off)  // lo + (hi - lo) * u rounded twice, as numpy does
+  for (size_t i = blockIdx.x * static_cast<size_t>(kThreads) + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * kThreads) {
+    const unsigned long long z = splitmix64(key + offset + i);
+    const double u = static_cast<double>(z >> 11) * (1.0 / 9007199254740992.0);
+    x[i] = static_cast<T>(lo + (hi - lo) * u);
+  }
+}
+
+unsigned long long splitmix64_host(unsigned long long z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+TileDims dims_of(const ProjTables& t) {
+  return TileDims{t.ntheta, t.nphi, t.npsi, t.nmu2, t.no, t.np, t.mmax};
+}
+
+template <typename TC>
+TabPtrs<TC> tabs_of(const ProjTables& t, bool forward);
+template <>
+TabPtrs<double> tabs_of<double>(const ProjTables& t, bool forward) {
+  return TabPtrs<double>{t.ephi64, t.epsi64, forward ? t.wfwd64 : t.wbwd64, t.ipinfo, t.yidx};
+}
+template <>
+TabPtrs<float> tabs_of<float>(const ProjTables& t, bool forward) {
+  return TabPtrs<float>{t.ephi32, t.epsi32, forward ? t.wfwd32 : t.wbwd32, t.ipinfo, t.yidx};
+}
+
+template <typename TC>
+size_t a2p_lds(const ProjTables& t) {
+  const size_t cells = static_cast<size_t>(kRT) * t.ntheta * t.nphi * t.nmu2;
+  const size_t regA = std::max(cells * 2, static_cast<size_t>(kRT) * t.no);
+  return regA * sizeof(TC) + (cells + t.nphi + t.nmu2 * t.npsi) * sizeof(cplx<TC>) +
+         static_cast<size_t>(t.ntheta) * t.np * sizeof(TC);
+}
+
+template <typename TC>
+size_t p2a_lds(const ProjTables& t) {
+  const size_t cells = static_cast<size_t>(kRT) * t.ntheta * t.nphi * t.nmu2;
+  const size_t regA = std::max(cells, static_cast<size_t>(kRT) * t.np);
+  return (regA + cells + t.nphi + t.nmu2 * t.npsi) * sizeof(cplx<TC>) +
+         static_cast<size_t>(t.ntheta) * t.np * sizeof(TC);
+}
+
+template <typename T>
+T* upload(const std::vector<T>& h) {
+  T* d = nullptr;
+  MDFT_HIP_CHECK(hipMalloc(&d, h.size() * sizeof(T)));
+  MDFT_HIP_CHECK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return d;
+}
+
+}  // namespace
+
+ProjTables::ProjTables(const AngularGrid& g)
+    : mmax(g.mmax), mrso(g.mrso), ntheta(g.ntheta), nphi(g.nphi), npsi(g.npsi), nmu2(g.nmu2), no(g.no), np(g.np) {
+  throw_if(npsi % 2 == 0, "npsi must be odd");
+  throw_if(nphi > 255 || nmu2 > 255, "angular grid too large for the packed index table");
+  std::vector<double2> eph(nphi), eps(static_cast<size_t>(nmu2) * npsi);
+  for (int t = 0; t < nphi; ++t) {
+    const double a = 2.0 * M_PI * t / nphi;
+    eph[t].x = std::cos(a);
+    eph[t].y = std::sin(a);
+  }
+  for (int m2 = 0; m2 < nmu2; ++m2)
+    for (int k = 0; k < npsi; ++k) {
+      const double a = 2.0 * M_PI * static_cast<double>((m2 * k) % npsi) / npsi;
+      eps[m2 * npsi + k].x = std::cos(a);
+      eps[m2 * npsi + k].y = std::sin(a);
+    }
+  std::vector<double> wf(static_cast<size_t>(ntheta) * np), wb(static_cast<size_t>(ntheta) * np);
+  const double inv = 1.0 / static_cast<double>(nphi * npsi);
+  for (int th = 0; th < ntheta; ++th)
+    for (int ip = 0; ip < np; ++ip) {
+      const double dv = g.wigner_d[th * np + ip], fm = g.fm[g.m[ip]];
+      wf[th * np + ip] = fm * g.wtheta[th] * dv * inv;
+      wb[th * np + ip] = dv * fm;
+    }
+  std::vector<int> info(np), yi(static_cast<size_t>(nphi) * nmu2 * (mmax + 1), -1);
+  for (int ip = 0; ip < np; ++ip) {
+    info[ip] = (g.mup[ip] + mmax) | (g.mu2[ip] << 8) | (g.m[ip] << 16);
+    yi[((g.mup[ip] + mmax) * nmu2 + g.mu2[ip]) * (mmax + 1) + g.m[ip]] = ip;
+  }
+  std::vector<float2> eph32(eph.size()), eps32(eps.size());
+  for (size_t i = 0; i < eph.size(); ++i) eph32[i] = make_float2(static_cast<float>(eph[i].x), static_cast<float>(eph[i].y));
+  for (size_t i = 0; i < eps.size(); ++i) eps32[i] = make_float2(static_cast<float>(eps[i].x), static_cast<float>(eps[i].y));
+  std::vector<float> wf32(wf.begin(), wf.end()), wb32(wb.begin(), wb.end());
+  ephi64 = upload(eph);
+  epsi64 = upload(eps);
+  ephi32 = upload(eph32);
+  epsi32 = upload(eps32);
+  wfwd64 = upload(wf);
+  wbwd64 = upload(wb);
+  wfwd32 = upload(wf32);
+  wbwd32 = upload(wb32);
+  ipinfo = upload(info);
+  yidx = upload(yi);
+  w = upload(g.w);
+}
+
+ProjTables::~ProjTables() {
+  for (void* q : {static_cast<void*>(ephi64), static_cast<void*>(epsi64), static_cast<void*>(ephi32),
+                  static_cast<void*>(epsi32), static_cast<void*>(wfwd64), static_cast<void*>(wbwd64),
+                  static_cast<void*>(wfwd32), static_cast<void*>(wbwd32), static_cast<void*>(ipinfo),
+                  static_cast<void*>(yidx), static_cast<void*>(w)})
+    if (q) (void)hipFree(q);
+}
+
+template <typename TI, typename TP, typename TC>
+void angl2proj_factorized(const ProjTables& t, long N, const TI* o, cplx<TP>* p, const double* drho_rho0,
+                          hipStream_t s) {
+  if (N == 0) return;
+  const long nblk = (N + kRT - 1) / kRT;
+  throw_if(nblk > 0x7fffffffL, "grid too large");
+  const size_t lds = a2p_lds<TC>(t);
+  throw_if(lds > 160 * 1024, "angular grid too large for the projection tile");
+  hipLaunchKernelGGL((angl2proj_kernel<TI, TP, TC>), dim3(static_cast<unsigned>(nblk)), dim3(kThreads), lds, s, o, p,
+                     N, dims_of(t), tabs_of<TC>(t, true), drho_rho0 ? 1 : 0, drho_rho0 ? *drho_rho0 : 0.0);
+  MDFT_HIP_CHECK(hipGetLastError());
+}
+
+int proj2angl_blocks(const ProjTables&, long N) { return static_cast<int>((N + kRT - 1) / kRT); }
+
+template <typename TP, typename TO, typename TC>
+void proj2angl_factorized(const ProjTables& t, long N, const cplx<TP>* p, TO* o, const TO* xi, TO* df,
+                          double rho0, double* partials, hipStream_t s) {
+  if (N == 0) return;
+  const long nblk = (N + kRT - 1) / kRT;
+  throw_if(nblk > 0x7fffffffL, "grid too large");
+  const size_t lds = p2a_lds<TC>(t);
+  throw_if(lds > 160 * 1024, "angular grid too large for the projection tile");
+  hipLaunchKernelGGL((proj2angl_kernel<TP, TO, TC>), dim3(static_cast<unsigned>(nblk)), dim3(kThreads), lds, s, p, o,
+                     xi, df, t.w, N, dims_of(t), tabs_of<TC>(t, false), rho0, partials);
+  MDFT_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void delta_rho(const T* xi, T* out, size_t n, double rho0, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL((delta_rho_kernel<T>), dim3(grid_for(n, kThreads, 65536)), dim3(kThreads), 0, s, xi, out, n,
+                     static_cast<T>(rho0));
+  MDFT_HIP_CHECK(hipGetLastError());
+}
+
+int delta_f_blocks(long nxyz) { return static_cast<int>((nxyz + kDfRows - 1) / kDfRows); }
+
+template <typename T>
+int delta_f(const T* xi, const T* vexc, const double* w, T* df, long nxyz, int no, double rho0, double* partials,
+            int max_partials, hipStream_t s) {
+  const int nb = delta_f_blocks(nxyz);
+  throw_if(nb > max_partials, "partials buffer too small");
+  if (nb == 0) return 0;
+  hipLaunchKernelGGL((delta_f_kernel<T>), dim3(nb), dim3(kThreads), 0, s, xi, vexc, w, df, nxyz, no, rho0, partials);
+  MDFT_HIP_CHECK(hipGetLastError());
+  return nb;
+}
+
+void reduce_partials(const double* partials, int count, double scale, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(kThreads), 0, s, partials, count, scale, out);
+  MDFT_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void fill_uniform(T* x, size_t n, unsigned long long seed, unsigned long long offset, double lo, double hi,
+                  hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL((fill_uniform_kernel<T>), dim3(grid_for(n, kThreads, 65536)), dim3(kThreads), 0, s, x, n,
+                     splitmix64_host(seed), offset, lo, hi);
+  MDFT_HIP_CHECK(hipGetLastError());
+}
+
+template void angl2proj_factorized<double, double, double>(const ProjTables&, long, const double*, cplx<double>*,
+                                                           const double*, hipStream_t);
+template void angl2proj_factorized<float, float, float>(const ProjTables&, long, const float*, cplx<float>*,
+                                                        const double*, hipStream_t);
+template void angl2proj_factorized<double, float, float>(const ProjTables&, long, const double*, cplx<float>*,
+                                                         const double*, hipStream_t);
+template void proj2angl_factorized<double, double, double>(const ProjTables&, long, const cplx<double>*, double*,
+                                                           const double*, double*, double, double*, hipStream_t);
+template void proj2angl_factorized<float, float, float>(const ProjTables&, long, const cplx<float>*, float*,
+                                                        const float*, float*, double, double*, hipStream_t);
+template void proj2angl_factorized<float, double, float>(const ProjTables&, long, const cplx<float>*, double*,
+                                                         const double*, double*, double, double*, hipStream_t);
+template void delta_rho<double>(const double*, double*, size_t, double, hipStream_t);
+template void delta_rho<float>(const float*, float*, size_t, double, hipStream_t);
+template int delta_f<double>(const double*, const double*, const double*, double*, long, int, double, double*, int,
+                             hipStream_t);
+template int delta_f<float>(const float*, const float*, const double*, float*, long, int, double, double*, int,
+                            hipStream_t);
+template void fill_uniform<double>(double*, size_t, unsigned long long, unsigned long long, double, double,
+                                   hipStream_t);
+template void fill_uniform<float>(float*, size_t, unsigned long long, unsigned long long, double, double,
+                                  hipStream_t);
+
+}  // namespace mdft

@@ -33,6 +33,9 @@ def uniform(seed: int, n: int, offset: int = 0) -> np.ndarray:
 
 
 def xi_field(seed: int, shape, amp: float = 0.1) -> np.ndarray:
-    """xi = 1 + amp * U(-1, 1), the synthetic density field of the benchmarks."""
+    """xi = 1 + amp * U(-1, 1), the synthetic density field of the benchmarks,
+    written as lo + (hi - lo) u with lo = 1 - amp, hi = 1 + amp (the formula of
+    mdft_fill_uniform, so the GPU fill gives the same doubles)."""
     n = int(np.prod(shape))
-    return (1.0 + amp * (2.0 * uniform(seed, n) - 1.0)).reshape(shape)
+    lo, hi = 1.0 - amp, 1.0 + amp
+    return (lo + (hi - lo) * uniform(seed, n)).reshape(shape)

@@ -1,0 +1,246 @@
+"""GPU parity of the HIP path (through the C ABI) against the FP64 oracle."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from mini_mdft_amd import FP32, FP64, MIXED, luc, rng
+from mini_mdft_amd import mdft as M
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+MRSO = 2
+DQ = luc.DQ_TIP3P
+
+
+def _dev():
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return torch.device("cuda:0")
+
+
+def _to(x, dtype):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(device=_dev(), dtype=dtype)
+
+
+def _np(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+def _problem(oracle, n, L, mmax, seed=1):
+    g = M.SpatialGrid(n, L)
+    ag = M.AngularGrid(mmax, MRSO)
+    mp = M.OrientationProjectionMap(ag)
+    qmax = luc.qmax_of_grid(n, L)
+    tab = luc.synthetic_c(mmax, MRSO, luc.nq_for(qmax, DQ), seed=seed)
+    ocv = oracle.Conv(n, L, mmax, MRSO, tab.tuples, tab.c, tab.dq)
+    return g, ag, mp, tab, ocv
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("shape", [(2, 5, 5, 5), (3, 4, 6, 5), (2, 8, 8, 8), (1, 16, 12, 10), (2, 32, 32, 32),
+                                   (1, 64, 64, 64), (1, 96, 48, 80), (2, 7, 11, 13), (1, 1, 1, 1)])
+@pytest.mark.parametrize("prec", [FP64, FP32])
+def test_fft3d(shape, prec):
+    r = np.random.default_rng(0)
+    x = r.standard_normal(shape) + 1j * r.standard_normal(shape)
+    ct = torch.complex128 if prec == FP64 else torch.complex64
+    tol = 1e-13 if prec == FP64 else 2e-6
+    t = _to(x, ct)
+    M.fft3d(t, -1)
+    assert rel_err(_np(t), np.fft.fftn(x, axes=(-3, -2, -1))) < tol * math.log2(max(2, np.prod(shape[1:])))
+    t = _to(x, ct)
+    M.fft3d(t, +1)
+    N = np.prod(shape[1:])
+    assert rel_err(_np(t), N * np.fft.ifftn(x, axes=(-3, -2, -1))) < tol * math.log2(max(2, N))
+
+
+# ---------------------------------------------------------------------------
+CASES = [((2, 3, 4), 3), ((2, 2, 3), 5), ((3, 3, 3), 1), ((2, 2, 2), 10), ((5, 4, 3), 4)]
+
+
+@pytest.mark.parametrize("n,mmax", CASES)
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-13), (FP32, 2e-6), (MIXED, 2e-6)])
+def test_angl2proj(oracle, n, mmax, prec, tol):
+    mrso = MRSO if mmax != 10 else 10
+    g = M.SpatialGrid(n, (1.0, 1.0, 1.0))
+    ag = M.AngularGrid(mmax, mrso)
+    opt = M.OrientationProjectionTransform(g, ag, M.OrientationProjectionMap(ag), prec)
+    r = np.random.default_rng(2)
+    o = r.random((g.N, ag.m_ntheta, ag.m_nphi, ag.m_npsi))
+    ref = oracle.angl2proj(o, n, mmax, mrso)
+    ot = _to(o, opt.real_dtype)
+    p = torch.zeros((ag.m_np,) + n, dtype=opt.complex_dtype, device=_dev())
+    opt.angl2proj(ot, p)
+    assert rel_err(_np(p), ref) < tol
+
+
+@pytest.mark.parametrize("n,mmax", CASES)
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-13), (FP32, 2e-6), (MIXED, 2e-6)])
+def test_proj2angl(oracle, n, mmax, prec, tol):
+    mrso = MRSO if mmax != 10 else 10
+    g = M.SpatialGrid(n, (1.0, 1.0, 1.0))
+    ag = M.AngularGrid(mmax, mrso)
+    opt = M.OrientationProjectionTransform(g, ag, M.OrientationProjectionMap(ag), prec)
+    r = np.random.default_rng(3)
+    # random complex projections (non-Hermitian mu2 = 0 columns included)
+    p = r.standard_normal((ag.m_np,) + n) + 1j * r.standard_normal((ag.m_np,) + n)
+    ref = oracle.proj2angl(p, mmax, mrso)
+    o = torch.zeros((g.N, ag.m_ntheta, ag.m_nphi, ag.m_npsi), dtype=opt.real_dtype, device=_dev())
+    opt.proj2angl(_to(p, opt.complex_dtype), o)
+    assert rel_err(_np(o), ref) < tol
+
+
+# ---------------------------------------------------------------------------
+CONV_CASES = [((5, 5, 5), (7.0, 8.0, 9.0), 3), ((6, 6, 6), (7.0, 7.0, 7.0), 3), ((4, 5, 6), (6.0, 7.0, 8.0), 3),
+              ((8, 8, 8), (10.0, 10.0, 10.0), 5), ((10, 10, 10), (1.0, 1.0, 1.0), 3),
+              ((12, 9, 8), (12.0, 9.0, 8.0), 5), ((16, 16, 16), (20.0, 20.0, 20.0), 4), ((1, 1, 2), (3.0, 3.0, 3.0), 2)]
+
+
+@pytest.mark.parametrize("n,L,mmax", CONV_CASES)
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-11), (FP32, 2e-5)])
+def test_conv_execute(oracle, n, L, mmax, prec, tol):
+    g, ag, mp, tab, ocv = _problem(oracle, n, L, mmax)
+    conv = M.Convolution.from_table(tab, g, ag, mp, prec)
+    assert (conv.nq, conv.np_new, conv.npairs) == (ocv.nq, ocv.np_new, ocv.npairs)
+    assert abs(conv.dq() - ocv.dq) == 0
+    r = np.random.default_rng(4)
+    p = r.standard_normal((ag.m_np,) + n) + 1j * r.standard_normal((ag.m_np,) + n)
+    ref = ocv.execute(p)
+    t = _to(p, conv.complex_dtype)
+    conv.execute(t)
+    assert rel_err(_np(t), ref) < tol
+    # accessors
+    assert np.abs(conv.mnmunukhi_q() - ocv.table()).max() <= 1e-14 * np.abs(ocv.table()).max()
+    assert np.array_equal(conv.gamma_p_map(), oracle.qpairs(*n))
+
+
+def test_conv_from_file_equals_from_table(oracle, tmp_path):
+    n, L, mmax = (6, 6, 5), (8.0, 8.0, 7.0), 3
+    g, ag, mp, tab, ocv = _problem(oracle, n, L, mmax, seed=7)
+    path = str(tmp_path / "c.luc")
+    luc.write_luc(path, tab)
+    a = M.Convolution(path, g, ag, mp, 252, FP64)
+    b = M.Convolution.from_table(tab, g, ag, mp, FP64)
+    assert np.array_equal(a.mnmunukhi_q(), b.mnmunukhi_q())
+    # the file's dq is a difference of two printed abscissae: equal to 1 ulp
+    assert abs(a.dq() - b.dq()) <= 2 * np.finfo(float).eps * b.dq()
+    r = np.random.default_rng(5)
+    p = r.standard_normal((ag.m_np,) + n) + 1j * r.standard_normal((ag.m_np,) + n)
+    ta, tb = _to(p, torch.complex128), _to(p, torch.complex128)
+    a.execute(ta)
+    b.execute(tb)
+    assert rel_err(_np(ta), _np(tb)) < 1e-13
+    assert rel_err(_np(ta), ocv.execute(p)) < 1e-11
+
+
+# ---------------------------------------------------------------------------
+def test_delta_rho_and_delta_f():
+    r = np.random.default_rng(12345)
+    shape = (2, 3, 4, 5)
+    xi, vexc, w = r.random(shape), r.random(shape), r.random(5)
+    rho0, pref = 0.3, 0.334
+    for dt, tol in ((torch.float64, 1e-15), (torch.float32, 1e-6)):
+        xt = _to(xi, dt)
+        out = torch.empty_like(xt)
+        M.get_delta_rho(xt, out, rho0)
+        assert rel_err(_np(out), rho0 * (xi * xi - 1)) < tol
+        df = torch.empty_like(xt)
+        ff = M.get_delta_f(xt, _to(vexc, dt), _to(w, torch.float64), df, rho0, pref)
+        ff_ref = float(np.sum(rho0 * (xi * xi - 1) * pref * w * vexc))
+        assert abs(ff - ff_ref) <= 100 * tol * abs(ff_ref)
+        assert rel_err(_np(df), 2 * rho0 * xi * vexc) < tol
+
+
+def test_fill_uniform_matches_rng():
+    t = torch.empty(100003, dtype=torch.float64, device=_dev())
+    M.fill_uniform(t, seed=12345, lo=0.9, hi=1.1, offset=7)
+    ref = 0.9 + (1.1 - 0.9) * rng.uniform(12345, 100003, offset=7)
+    assert np.array_equal(_np(t), ref)
+    t = torch.empty(1000, dtype=torch.float64, device=_dev())
+    M.fill_uniform(t, seed=5, lo=0.9, hi=1.1)
+    assert np.array_equal(_np(t), rng.xi_field(5, (1000,), 0.1))
+
+
+# ---------------------------------------------------------------------------
+ENERGY_CASES = [((8, 8, 8), (10.0, 10.0, 10.0), 3), ((6, 5, 4), (6.0, 6.0, 6.0), 5), ((16, 16, 16), (12.0, 12.0, 12.0), 5)]
+
+
+@pytest.mark.parametrize("n,L,mmax", ENERGY_CASES)
+def test_energy_fp64_parity(oracle, n, L, mmax):
+    """Plain FP64 GPU result vs the CPU path: 1e-10 relative (ff and df)."""
+    g, ag, mp, tab, ocv = _problem(oracle, n, L, mmax, seed=11)
+    conv = M.Convolution.from_table(tab, g, ag, mp, FP64)
+    n0 = M.SOLVENTS["spce"]["n0"]
+    rho0, pref = M.rho0_of(n0, MRSO), M.prefactor_of(298.15, MRSO, n0)
+    solver = M.Solver(conv, rho0, pref, FP64)
+    xi = rng.xi_field(12345, (g.N, ag.m_no))
+    ff_ref, df_ref = ocv.energy_cproj(xi, rho0, pref)
+    xt = _to(xi, torch.float64)
+    df = torch.empty_like(xt)
+    ff = solver.energy_cproj(xt, df)
+    assert abs(ff - ff_ref) <= 1e-10 * abs(ff_ref)
+    assert rel_err(_np(df), df_ref) < 1e-10
+    # deterministic: a second evaluation is bit-identical
+    df2 = torch.empty_like(xt)
+    assert solver.energy_cproj(xt, df2) == ff and torch.equal(df, df2)
+
+
+@pytest.mark.parametrize("prec,tol", [(FP32, 1e-3), (MIXED, 1e-4)])
+def test_energy_reduced_precision(oracle, prec, tol):
+    """FP32 chops gamma at FLT_EPSILON like the reference's float build;
+    MIXED keeps the FP64 threshold."""
+    n, L, mmax = (8, 8, 8), (10.0, 10.0, 10.0), 5
+    g, ag, mp, tab, ocv = _problem(oracle, n, L, mmax, seed=11)
+    conv = M.Convolution.from_table(tab, g, ag, mp, prec)
+    n0 = M.SOLVENTS["spce"]["n0"]
+    rho0, pref = M.rho0_of(n0, MRSO), M.prefactor_of(298.15, MRSO, n0)
+    solver = M.Solver(conv, rho0, pref, prec)
+    xi = rng.xi_field(12345, (g.N, ag.m_no))
+    ff_ref, df_ref = ocv.energy_cproj(xi, rho0, pref)
+    dt = torch.float32 if prec == FP32 else torch.float64
+    xt = _to(xi, dt)
+    df = torch.empty_like(xt)
+    ff = solver.energy_cproj(xt, df)
+    assert abs(ff - ff_ref) <= tol * abs(ff_ref)
+    assert rel_err(_np(df).astype(np.float64), df_ref) < tol
+
+
+def test_energy_equals_stagewise_api(oracle):
+    """The fused solver equals delta_rho -> angl2proj -> execute -> proj2angl -> delta_f."""
+    n, L, mmax = (6, 6, 6), (8.0, 8.0, 8.0), 3
+    g, ag, mp, tab, ocv = _problem(oracle, n, L, mmax)
+    conv = M.Convolution.from_table(tab, g, ag, mp, FP64)
+    opt = M.OrientationProjectionTransform(g, ag, mp, FP64)
+    rho0, pref = 0.00084, -5.0
+    xi = _to(rng.xi_field(3, (g.N, ag.m_no)), torch.float64)
+    drho = torch.empty_like(xi)
+    M.get_delta_rho(xi, drho, rho0)
+    p = torch.empty((ag.m_np,) + n, dtype=torch.complex128, device=_dev())
+    opt.angl2proj(drho, p)
+    conv.execute(p)
+    vexc = torch.empty_like(xi)
+    opt.proj2angl(p, vexc)
+    df = torch.empty_like(xi)
+    ff = M.get_delta_f(xi, vexc, _to(ag.m_w, torch.float64), df, rho0, pref)
+    df2 = torch.empty_like(xi)
+    ff2 = M.Solver(conv, rho0, pref, FP64).energy_cproj(xi, df2)
+    assert abs(ff - ff2) <= 1e-12 * abs(ff)
+    assert rel_err(_np(df2), _np(df)) < 1e-13
+
+
+def test_argument_errors():
+    g = M.SpatialGrid((4, 4, 4), (1.0, 1.0, 1.0))
+    ag = M.AngularGrid(3, 2)
+    opt = M.OrientationProjectionTransform(g, ag, M.OrientationProjectionMap(ag), FP64)
+    o = torch.zeros((g.N, ag.m_no), dtype=torch.float32, device=_dev())
+    p = torch.zeros((ag.m_np,) + (4, 4, 4), dtype=torch.complex128, device=_dev())
+    with pytest.raises(M.MDFTError, match="dtype"):
+        opt.angl2proj(o, p)
+    with pytest.raises(M.MDFTError, match="elements"):
+        opt.angl2proj(torch.zeros(5, dtype=torch.float64, device=_dev()), p)
+    with pytest.raises(M.MDFTError):
+        M.Convolution("/nonexistent/file", g, ag, M.OrientationProjectionMap(ag), 252, FP64)
