@@ -100,7 +100,7 @@ def main():
     import numpy as np
     import torch
 
-    from mini_mdft_amd import FP64, MIXED, luc
+    from mini_mdft_amd import FP64, MIXED, MIXED_X6, luc
     from mini_mdft_amd import mdft as M
 
     n, L = (N_SIDE,) * 3, (BOX,) * 3
@@ -118,7 +118,7 @@ def main():
     ffd = torch.zeros(1, dtype=torch.float64, device=dev)
 
     results = {}
-    for form, prec in (("mixed", MIXED), ("fp64", FP64)):
+    for form, prec in (("mixed", MIXED), ("mixed_x6", MIXED_X6), ("fp64", FP64)):
         conv = M.Convolution.from_table(tab, g, ag, mp, prec)
         solver = M.Solver(conv, rho0, pref, prec)
         df = torch.empty_like(xi)
@@ -129,9 +129,11 @@ def main():
               f"{ {k: round(v, 4) for k, v in stages.items()} }", file=sys.stderr, flush=True)
         del solver, conv
 
-    mix, f64 = results["mixed"], results["fp64"]
+    mix, f64, x6 = results["mixed"], results["fp64"], results["mixed_x6"]
     dF = mix["ff"] - f64["ff"]
     grad_rel = float(torch.linalg.norm(mix["df"] - f64["df"]) / torch.linalg.norm(f64["df"]))
+    dF_x6 = x6["ff"] - f64["ff"]
+    grad_rel_x6 = float(torch.linalg.norm(x6["df"] - f64["df"]) / torch.linalg.norm(f64["df"]))
     value = args.steps / mix["sec"]
     fp64_rate = args.steps / f64["sec"]
 
@@ -171,6 +173,9 @@ def main():
         "F_fp64": f64["ff"],
         "dF_mixed_vs_fp64": dF,
         "grad_rel_l2_mixed_vs_fp64": grad_rel,
+        "mixed_x6_evals_per_s": round(args.steps / x6["sec"], 2),
+        "dF_mixed_x6_vs_fp64": dF_x6,
+        "grad_rel_l2_mixed_x6_vs_fp64": grad_rel_x6,
         "stages_mixed": stages,
         "stages_fp64_ms": {k: round(v, 5) for k, v in f64["stages"].items()},
         "roofline": roofline,

@@ -23,6 +23,10 @@
  *                                             FP32 accumulation; FFT, rotation,
  *                                             OZ in FP32; R(q), interpolation
  *                                             weights, ff in FP64
+ *   MDFT_MIXED_X6  as MDFT_MIXED, but angl2proj splits data and weights in
+ *                  three BF16 parts (BF16x6 MFMA: data exact to FP32,
+ *                  weights to ~2^-26) and proj2angl runs in FP32 on the
+ *                  vector ALUs; |dF| ~1e-3 kJ/mol at 64^3, mmax = 5
  */
 #ifndef MDFT_HIP_H
 #define MDFT_HIP_H
@@ -36,6 +40,7 @@ extern "C" {
 #define MDFT_FP64 0
 #define MDFT_FP32 1
 #define MDFT_MIXED 2
+#define MDFT_MIXED_X6 3
 
 /* element type codes of the free functions */
 #define MDFT_DOUBLE 0

@@ -33,7 +33,16 @@ int guarded(F&& f) {
 }
 
 void check_precision(int precision) {
-  throw_if(precision != MDFT_FP64 && precision != MDFT_FP32 && precision != MDFT_MIXED, "unknown precision form");
+  throw_if(precision != MDFT_FP64 && precision != MDFT_FP32 && precision != MDFT_MIXED &&
+               precision != MDFT_MIXED_X6,
+           "unknown precision form");
+}
+
+bool is_mixed(int precision) { return precision == MDFT_MIXED || precision == MDFT_MIXED_X6; }
+
+std::unique_ptr<ProjGemm> make_gemm(const AngularGrid& g, int precision) {
+  if (!is_mixed(precision)) return nullptr;
+  return precision == MDFT_MIXED_X6 ? std::make_unique<ProjGemm>(g, 3, 3, false) : std::make_unique<ProjGemm>(g, 2, 2, true);
 }
 
 void check_dtype(int dtype) { throw_if(dtype != MDFT_DOUBLE && dtype != MDFT_FLOAT, "unknown element type"); }
@@ -183,7 +192,7 @@ int mdft_opt_create(int nx, int ny, int nz, int mmax, int mrso, int precision, m
     h->precision = precision;
     h->grid = make_angular_grid(mmax, mrso);
     h->tab = std::make_unique<ProjTables>(h->grid);
-    if (precision == MDFT_MIXED) h->gemm = std::make_unique<ProjGemm>(h->grid);
+    h->gemm = make_gemm(h->grid, precision);
     *out = h.release();
   });
 }
@@ -377,7 +386,7 @@ int mdft_solver_create(mdft_conv_t conv, double rho0, double prefactor, int prec
     const ConvTables& t = conv->dev->tab;
     h->N = static_cast<long>(t.nx) * t.ny * t.nz;
     h->tab = std::make_unique<ProjTables>(t.ang);
-    if (precision == MDFT_MIXED) h->gemm = std::make_unique<ProjGemm>(t.ang);
+    h->gemm = make_gemm(t.ang, precision);
     const size_t pbytes = static_cast<size_t>(h->N) * t.ang.np * (precision == MDFT_FP64 ? 16 : 8);
     h->nparts = std::max(proj2angl_blocks(*h->tab, h->N), h->gemm ? h->gemm->p2a_blocks(h->N) : 0);
     MDFT_HIP_CHECK(hipMalloc(&h->pbuf, pbytes));

@@ -5,16 +5,22 @@
 // runs one wavefront per pair:
 //   1. R^l_{m,m1}(q) by the reference recursion (common/MDFT_Rotation.hpp:
 //      103-283) in FP64, parallel over (m, m1) inside each level l, stored
-//      compactly (only |m|, |m1| <= l); entries the recursion reads outside a
-//      block are taken as 0 (their coefficients vanish);
+//      compactly (|m|, |m1| <= l); entries the recursion reads outside a block
+//      are taken as 0 (their coefficients vanish).  R is the Wigner D-matrix
+//      of the q frame, R^l_{m,m1} = e^{-i m phi_q} d^l_{m,m1}(theta_q), so it
+//      is split into the phases e^{-i m phi_q} and the real matrices
+//      d = Re(R e^{+i m phi_q}); both rotations are then a phase and a real
+//      matrix-vector product, half the work of the complex product;
 //   2. the rotation to the q frame (:427-443), the c(|q|) interpolation
-//      (:449-463, weights in FP64), the OZ product (:468-496) with the
-//      precomputed (ip, n, nu2) -> source-projection map, the chop of gamma
-//      (:514-517) and the inverse rotation and the q / -q stores (:518-541,
-//      conjugated -q store at self-conjugate mid-k points).
-// Pairs are processed in (ix, iy, iz) memory order (same set and q / -q
-// designation as the reference list) and the block -> pair map gives every XCD
-// a contiguous range of pairs, so neighbouring pairs share L2 lines.
+//      (:449-463, weights in FP64, one coalesced pass over the two table rows
+//      into LDS), the OZ product (:468-496), the chop of gamma (:514-517), the
+//      inverse rotation and the q / -q stores (:518-541, conjugated -q store
+//      at self-conjugate mid-k points).
+// The projection index is computed, not looked up: ip(m, mup, mu2) =
+// base_m + (mup + m)(m/mrso + 1) + mu2.  Pairs are processed in (ix, iy, iz)
+// memory order (same set and q / -q designation as the reference list), and
+// the block -> pair map gives every XCD a contiguous range of pairs so that
+// neighbouring pairs share L2 lines.
 #include "mdft_conv.hpp"
 
 #include <algorithm>
@@ -28,23 +34,21 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kXcds = 8;
+constexpr int kMaxM = 6;
 
 template <typename T>
 struct OzArgs {
-  int nx, ny, nz, mmax, mrso, np, np_new, nq, nmu2;
+  int nx, ny, nz, mmax, mrso, np, np_new, nq;
   double dq;
   long npairs, chunk, N;
   const double* kxyz;
   const int* pairs;
-  const int* ia_start;
-  const int* oz_src;
-  const int* ipinfo;
-  const int* pmap;
+  const int* ipinfo;    // m | (mup + mmax) << 8 | mu2 << 16
+  const int* ia_start;  // np + 1
   const cplx<T>* ctab;  // (nq, np_new)
   const double* rc;     // a (5x13x13) | b (5x13x13) | c (5x13) | d (5x13)
   T chop_eps;
 };
-
 
 __device__ __forceinline__ int roff(int l) { return l * (4 * l * l - 1) / 3; }
 
@@ -56,11 +60,6 @@ __device__ __forceinline__ void rset(double2* R, int l, int m, int m1, double2 v
   R[roff(l) + (m + l) * (2 * l + 1) + (m1 + l)] = v;
 }
 
-template <typename T>
-__device__ __forceinline__ cplx<T> to_t(double2 v) {
-  return mkc<T>(static_cast<T>(v.x), static_cast<T>(v.y));
-}
-
 // prevent_underflow (common/MDFT_Math_Utils.hpp:126-137): absolute threshold per component
 template <typename T>
 __device__ __forceinline__ cplx<T> chop(cplx<T> v, T eps) {
@@ -69,30 +68,9 @@ __device__ __forceinline__ cplx<T> chop(cplx<T> v, T eps) {
   return v;
 }
 
-// One pair per 64-lane block.
-template <typename T>
-__global__ __launch_bounds__(kWave) void oz_pair_kernel(cplx<T>* __restrict__ p, OzArgs<T> a) {
-  extern __shared__ unsigned char lds_raw[];
-  const int nR = roff(a.mmax + 1);
-  double2* R = reinterpret_cast<double2*>(lds_raw);
-  cplx<T>* gq = reinterpret_cast<cplx<T>*>(R + nR);
-  cplx<T>* gmq = gq + a.np;
-  cplx<T>* dq = gmq + a.np;
-  cplx<T>* dmq = dq + a.np;
-  const int lane = threadIdx.x;
-
-  const long b = blockIdx.x;
-  const long pair = (b % kXcds) * a.chunk + b / kXcds;
-  if (pair >= a.npairs) return;  // uniform across the block
-  const int pk = a.pairs[pair];
-  const int ix = pk & 1023, iy = (pk >> 10) & 1023, iz = (pk >> 20) & 1023;
-  const int ixm = ix == 0 ? 0 : a.nx - ix, iym = iy == 0 ? 0 : a.ny - iy, izm = iz == 0 ? 0 : a.nz - iz;
-  const long sq = (static_cast<long>(ix) * a.ny + iy) * a.nz + iz;
-  const long smq = (static_cast<long>(ixm) * a.ny + iym) * a.nz + izm;
-  const double q0 = a.kxyz[ix], q1 = a.kxyz[a.nx + iy], q2 = a.kxyz[a.nx + a.ny + iz];
-  const int mmax = a.mmax;
-
-  // ---- R(q): levels 0 and 1 (Rotation.hpp:116-174) ----
+// R(q) for levels 0..mmax into R (FP64, compact), one wave.
+__device__ void rotation_recursion(double2* R, const double* rc, int mmax, double q0, double q1, double q2,
+                                   int lane) {
   if (lane == 0) {
     double ez0, ez1, ez2, ey0, ey1, ey2;
     if (q0 == 0.0 && q1 == 0.0 && q2 == 0.0) {
@@ -100,9 +78,9 @@ __global__ __launch_bounds__(kWave) void oz_pair_kernel(cplx<T>* __restrict__ p,
     } else {
       ez0 = q0; ez1 = q1; ez2 = q2;
     }
-    if (ez0 != 0.0 || ez1 != 0.0) {
+    if (ez0 != 0.0 || ez1 != 0.0) {  // y' = z x z'
       ey0 = -ez1; ey1 = ez0; ey2 = 0.0;
-    } else {
+    } else {  // y' = z' x x
       ey0 = 0.0; ey1 = ez2; ey2 = -ez1;
     }
     double s = sqrt(ez0 * ez0 + ez1 * ez1 + ez2 * ez2);
@@ -112,30 +90,26 @@ __global__ __launch_bounds__(kWave) void oz_pair_kernel(cplx<T>* __restrict__ p,
     const double ex0 = ey1 * ez2 - ey2 * ez1, ex1 = ey2 * ez0 - ey0 * ez2, ex2 = ey0 * ez1 - ey1 * ez0;
     const double h = 1.0 / sqrt(2.0);
     rset(R, 0, 0, 0, make_double2(1.0, 0.0));
-    if (mmax >= 1) {
-      rset(R, 1, -1, -1, make_double2((ey1 + ex0) * 0.5, (ex1 - ey0) * 0.5));
-      rset(R, 1, -1, 0, make_double2(ez0 * h, ez1 * h));
-      rset(R, 1, -1, 1, make_double2((ey1 - ex0) * 0.5, (-ex1 - ey0) * 0.5));
-      rset(R, 1, 0, -1, make_double2(ex2 * h, -ey2 * h));
-      rset(R, 1, 0, 0, make_double2(ez2, 0.0));
-      rset(R, 1, 0, 1, make_double2(-ex2 * h, -ey2 * h));
-      rset(R, 1, 1, -1, make_double2((ey1 - ex0) * 0.5, (ex1 + ey0) * 0.5));
-      rset(R, 1, 1, 0, make_double2(-ez0 * h, ez1 * h));
-      rset(R, 1, 1, 1, make_double2((ey1 + ex0) * 0.5, (ey0 - ex1) * 0.5));
-    }
+    rset(R, 1, -1, -1, make_double2((ey1 + ex0) * 0.5, (ex1 - ey0) * 0.5));
+    rset(R, 1, -1, 0, make_double2(ez0 * h, ez1 * h));
+    rset(R, 1, -1, 1, make_double2((ey1 - ex0) * 0.5, (-ex1 - ey0) * 0.5));
+    rset(R, 1, 0, -1, make_double2(ex2 * h, -ey2 * h));
+    rset(R, 1, 0, 0, make_double2(ez2, 0.0));
+    rset(R, 1, 0, 1, make_double2(-ex2 * h, -ey2 * h));
+    rset(R, 1, 1, -1, make_double2((ey1 - ex0) * 0.5, (ex1 + ey0) * 0.5));
+    rset(R, 1, 1, 0, make_double2(-ez0 * h, ez1 * h));
+    rset(R, 1, 1, 1, make_double2((ey1 + ex0) * 0.5, (ey0 - ex1) * 0.5));
   }
   __syncthreads();
-
-  // ---- levels 2..mmax (Rotation.hpp:177-282) ----
-  const double* ca = a.rc;
-  const double* cb = a.rc + 5 * 13 * 13;
-  const double* cc = a.rc + 2 * 5 * 13 * 13;
-  const double* cdd = a.rc + 2 * 5 * 13 * 13 + 5 * 13;
+  const double* ca = rc;
+  const double* cb = rc + 5 * 13 * 13;
+  const double* cc = rc + 2 * 5 * 13 * 13;
+  const double* cdd = rc + 2 * 5 * 13 * 13 + 5 * 13;
+  const int L6 = 6;
   for (int l = 2; l <= mmax; ++l) {
-    const int k = l - 1;
+    const int k = l - 1, lo = l - 2;
     const int nlow = (l + 1) * (l + 1);
-    const int nitems = nlow + l * l;
-    for (int it = lane; it < nitems; it += kWave) {
+    for (int it = lane; it < nlow + l * l; it += kWave) {
       int m, m1;
       if (it < nlow) {
         m = -l + it / (l + 1);
@@ -145,7 +119,6 @@ __global__ __launch_bounds__(kWave) void oz_pair_kernel(cplx<T>* __restrict__ p,
         m = 1 + t / l;
         m1 = 1 + t % l;
       }
-      const int L6 = 6, lo = l - 2;
       double re, im;
       if (m1 < l) {
         if (m == -l || m == l) {
@@ -186,31 +159,103 @@ __global__ __launch_bounds__(kWave) void oz_pair_kernel(cplx<T>* __restrict__ p,
     }
     __syncthreads();
   }
+}
 
-  // ---- gather g(q), g(-q) (Convolution.hpp:420-425) ----
-  for (int ip = lane; ip < a.np; ip += kWave) {
-    gq[ip] = p[static_cast<long>(ip) * a.N + sq];
-    gmq[ip] = p[static_cast<long>(ip) * a.N + smq];
+// e^{i mup phi_q} as (cos, sin) in FP64: e^{i phi} = (qx + i qy)/|q_xy|,
+// phi = 0 on the z axis (the designated q of a pair never points along -z:
+// its kz >= 0).
+__device__ __forceinline__ double2 phase_pow(double q0, double q1, int mup) {
+  const double rxy = sqrt(q0 * q0 + q1 * q1);
+  const double c1 = rxy > 0.0 ? q0 / rxy : 1.0, s1 = rxy > 0.0 ? q1 / rxy : 0.0;
+  double c = 1.0, s = 0.0;
+  const int am = mup < 0 ? -mup : mup;
+  for (int k = 0; k < am; ++k) {
+    const double t = c * c1 - s * s1;
+    s = s * c1 + c * s1;
+    c = t;
   }
-  __syncthreads();
+  return make_double2(c, mup < 0 ? -s : s);
+}
 
-  const int W = 2 * mmax + 1;
-  // ---- rotation to the q frame (:427-443) ----
-  for (int ip = lane; ip < a.np; ip += kWave) {
-    const int info = a.ipinfo[ip];
-    const int im = info & 0xff, ikhi = ((info >> 8) & 0xff) - mmax, imu2 = info >> 16;
-    cplx<T> s0 = mkc<T>(0, 0), s1 = mkc<T>(0, 0);
-    for (int imup = -im; imup <= im; ++imup) {
-      const int src = a.pmap[(im * W + imup + mmax) * a.nmu2 + imu2];
-      s0 = cfma<T>(s0, gq[src], to_t<T>(rget(R, im, imup, ikhi)));
-      s1 = cfma<T>(s1, gmq[src], to_t<T>(rget(R, im, imup, -ikhi)));
+// Set-up pass (once per Convolution and precision): the real matrices
+// d^l_{mup,chi}(theta_q) = Re(R^l_{mup,chi}(q) e^{+i mup phi_q}) of every pair,
+// R from the FP64 recursion, rounded to T.  Row `pair` of dtab holds the nR
+// compact entries.
+template <typename T>
+__global__ __launch_bounds__(kWave) void oz_dtable_kernel(T* __restrict__ dtab, OzArgs<T> a) {
+  extern __shared__ unsigned char lds_raw[];
+  double2* Rd = reinterpret_cast<double2*>(lds_raw);
+  const int lane = threadIdx.x;
+  const long pair = blockIdx.x;
+  if (pair >= a.npairs) return;
+  const int pk = a.pairs[pair];
+  const int ix = pk & 1023, iy = (pk >> 10) & 1023, iz = (pk >> 20) & 1023;
+  const double q0 = a.kxyz[ix], q1 = a.kxyz[a.nx + iy], q2 = a.kxyz[a.nx + a.ny + iz];
+  rotation_recursion(Rd, a.rc, a.mmax, q0, q1, q2, lane);
+  const int nR = roff(a.mmax + 1);
+  for (int it = lane; it < nR; it += kWave) {
+    int l = 0;
+    while (roff(l + 1) <= it) ++l;
+    const int mup = (it - roff(l)) / (2 * l + 1) - l;
+    const double2 e = phase_pow(q0, q1, mup);
+    const double2 v = Rd[it];
+    dtab[pair * nR + it] = static_cast<T>(v.x * e.x - v.y * e.y);
+  }
+}
+
+// One pair per 64-lane block (per-evaluation pass).
+template <typename T, bool STAGE_C>
+__global__ __launch_bounds__(kWave) void oz_pair_kernel(cplx<T>* __restrict__ p, const T* __restrict__ dtab,
+                                                        OzArgs<T> a) {
+  extern __shared__ unsigned char lds_raw[];
+  const int mmax = a.mmax, np = a.np;
+  const int nR = roff(mmax + 1);
+  cplx<T>* gq = reinterpret_cast<cplx<T>*>(lds_raw);
+  cplx<T>* gmq = gq + np;
+  cplx<T>* dq = gmq + np;
+  cplx<T>* dmq = dq + np;
+  T* dm = reinterpret_cast<T*>(dmq + np);  // nR real d entries
+  cplx<T>* ceff = reinterpret_cast<cplx<T>*>(lds_raw + 4 * np * sizeof(cplx<T>) + ((nR * sizeof(T) + 15) / 16) * 16);
+  __shared__ cplx<T> ph[2 * kMaxM + 1];  // e^{-i mup phi_q}, mup = -mmax..mmax
+  __shared__ int mbase[kMaxM + 2];       // first ip of each m
+  const int lane = threadIdx.x;
+
+  const long b = blockIdx.x;
+  const long pair = (b % kXcds) * a.chunk + b / kXcds;
+  if (pair >= a.npairs) return;  // uniform across the block
+  const int pk = a.pairs[pair];
+  const int ix = pk & 1023, iy = (pk >> 10) & 1023, iz = (pk >> 20) & 1023;
+  const int ixm = ix == 0 ? 0 : a.nx - ix, iym = iy == 0 ? 0 : a.ny - iy, izm = iz == 0 ? 0 : a.nz - iz;
+  const long sq = (static_cast<long>(ix) * a.ny + iy) * a.nz + iz;
+  const long smq = (static_cast<long>(ixm) * a.ny + iym) * a.nz + izm;
+  const double q0 = a.kxyz[ix], q1 = a.kxyz[a.nx + iy], q2 = a.kxyz[a.nx + a.ny + iz];
+
+  // gathers of g(q), g(-q) (:420-425), issued first
+  cplx<T> g0[4], g1[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int ip = lane + t * kWave;
+    if (ip < np) {
+      g0[t] = p[static_cast<long>(ip) * a.N + sq];
+      g1[t] = p[static_cast<long>(ip) * a.N + smq];
     }
-    dq[ip] = s0;
-    dmq[ip] = (im & 1) ? mkc<T>(-s1.x, -s1.y) : s1;
   }
-  __syncthreads();
-
-  // ---- c(|q|) interpolation weights (:449-455) ----
+  // this pair's d matrices (coalesced) and phases
+  const T* dsrc = dtab + pair * nR;
+  for (int it = lane; it < nR; it += kWave) dm[it] = dsrc[it];
+  if (lane <= 2 * mmax) {
+    const double2 e = phase_pow(q0, q1, lane - mmax);
+    ph[lane] = mkc<T>(static_cast<T>(e.x), static_cast<T>(-e.y));
+  }
+  if (lane == 0) {
+    int acc = 0;
+    for (int m = 0; m <= mmax; ++m) {
+      mbase[m] = acc;
+      acc += (2 * m + 1) * (m / a.mrso + 1);
+    }
+    mbase[mmax + 1] = acc;
+  }
+  // c(|q|): rows iq and iq+1 of the (nq, np_new) table
   const double qn = sqrt(q0 * q0 + q1 * q1 + q2 * q2);
   const double eff = qn / a.dq;
   int iq = static_cast<int>(eff);
@@ -219,22 +264,85 @@ __global__ __launch_bounds__(kWave) void oz_pair_kernel(cplx<T>* __restrict__ p,
   const T al = static_cast<T>(alpha), bl = static_cast<T>(1.0 - alpha);
   const cplx<T>* c0 = a.ctab + static_cast<long>(iq) * a.np_new;
   const cplx<T>* c1 = c0 + a.np_new;
-
-  // ---- OZ product (:468-496); gamma overwrites g ----
-  for (int ip = lane; ip < a.np; ip += kWave) {
-    cplx<T> s0 = mkc<T>(0, 0), s1 = mkc<T>(0, 0);
-    const int e = a.ia_start[ip + 1];
-    for (int ia = a.ia_start[ip]; ia < e; ++ia) {
-      const int src = a.oz_src[ia];
-      const int ipm = src & 0xffff;
+  if (STAGE_C) {
+    for (int ia = lane; ia < a.np_new; ia += kWave) {
       const cplx<T> lo = c0[ia], hi = c1[ia];
-      const cplx<T> ce = mkc<T>(al * hi.x + bl * lo.x, al * hi.y + bl * lo.y);
-      if (src >> 16) {
-        s0 = cfma<T>(s0, ce, dq[ipm]);
-        s1 = cfma<T>(s1, ce, dmq[ipm]);
-      } else {
-        s0 = cfma_conjb<T>(s0, ce, dmq[ipm]);
-        s1 = cfma_conjb<T>(s1, ce, dq[ipm]);
+      ceff[ia] = mkc<T>(al * hi.x + bl * lo.x, al * hi.y + bl * lo.y);
+    }
+  }
+  __syncthreads();
+
+  // store the phased gathers g'[ip] = g[ip] e^{-i mup phi}
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int ip = lane + t * kWave;
+    if (ip < np) {
+      const int info = a.ipinfo[ip];
+      const cplx<T> e = ph[(info >> 8) & 0xff];
+      gq[ip] = cmul<T>(g0[t], e);
+      gmq[ip] = cmul<T>(g1[t], e);
+    }
+  }
+  __syncthreads();
+
+  // rotation to the q frame: d_q[m,chi,mu2] = sum_mup g'_q d_{mup,chi};
+  // d_-q[m,chi,mu2] = (-1)^m sum_mup g'_-q d_{mup,-chi}
+  for (int ip = lane; ip < np; ip += kWave) {
+    const int info = a.ipinfo[ip];
+    const int m = info & 0xff, chi = ((info >> 8) & 0xff) - mmax, mu2 = info >> 16;
+    const int km = m / a.mrso + 1, w = 2 * m + 1;
+    const cplx<T>* g0p = gq + mbase[m] + mu2;
+    const cplx<T>* g1p = gmq + mbase[m] + mu2;
+    const T* d0 = dm + roff(m) + (chi + m);
+    const T* d1 = dm + roff(m) + (-chi + m);
+    T ar = 0, ai = 0, br = 0, bi = 0;
+    for (int j = 0; j < w; ++j) {
+      const cplx<T> x = g0p[j * km], y = g1p[j * km];
+      const T u = d0[j * w], v = d1[j * w];
+      ar = fma(x.x, u, ar);
+      ai = fma(x.y, u, ai);
+      br = fma(y.x, v, br);
+      bi = fma(y.y, v, bi);
+    }
+    dq[ip] = mkc<T>(ar, ai);
+    dmq[ip] = (m & 1) ? mkc<T>(-br, -bi) : mkc<T>(br, bi);
+  }
+  __syncthreads();
+
+  // OZ product: for output (m, chi, mu2), terms (n, nu2), n >= |chi|
+  for (int ip = lane; ip < np; ip += kWave) {
+    const int info = a.ipinfo[ip];
+    const int chi = ((info >> 8) & 0xff) - mmax;
+    const int achi = chi < 0 ? -chi : chi;
+    int ia = a.ia_start[ip];
+    cplx<T> s0 = mkc<T>(0, 0), s1 = mkc<T>(0, 0);
+    for (int n = achi; n <= mmax; ++n) {
+      const int kn = n / a.mrso + 1;
+      const int base = mbase[n] + (chi + n) * kn;
+      const int h = n / a.mrso;
+      for (int nu2 = -h; nu2 < 0; ++nu2, ++ia) {
+        cplx<T> ce;
+        if (STAGE_C) {
+          ce = ceff[ia];
+        } else {
+          const cplx<T> lo = c0[ia], hi = c1[ia];
+          ce = mkc<T>(al * hi.x + bl * lo.x, al * hi.y + bl * lo.y);
+        }
+        const int src = base - nu2;
+        s0 = cfma<T>(s0, ce, dq[src]);
+        s1 = cfma<T>(s1, ce, dmq[src]);
+      }
+      for (int nu2 = 0; nu2 <= h; ++nu2, ++ia) {
+        cplx<T> ce;
+        if (STAGE_C) {
+          ce = ceff[ia];
+        } else {
+          const cplx<T> lo = c0[ia], hi = c1[ia];
+          ce = mkc<T>(al * hi.x + bl * lo.x, al * hi.y + bl * lo.y);
+        }
+        const int src = base + nu2;
+        s0 = cfma_conjb<T>(s0, ce, dmq[src]);
+        s1 = cfma_conjb<T>(s1, ce, dq[src]);
       }
     }
     gq[ip] = chop<T>(s0, a.chop_eps);
@@ -242,21 +350,31 @@ __global__ __launch_bounds__(kWave) void oz_pair_kernel(cplx<T>* __restrict__ p,
   }
   __syncthreads();
 
-  // ---- inverse rotation and stores (:518-541) ----
+  // inverse rotation and stores: out[m,mup,mu2] = e^{+i mup phi} sum_chi gamma d_{mup,chi}
   const bool self = (ixm == ix && iym == iy && izm == iz);
   const bool singular = self && (ix == a.nx / 2 || iy == a.ny / 2 || iz == a.nz / 2);
-  for (int ip = lane; ip < a.np; ip += kWave) {
+  for (int ip = lane; ip < np; ip += kWave) {
     const int info = a.ipinfo[ip];
-    const int im = info & 0xff, imup = ((info >> 8) & 0xff) - mmax, imu2 = info >> 16;
-    cplx<T> s0 = mkc<T>(0, 0), s1 = mkc<T>(0, 0);
-    for (int ikhi = -im; ikhi <= im; ++ikhi) {
-      const int src = a.pmap[(im * W + ikhi + mmax) * a.nmu2 + imu2];
-      s0 = cfma_conjb<T>(s0, gq[src], to_t<T>(rget(R, im, imup, ikhi)));
-      s1 = cfma_conjb<T>(s1, gmq[src], to_t<T>(rget(R, im, imup, -ikhi)));
+    const int m = info & 0xff, mup = ((info >> 8) & 0xff) - mmax, mu2 = info >> 16;
+    const int km = m / a.mrso + 1, w = 2 * m + 1;
+    const cplx<T>* g0p = gq + mbase[m] + mu2;
+    const cplx<T>* g1p = gmq + mbase[m] + mu2;
+    const T* drow = dm + roff(m) + (mup + m) * w;
+    T ar = 0, ai = 0, br = 0, bi = 0;
+    for (int j = 0; j < w; ++j) {
+      const cplx<T> x = g0p[j * km], y = g1p[j * km];
+      const T u = drow[j], v = drow[w - 1 - j];
+      ar = fma(x.x, u, ar);
+      ai = fma(x.y, u, ai);
+      br = fma(y.x, v, br);
+      bi = fma(y.y, v, bi);
     }
-    if (im & 1) s1 = mkc<T>(-s1.x, -s1.y);
-    p[static_cast<long>(ip) * a.N + sq] = s0;
-    p[static_cast<long>(ip) * a.N + smq] = singular ? cconj<T>(s1) : s1;
+    const cplx<T> e = ph[mup + mmax];  // e^{-i mup phi}; the output takes its conjugate
+    cplx<T> o0 = cmul<T>(mkc<T>(ar, ai), cconj<T>(e));
+    cplx<T> o1 = cmul<T>(mkc<T>(br, bi), cconj<T>(e));
+    if (m & 1) o1 = mkc<T>(-o1.x, -o1.y);
+    p[static_cast<long>(ip) * a.N + sq] = o0;
+    p[static_cast<long>(ip) * a.N + smq] = singular ? cconj<T>(o1) : o1;
   }
 }
 
@@ -291,7 +409,8 @@ ConvDevice::ConvDevice(int nx, int ny, int nz, double lx, double ly, double lz, 
                        const LucTable& luc)
     : tab(make_conv_tables(nx, ny, nz, lx, ly, lz, mmax, mrso, luc)), fft(nx, ny, nz) {
   const AngularGrid& g = tab.ang;
-  throw_if(g.np >= (1 << 16), "too many projections");
+  throw_if(mmax > kMaxM, "the OZ kernel supports mmax <= 6");
+  throw_if(2 * mmax + 1 > 255, "angular grid too large for the packed index table");
   // pairs: same set and designation as the reference list
   std::vector<int> mem = qpairs_memory_order(nx, ny, nz);
   {
@@ -314,15 +433,14 @@ ConvDevice::ConvDevice(int nx, int ny, int nz, double lx, double ly, double lz, 
   k3.insert(k3.end(), tab.kx.begin(), tab.kx.end());
   k3.insert(k3.end(), tab.ky.begin(), tab.ky.end());
   k3.insert(k3.end(), tab.kz.begin(), tab.kz.end());
-  std::vector<int> src(tab.np_new), info(g.np);
-  for (int ip = 0, ia = 0; ip < g.np; ++ip) {
-    const int khi = g.mup[ip];
+  std::vector<int> info(g.np);
+  for (int ip = 0; ip < g.np; ++ip) {
     info[ip] = g.m[ip] | ((g.mup[ip] + mmax) << 8) | (g.mu2[ip] << 16);
-    for (int n = std::abs(khi); n <= mmax; ++n)
-      for (int nu2 = -n / mrso; nu2 <= n / mrso; ++nu2, ++ia) {
-        const int ipm = g.P(n, khi, std::abs(nu2));
-        src[ia] = ipm | (nu2 < 0 ? (1 << 16) : 0);
-      }
+    // the kernel computes ip(m, mup, mu2) = base_m + (mup + m)(m/mrso + 1) + mu2
+    int base = 0;
+    for (int m = 0; m < g.m[ip]; ++m) base += (2 * m + 1) * (m / mrso + 1);
+    throw_if(base + (g.mup[ip] + g.m[ip]) * (g.m[ip] / mrso + 1) + g.mu2[ip] != ip,
+             "internal error: projection index formula");
   }
   std::vector<double2> t64(static_cast<size_t>(tab.nq) * tab.np_new);
   std::vector<float2> t32(t64.size());
@@ -342,9 +460,7 @@ ConvDevice::ConvDevice(int nx, int ny, int nz, double lx, double ly, double lz, 
   kxyz = upload(k3);
   pairs = upload(packed);
   ia_start = upload(tab.ia_start);
-  oz_src = upload(src);
   ipinfo = upload(info);
-  pmap = upload(g.pmap);
   ctab64 = upload(t64);
   ctab32 = upload(t32);
   rcoef = upload(coef);
@@ -352,32 +468,61 @@ ConvDevice::ConvDevice(int nx, int ny, int nz, double lx, double ly, double lz, 
 
 ConvDevice::~ConvDevice() {
   for (void* q : {static_cast<void*>(kxyz), static_cast<void*>(pairs), static_cast<void*>(ia_start),
-                  static_cast<void*>(oz_src), static_cast<void*>(ipinfo), static_cast<void*>(pmap),
-                  static_cast<void*>(ctab64), static_cast<void*>(ctab32), static_cast<void*>(rcoef)})
+                  static_cast<void*>(ipinfo), static_cast<void*>(ctab64), static_cast<void*>(ctab32),
+                  static_cast<void*>(rcoef), dtab64, dtab32})
     if (q) (void)hipFree(q);
+}
+
+size_t ConvDevice::oz_lds_bytes(size_t elem) const {
+  const int mmax = tab.mmax;
+  const size_t nR = static_cast<size_t>(mmax + 1) * (4 * (mmax + 1) * (mmax + 1) - 1) / 3;
+  const size_t stage_c = elem == sizeof(float) ? static_cast<size_t>(tab.np_new) * 2 * elem : 0;
+  return 4 * static_cast<size_t>(tab.ang.np) * 2 * elem + ((nR * elem + 15) / 16) * 16 + stage_c;
+}
+
+template <typename T>
+static OzArgs<T> oz_args(const ConvDevice& c, double chop_eps) {
+  OzArgs<T> a;
+  const ConvTables& tab = c.tab;
+  a.chop_eps = static_cast<T>(chop_eps);
+  a.nx = tab.nx; a.ny = tab.ny; a.nz = tab.nz; a.mmax = tab.mmax; a.mrso = tab.mrso;
+  a.np = tab.ang.np; a.np_new = tab.np_new; a.nq = tab.nq; a.dq = tab.dq;
+  a.npairs = c.npairs;
+  a.chunk = (c.npairs + kXcds - 1) / kXcds;
+  a.N = static_cast<long>(tab.nx) * tab.ny * tab.nz;
+  a.kxyz = c.kxyz; a.pairs = c.pairs; a.ipinfo = c.ipinfo; a.ia_start = c.ia_start;
+  a.ctab = std::is_same<T, double>::value ? reinterpret_cast<const cplx<T>*>(c.ctab64)
+                                          : reinterpret_cast<const cplx<T>*>(c.ctab32);
+  a.rc = c.rcoef;
+  return a;
+}
+
+template <typename T>
+const T* ConvDevice::dtable(hipStream_t s) const {
+  void*& slot = std::is_same<T, double>::value ? dtab64 : dtab32;
+  if (!slot) {
+    const size_t nR = static_cast<size_t>(tab.mmax + 1) * (4 * (tab.mmax + 1) * (tab.mmax + 1) - 1) / 3;
+    MDFT_HIP_CHECK(hipMalloc(&slot, std::max<size_t>(1, npairs * nR * sizeof(T))));
+    OzArgs<T> a = oz_args<T>(*this, 0.0);
+    hipLaunchKernelGGL((oz_dtable_kernel<T>), dim3(static_cast<unsigned>(npairs)), dim3(kWave),
+                       nR * sizeof(double2), s, static_cast<T*>(slot), a);
+    MDFT_HIP_CHECK(hipGetLastError());
+  }
+  return static_cast<const T*>(slot);
 }
 
 template <typename T>
 void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s) const {
   if (npairs == 0) return;
-  OzArgs<T> a;
-  a.chop_eps = static_cast<T>(chop_eps);
-  const AngularGrid& g = tab.ang;
-  a.nx = tab.nx; a.ny = tab.ny; a.nz = tab.nz; a.mmax = tab.mmax; a.mrso = tab.mrso;
-  a.np = g.np; a.np_new = tab.np_new; a.nq = tab.nq; a.nmu2 = g.nmu2; a.dq = tab.dq;
-  a.npairs = npairs;
-  a.chunk = (npairs + kXcds - 1) / kXcds;
-  a.N = static_cast<long>(tab.nx) * tab.ny * tab.nz;
-  a.kxyz = kxyz; a.pairs = pairs; a.ia_start = ia_start; a.oz_src = oz_src; a.ipinfo = ipinfo; a.pmap = pmap;
-  a.ctab = std::is_same<T, double>::value ? reinterpret_cast<const cplx<T>*>(ctab64)
-                                          : reinterpret_cast<const cplx<T>*>(ctab32);
-  a.rc = rcoef;
+  throw_if(tab.ang.np > 4 * kWave, "the OZ kernel handles at most 256 projections");
+  const T* dt = dtable<T>(s);
+  OzArgs<T> a = oz_args<T>(*this, chop_eps);
   const long nblk = a.chunk * kXcds;
   throw_if(nblk > 0x7fffffffL, "too many q pairs");
-  const int nR = (tab.mmax + 1) * (4 * (tab.mmax + 1) * (tab.mmax + 1) - 1) / 3;
-  const size_t lds = nR * sizeof(double2) + 4 * static_cast<size_t>(g.np) * sizeof(cplx<T>);
+  const size_t lds = oz_lds_bytes(sizeof(T));
   throw_if(lds > 64 * 1024, "angular grid too large for the OZ kernel");
-  hipLaunchKernelGGL((oz_pair_kernel<T>), dim3(static_cast<unsigned>(nblk)), dim3(kWave), lds, s, p, a);
+  constexpr bool stage = std::is_same<T, float>::value;
+  hipLaunchKernelGGL((oz_pair_kernel<T, stage>), dim3(static_cast<unsigned>(nblk)), dim3(kWave), lds, s, p, dt, a);
   MDFT_HIP_CHECK(hipGetLastError());
 }
 

@@ -13,10 +13,8 @@ struct ConvDevice {
   long npairs = 0;
   double* kxyz = nullptr;     // kx | ky | kz (FP64)
   int* pairs = nullptr;       // packed ix | iy << 10 | iz << 20, (ix, iy, iz)-ordered
-  int* ia_start = nullptr;    // np + 1
-  int* oz_src = nullptr;      // np_new: source ip of term ia | (nu2 < 0) << 16
+  int* ia_start = nullptr;    // np + 1: first OZ term of each output projection
   int* ipinfo = nullptr;      // np: m | (mup + mmax) << 8 | mu2 << 16
-  int* pmap = nullptr;        // (mmax+1) x (2mmax+1) x nmu2
   double2* ctab64 = nullptr;  // (nq, np_new): the OZ table, transposed
   float2* ctab32 = nullptr;
   double* rcoef = nullptr;    // rotation recursion coefficients a | b | c | d
@@ -24,6 +22,14 @@ struct ConvDevice {
   ~ConvDevice();
   ConvDevice(const ConvDevice&) = delete;
   ConvDevice& operator=(const ConvDevice&) = delete;
+  // LDS bytes of one OZ block for elements of elem bytes (float or double)
+  size_t oz_lds_bytes(size_t elem) const;
+  // per-pair real rotation matrices d^l(theta_q) in T, built on first use
+  // by an FP64 set-up kernel on stream s (npairs x nR entries)
+  template <typename T>
+  const T* dtable(hipStream_t s) const;
+  mutable void* dtab64 = nullptr;
+  mutable void* dtab32 = nullptr;
 
   // the per-(q, -q) rotation / OZ / inverse-rotation pass on FFT'd data;
   // gamma components below chop_eps in magnitude are set to 0

@@ -65,6 +65,74 @@ __device__ __forceinline__ void dft_small(cplx<T>* v, int R, int sign, const cpl
   }
 }
 
+// DFT of R values held in registers (R fixed at compile time)
+template <typename T, int R>
+__device__ __forceinline__ void dft_fixed(cplx<T> (&v)[R], int sign, const cplx<T>* tw, int n) {
+  if constexpr (R == 2 || R == 3 || R == 4) {
+    dft_small<T>(v, R, sign, tw, n);
+  } else {
+    cplx<T> out[R];
+    const int step = n / R;
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      cplx<T> acc = v[0];
+#pragma unroll
+      for (int r = 1; r < R; ++r) acc = cfma<T>(acc, v[r], twiddle<T>(tw, ((r * s) % R) * step, sign));
+      out[s] = acc;
+    }
+#pragma unroll
+    for (int s = 0; s < R; ++s) v[s] = out[s];
+  }
+}
+
+// One Stockham stage (radix R, Ns = product of the earlier radices) over the
+// wvalid lines of a tile: butterfly j of a line reads x[j + r n/R], applies the
+// twiddles W_n^{r (j mod Ns) n/(Ns R)}, and writes its outputs at
+// (j / Ns) Ns R + (j mod Ns) + r Ns.
+template <typename T, int R>
+__device__ __forceinline__ void stockham_stage(const cplx<T>* src, cplx<T>* dst, const cplx<T>* tw, int n, int ld,
+                                               int Ns, int wvalid, int sign, int tid, int nthreads) {
+  const int nb = n / R;
+  const int tstep = n / (Ns * R);
+  for (int beta = tid; beta < wvalid * nb; beta += nthreads) {
+    const int w = beta / nb, j = beta - w * nb;
+    const int k = j % Ns;
+    const cplx<T>* in = src + w * ld + j;
+    cplx<T> v[R];
+    v[0] = in[0];
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = cmul<T>(in[r * nb], twiddle<T>(tw, r * k * tstep, sign));
+    dft_fixed<T, R>(v, sign, tw, n);
+    cplx<T>* out = dst + w * ld + (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) out[r * Ns] = v[r];
+  }
+}
+
+// The same stage for an odd prime radix 7..13 known only at run time (rare
+// lengths): one thread per butterfly output, O(R) LDS reads each, no private
+// arrays.
+template <typename T>
+__device__ __forceinline__ void stockham_stage_generic(const cplx<T>* src, cplx<T>* dst, const cplx<T>* tw, int n,
+                                                       int ld, int Ns, int R, int wvalid, int sign, int tid,
+                                                       int nthreads) {
+  const int nb = n / R;
+  const int tstep = n / (Ns * R);
+  const int step = n / R;
+  for (int it = tid; it < wvalid * nb * R; it += nthreads) {
+    const int s = it % R, beta = it / R;
+    const int w = beta / nb, j = beta - w * nb;
+    const int k = j % Ns;
+    const cplx<T>* in = src + w * ld + j;
+    cplx<T> acc = in[0];
+    for (int r = 1; r < R; ++r) {
+      const cplx<T> x = cmul<T>(in[r * nb], twiddle<T>(tw, r * k * tstep, sign));
+      acc = cfma<T>(acc, x, twiddle<T>(tw, ((r * s) % R) * step, sign));
+    }
+    dst[w * ld + (j / Ns) * Ns * R + k + s * Ns] = acc;
+  }
+}
+
 // One block transforms a tile of W lines of length n.
 //   STRIDED = false: line l occupies data[l*n .. l*n+n); the tile is lines
 //                    [tile*W, tile*W + W) (clipped at nlines).
@@ -111,18 +179,22 @@ __global__ __launch_bounds__(kFftThreads) void fft_axis_kernel(cplx<T>* __restri
   int Ns = 1;
   for (int s = 0; s < st.nstages; ++s) {
     const int R = st.radix[s];
-    const int nb = n / R;
-    const int tstep = n / (Ns * R);
-    for (int beta = tid; beta < wvalid * nb; beta += blockDim.x) {
-      const int w = beta / nb, j = beta - w * nb;
-      const int k = j % Ns;
-      cplx<T> v[kMaxRadix];
-      const cplx<T>* in = src + w * ld + j;
-      v[0] = in[0];
-      for (int r = 1; r < R; ++r) v[r] = cmul<T>(in[r * nb], twiddle<T>(tw, (r * k * tstep) % n, sign));
-      dft_small<T>(v, R, sign, tw, n);
-      cplx<T>* out = dst + w * ld + (j / Ns) * Ns * R + k;
-      for (int r = 0; r < R; ++r) out[r * Ns] = v[r];
+    switch (R) {
+      case 4:
+        stockham_stage<T, 4>(src, dst, tw, n, ld, Ns, wvalid, sign, tid, blockDim.x);
+        break;
+      case 2:
+        stockham_stage<T, 2>(src, dst, tw, n, ld, Ns, wvalid, sign, tid, blockDim.x);
+        break;
+      case 3:
+        stockham_stage<T, 3>(src, dst, tw, n, ld, Ns, wvalid, sign, tid, blockDim.x);
+        break;
+      case 5:
+        stockham_stage<T, 5>(src, dst, tw, n, ld, Ns, wvalid, sign, tid, blockDim.x);
+        break;
+      default:
+        stockham_stage_generic<T>(src, dst, tw, n, ld, Ns, R, wvalid, sign, tid, blockDim.x);
+        break;
     }
     __syncthreads();
     cplx<T>* t = src;

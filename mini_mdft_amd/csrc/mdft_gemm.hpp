@@ -1,4 +1,4 @@
-// Projections of the MDFT_MIXED form (see mdft_gemm.hip).
+// Projections of the MDFT_MIXED forms as split-BF16 MFMA GEMMs (see mdft_gemm.hip).
 #pragma once
 
 #include <memory>
@@ -8,7 +8,12 @@
 namespace mdft {
 
 struct ProjGemm {
-  explicit ProjGemm(const AngularGrid& g);
+  // angl2proj split (data parts, weight parts): (2, 2) BF16x3, three products
+  // per term; (3, 3) BF16x6, six products, data exact to FP32 and weights to
+  // ~2^-26.  p2a_mfma: proj2angl as a BF16x3 GEMM (needs (2, 2)), else the
+  // factorised FP32 VALU kernel.  Grids beyond the MFMA register tiles
+  // (2 np > 256 or no > 384) run the factorised FP32 projections instead.
+  ProjGemm(const AngularGrid& g, int data_parts, int weight_parts, bool p2a_mfma);
   ~ProjGemm();
   ProjGemm(const ProjGemm&) = delete;
   ProjGemm& operator=(const ProjGemm&) = delete;

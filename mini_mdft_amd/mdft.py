@@ -14,7 +14,7 @@ import math
 import numpy as np
 
 from . import _lib
-from ._lib import DOUBLE, FLOAT, FP32, FP64, MIXED, MDFTError, check, lib, ptr, stream_handle
+from ._lib import DOUBLE, FLOAT, FP32, FP64, MIXED, MIXED_X6, MDFTError, check, lib, ptr, stream_handle
 
 # common/MDFT_Constants.hpp:25-56
 BOLTZ = 1.38064852e-23
@@ -59,7 +59,7 @@ def _dtypes(precision):
         return torch.float64, torch.complex128
     if precision == FP32:
         return torch.float32, torch.complex64
-    if precision == MIXED:
+    if precision in (MIXED, MIXED_X6):
         return torch.float64, torch.complex64
     raise MDFTError("unknown precision form")
 
@@ -164,13 +164,15 @@ class OrientationProjectionTransform:
         h = ctypes.c_void_p()
         check(lib().mdft_opt_create(spatial_grid.m_nx, spatial_grid.m_ny, spatial_grid.m_nz, angular_grid.m_mmax,
                                     angular_grid.m_molrotsymorder, precision, ctypes.byref(h)))
-        self._h = h
+        self._h, self._destroy = h, lib().mdft_opt_destroy
         self.real_dtype, self.complex_dtype = _dtypes(precision)
 
     def __del__(self):
-        h = getattr(self, "_h", None)
-        if h:
-            lib().mdft_opt_destroy(h)
+        # the destroy entry point is bound at creation: module globals may
+        # already be gone when this runs at interpreter exit
+        h, destroy = getattr(self, "_h", None), getattr(self, "_destroy", None)
+        if h and destroy is not None:
+            destroy(h)
 
     def angl2proj(self, o, p, stream=None):
         """o: (N, ntheta, nphi, npsi) real -> p: (np, nx, ny, nz) complex (:267-380)."""
@@ -206,7 +208,7 @@ class Convolution:
                                                     tuples.shape[0], tuples.ctypes.data_as(_lib._pi),
                                                     c.ctypes.data_as(_lib._pd), c.shape[1], ctypes.c_double(dq),
                                                     precision, ctypes.byref(h)))
-        self._h = h
+        self._h, self._destroy = h, lib().mdft_conv_destroy
         nq, npn, npairs = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         dq_ = ctypes.c_double()
         check(lib().mdft_conv_info(h, ctypes.byref(nq), ctypes.byref(npn), ctypes.byref(npairs), ctypes.byref(dq_)))
@@ -220,9 +222,11 @@ class Convolution:
                    _table=(table.tuples, table.c, table.dq))
 
     def __del__(self):
-        h = getattr(self, "_h", None)
-        if h:
-            lib().mdft_conv_destroy(h)
+        # the destroy entry point is bound at creation: module globals may
+        # already be gone when this runs at interpreter exit
+        h, destroy = getattr(self, "_h", None), getattr(self, "_destroy", None)
+        if h and destroy is not None:
+            destroy(h)
 
     def dq(self):
         return self._dq
@@ -289,6 +293,7 @@ class Solver:
         check(lib().mdft_solver_create(conv._h, ctypes.c_double(rho0), ctypes.c_double(prefactor), precision,
                                        ctypes.byref(h)))
         self._h, self.conv, self.precision = h, conv, precision
+        self._destroy = lib().mdft_solver_destroy
         self.rho0, self.prefactor = rho0, prefactor
         self.real_dtype = _dtypes(precision)[0]
         b = ctypes.c_size_t()
@@ -296,9 +301,11 @@ class Solver:
         self.workspace_bytes = b.value
 
     def __del__(self):
-        h = getattr(self, "_h", None)
-        if h:
-            lib().mdft_solver_destroy(h)
+        # the destroy entry point is bound at creation: module globals may
+        # already be gone when this runs at interpreter exit
+        h, destroy = getattr(self, "_h", None), getattr(self, "_destroy", None)
+        if h and destroy is not None:
+            destroy(h)
 
     def energy_cproj(self, xi, df, stream=None) -> float:
         n = self.conv.sg.N * self.conv.ag.m_no
