@@ -231,6 +231,137 @@ __global__ __launch_bounds__(kThreads) void proj2angl_kernel(const cplx<TP>* __r
   if (xi != nullptr) block_sum_store<TC>(acc_ff, partials + blockIdx.x);
 }
 
+// proj2angl for angular grids with compile-time sizes: one 64-lane block per
+// (64 consecutive r, theta).  Each lane keeps its S[mup][mu2] in registers
+// (the m sums in the same ascending-m order as proj2angl_kernel), runs the
+// phi sums on the (mup, -mup) pairs, Z = S_0 + sum_{mup > 0} (A cos + i B sin)
+// with A = S_mup + S_-mup and B = S_mup - S_-mup, then the psi c2r.  All
+// indices and twiddle offsets are constants after unrolling (scalar table
+// loads).  The 64 x (nphi npsi) outputs are staged in LDS and written (with
+// the fused get_delta_f) in runs of consecutive io, the layout of o/xi/df.
+// Blocks are ordered so that the ntheta blocks of one r range run on the same
+// XCD (block b runs on XCD b mod 8) and share their projection reads in L2.
+template <int MMAX, int MRSO>
+struct AngSizes {
+  static constexpr int kNth = MMAX + 1, kNphi = 2 * MMAX + 1, kNmu2 = MMAX / MRSO + 1;
+  static constexpr int kNpsi = 2 * (MMAX / MRSO) + 1, kNjk = kNphi * kNpsi, kNo = kNth * kNjk;
+  static constexpr int np() {
+    int n = 0;
+    for (int m = 0; m <= MMAX; ++m) n += (2 * m + 1) * (m / MRSO + 1);
+    return n;
+  }
+};
+
+constexpr int kWaveRows = 64;  // r values per block of the compile-time kernel
+constexpr int kXcds = 8;
+
+template <int MMAX, int MRSO, typename TP, typename TO, typename TC>
+__global__ __launch_bounds__(64) void proj2angl_wave_kernel(const cplx<TP>* __restrict__ p, TO* __restrict__ o,
+                                                            const TO* __restrict__ xi, TO* __restrict__ df,
+                                                            const double* __restrict__ wq, long N, long nrb,
+                                                            TabPtrs<TC> tb, double rho0,
+                                                            double* __restrict__ partials) {
+  using A = AngSizes<MMAX, MRSO>;
+  constexpr int NP = A::np();
+  __shared__ TC stage[kWaveRows * A::kNjk];
+  const int lane = threadIdx.x;
+  const int q = blockIdx.x / kXcds;
+  const int th = q % A::kNth;
+  const long rb = static_cast<long>(q / A::kNth) * kXcds + blockIdx.x % kXcds;
+  if (rb >= nrb) {  // padding block of the XCD ordering
+    if (partials && lane == 0) partials[blockIdx.x] = 0.0;
+    return;
+  }
+  const long r0 = rb * kWaveRows;
+  const long r = min(r0 + lane, N - 1);
+  const TC* w = tb.wts + th * NP;  // d f_m of this theta
+
+  TC sr[A::kNphi][A::kNmu2], si[A::kNphi][A::kNmu2];
+#pragma unroll
+  for (int a = 0; a < A::kNphi; ++a)
+#pragma unroll
+    for (int b = 0; b < A::kNmu2; ++b) sr[a][b] = si[a][b] = TC(0);
+  {
+    int ip = 0;
+#pragma unroll
+    for (int m = 0; m <= MMAX; ++m)
+#pragma unroll
+      for (int mup = -m; mup <= m; ++mup)
+#pragma unroll
+        for (int mu = 0; mu <= m; mu += MRSO, ++ip) {
+          const cplx<TP> v = p[static_cast<long>(ip) * N + r];
+          const TC wt = w[ip];
+          sr[mup + MMAX][mu / MRSO] = fma(static_cast<TC>(v.x), wt, sr[mup + MMAX][mu / MRSO]);
+          si[mup + MMAX][mu / MRSO] = fma(static_cast<TC>(v.y), wt, si[mup + MMAX][mu / MRSO]);
+        }
+  }
+  // S = conj(sum); sr/si of +-mup become A = S+ + S- and B = S+ - S-
+#pragma unroll
+  for (int b = 0; b < A::kNmu2; ++b) {
+    si[MMAX][b] = -si[MMAX][b];
+#pragma unroll
+    for (int mup = 1; mup <= MMAX; ++mup) {
+      const TC pr = sr[MMAX + mup][b], pi = -si[MMAX + mup][b];
+      const TC nr = sr[MMAX - mup][b], ni = -si[MMAX - mup][b];
+      sr[MMAX + mup][b] = pr + nr;
+      si[MMAX + mup][b] = pi + ni;
+      sr[MMAX - mup][b] = pr - nr;
+      si[MMAX - mup][b] = pi - ni;
+    }
+  }
+  TC* out = stage + lane * A::kNjk;
+#pragma unroll
+  for (int j = 0; j < A::kNphi; ++j) {
+    TC zr[A::kNmu2], zi[A::kNmu2];
+#pragma unroll
+    for (int b = 0; b < A::kNmu2; ++b) {
+      zr[b] = sr[MMAX][b];
+      zi[b] = si[MMAX][b];
+#pragma unroll
+      for (int mup = 1; mup <= MMAX; ++mup) {
+        const cplx<TC> e = tb.ephi[(mup * j) % A::kNphi];
+        zr[b] = fma(sr[MMAX + mup][b], e.x, fma(-si[MMAX - mup][b], e.y, zr[b]));
+        zi[b] = fma(si[MMAX + mup][b], e.x, fma(sr[MMAX - mup][b], e.y, zi[b]));
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < A::kNpsi; ++k) {
+      TC v = zr[0];
+#pragma unroll
+      for (int b = 1; b < A::kNmu2; ++b) {
+        const cplx<TC> e = tb.epsi[b * A::kNpsi + k];
+        v += static_cast<TC>(2) * (zr[b] * e.x - zi[b] * e.y);
+      }
+      out[j * A::kNpsi + k] = v;
+    }
+  }
+  __syncthreads();
+
+  double acc_ff = 0.0;
+  const long base = r0 * A::kNo + th * A::kNjk;
+#pragma unroll 5
+  for (int e = lane; e < kWaveRows * A::kNjk; e += 64) {
+    const int rl = e / A::kNjk, jk = e - rl * A::kNjk;
+    if (r0 + rl < N) {
+      const long g = base + static_cast<long>(rl) * A::kNo + jk;
+      const TC v = stage[e];
+      if (xi == nullptr) {
+        o[g] = static_cast<TO>(v);
+      } else {
+        const double x = static_cast<double>(xi[g]);
+        const double vx = static_cast<double>(static_cast<TO>(v));
+        acc_ff += vx * wq[th * A::kNjk + jk] * (x * x - 1.0);
+        df[g] = static_cast<TO>(2.0 * rho0 * x * vx);
+      }
+    }
+  }
+  if (xi != nullptr) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc_ff += __shfl_xor(acc_ff, off);
+    if (lane == 0) partials[blockIdx.x] = acc_ff;
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kThreads) void delta_rho_kernel(const T* __restrict__ xi, T* __restrict__ out, size_t n,
                                                              T rho0) {
@@ -410,12 +541,55 @@ void angl2proj_factorized(const ProjTables& t, long N, const TI* o, cplx<TP>* p,
   MDFT_HIP_CHECK(hipGetLastError());
 }
 
-int proj2angl_blocks(const ProjTables&, long N) { return static_cast<int>((N + kRT - 1) / kRT); }
+namespace {
+
+// the angular grids with a compile-time proj2angl kernel: mrso = 2 with mmax
+// 1..6 and mrso = 1 with mmax 1..3 (larger mrso = 1 grids hold too many S
+// registers per lane and take the generic kernel)
+template <int MMAX = 1, int MRSO = 1, typename F>
+bool dispatch_angular(int mmax, int mrso, F&& f) {
+  if constexpr (MMAX > 6) {
+    return false;
+  } else if constexpr (MRSO > 2 || (MRSO == 1 && MMAX > 3)) {
+    return dispatch_angular<MMAX + (MRSO > 2 ? 1 : 0), MRSO + (MRSO > 2 ? -2 : 1)>(mmax, mrso, f);
+  } else {
+    if (mmax == MMAX && mrso == MRSO) {
+      f(std::integral_constant<int, MMAX>{}, std::integral_constant<int, MRSO>{});
+      return true;
+    }
+    return dispatch_angular<MMAX, MRSO + 1>(mmax, mrso, f);
+  }
+}
+
+long wave_row_blocks(long N) { return (N + kWaveRows - 1) / kWaveRows; }
+
+long wave_blocks(const ProjTables& t, long N) {
+  const long nrbx = (wave_row_blocks(N) + kXcds - 1) / kXcds;
+  return nrbx * kXcds * t.ntheta;
+}
+
+}  // namespace
+
+int proj2angl_blocks(const ProjTables& t, long N) {
+  if (dispatch_angular(t.mmax, t.mrso, [](auto, auto) {})) return static_cast<int>(wave_blocks(t, N));
+  return static_cast<int>((N + kRT - 1) / kRT);
+}
 
 template <typename TP, typename TO, typename TC>
 void proj2angl_factorized(const ProjTables& t, long N, const cplx<TP>* p, TO* o, const TO* xi, TO* df,
                           double rho0, double* partials, hipStream_t s) {
   if (N == 0) return;
+  const long nwb = wave_blocks(t, N);
+  throw_if(nwb > 0x7fffffffL, "grid too large");
+  const bool compiled = dispatch_angular(t.mmax, t.mrso, [&](auto mm, auto mr) {
+    hipLaunchKernelGGL((proj2angl_wave_kernel<decltype(mm)::value, decltype(mr)::value, TP, TO, TC>),
+                       dim3(static_cast<unsigned>(nwb)), dim3(64), 0, s, p, o, xi, df, t.w, N, wave_row_blocks(N),
+                       tabs_of<TC>(t, false), rho0, partials);
+  });
+  if (compiled) {
+    MDFT_HIP_CHECK(hipGetLastError());
+    return;
+  }
   const long nblk = (N + kRT - 1) / kRT;
   throw_if(nblk > 0x7fffffffL, "grid too large");
   const size_t lds = p2a_lds<TC>(t);

@@ -59,13 +59,15 @@ def test_fft3d(shape, prec):
 
 
 # ---------------------------------------------------------------------------
-CASES = [((2, 3, 4), 3), ((2, 2, 3), 5), ((3, 3, 3), 1), ((2, 2, 2), 10), ((5, 4, 3), 4)]
+# (grid, mmax, mrso): compile-time proj2angl kernels (mrso 2 and small mrso 1
+# grids, including partial 64-point blocks) and the generic kernel ((4, 1), (10, 10))
+CASES = [((2, 3, 4), 3, 2), ((2, 2, 3), 5, 2), ((3, 3, 3), 1, 2), ((2, 2, 2), 10, 10), ((5, 4, 3), 4, 2),
+         ((3, 2, 2), 2, 1), ((2, 3, 2), 3, 1), ((2, 2, 2), 4, 1), ((9, 7, 5), 5, 2), ((4, 4, 4), 6, 2)]
 
 
-@pytest.mark.parametrize("n,mmax", CASES)
+@pytest.mark.parametrize("n,mmax,mrso", CASES)
 @pytest.mark.parametrize("prec,tol", [(FP64, 1e-13), (FP32, 2e-6), (MIXED, 5e-5), (MIXED_X6, 5e-5)])
-def test_angl2proj(oracle, n, mmax, prec, tol):
-    mrso = MRSO if mmax != 10 else 10
+def test_angl2proj(oracle, n, mmax, mrso, prec, tol):
     g = M.SpatialGrid(n, (1.0, 1.0, 1.0))
     ag = M.AngularGrid(mmax, mrso)
     opt = M.OrientationProjectionTransform(g, ag, M.OrientationProjectionMap(ag), prec)
@@ -78,10 +80,9 @@ def test_angl2proj(oracle, n, mmax, prec, tol):
     assert rel_err(_np(p), ref) < tol
 
 
-@pytest.mark.parametrize("n,mmax", CASES)
+@pytest.mark.parametrize("n,mmax,mrso", CASES)
 @pytest.mark.parametrize("prec,tol", [(FP64, 1e-13), (FP32, 2e-6), (MIXED, 5e-5), (MIXED_X6, 5e-5)])
-def test_proj2angl(oracle, n, mmax, prec, tol):
-    mrso = MRSO if mmax != 10 else 10
+def test_proj2angl(oracle, n, mmax, mrso, prec, tol):
     g = M.SpatialGrid(n, (1.0, 1.0, 1.0))
     ag = M.AngularGrid(mmax, mrso)
     opt = M.OrientationProjectionTransform(g, ag, M.OrientationProjectionMap(ag), prec)
