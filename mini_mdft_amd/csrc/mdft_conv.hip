@@ -35,6 +35,9 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kXcds = 8;
 constexpr int kMaxM = 6;
+// compact d entries of l <= kMaxM (sum (2l+1)^2 = 455), rounded up to whole waves
+constexpr int kMaxR = 512;
+constexpr int kCBatch = 8;  // c(q) rows per lane in flight while staging
 
 template <typename T>
 struct OzArgs {
@@ -180,9 +183,11 @@ __device__ __forceinline__ double2 phase_pow(double q0, double q1, int mup) {
 // Set-up pass (once per Convolution and precision): the real matrices
 // d^l_{mup,chi}(theta_q) = Re(R^l_{mup,chi}(q) e^{+i mup phi_q}) of every pair,
 // R from the FP64 recursion, rounded to T.  Row `pair` of dtab holds the nR
-// compact entries.
+// compact entries, or with padw > 0 the padded layout of the compile-time OZ
+// kernel: (mmax+1)^2 rows (l, mup) of padw entries, chi + l in the first
+// 2l+1 and zeros after.
 template <typename T>
-__global__ __launch_bounds__(kWave) void oz_dtable_kernel(T* __restrict__ dtab, OzArgs<T> a) {
+__global__ __launch_bounds__(kWave) void oz_dtable_kernel(T* __restrict__ dtab, OzArgs<T> a, int padw) {
   extern __shared__ unsigned char lds_raw[];
   double2* Rd = reinterpret_cast<double2*>(lds_raw);
   const int lane = threadIdx.x;
@@ -192,14 +197,292 @@ __global__ __launch_bounds__(kWave) void oz_dtable_kernel(T* __restrict__ dtab, 
   const int ix = pk & 1023, iy = (pk >> 10) & 1023, iz = (pk >> 20) & 1023;
   const double q0 = a.kxyz[ix], q1 = a.kxyz[a.nx + iy], q2 = a.kxyz[a.nx + a.ny + iz];
   rotation_recursion(Rd, a.rc, a.mmax, q0, q1, q2, lane);
-  const int nR = roff(a.mmax + 1);
-  for (int it = lane; it < nR; it += kWave) {
-    int l = 0;
-    while (roff(l + 1) <= it) ++l;
-    const int mup = (it - roff(l)) / (2 * l + 1) - l;
-    const double2 e = phase_pow(q0, q1, mup);
-    const double2 v = Rd[it];
-    dtab[pair * nR + it] = static_cast<T>(v.x * e.x - v.y * e.y);
+  if (padw == 0) {
+    const int nR = roff(a.mmax + 1);
+    for (int it = lane; it < nR; it += kWave) {
+      int l = 0;
+      while (roff(l + 1) <= it) ++l;
+      const int mup = (it - roff(l)) / (2 * l + 1) - l;
+      const double2 e = phase_pow(q0, q1, mup);
+      const double2 v = Rd[it];
+      dtab[pair * nR + it] = static_cast<T>(v.x * e.x - v.y * e.y);
+    }
+  } else {
+    const int nrow = (a.mmax + 1) * (a.mmax + 1);
+    for (int pos = lane; pos < nrow * padw; pos += kWave) {
+      const int row = pos / padw, col = pos - row * padw;
+      int l = 0;
+      while ((l + 1) * (l + 1) <= row) ++l;
+      const int mup = row - l * l - l;
+      double v = 0.0;
+      if (col < 2 * l + 1) {
+        const double2 e = phase_pow(q0, q1, mup);
+        const double2 r = Rd[roff(l) + (mup + l) * (2 * l + 1) + col];
+        v = r.x * e.x - r.y * e.y;
+      }
+      dtab[pair * nrow * padw + pos] = static_cast<T>(v);
+    }
+  }
+}
+
+// The OZ pass for angular grids with compile-time (mmax, mrso): one pair per
+// 64-lane block as in oz_pair_kernel, with every LDS access a lane base plus a
+// compile-time offset and every loop unrolled.  Layouts (W = 2 mmax + 1):
+//   G, Gm:  rows (m, mu2) of W entries, entry mup + m: the phased gathers,
+//           later gamma with entry chi + m (zero padding past 2m+1);
+//   Dq, Dmq: the rotated vectors in projection order;
+//   dP:     one leading zero row, then rows (m, mup) of W entries d^m_{mup,chi}
+//           at chi + m (the padded dtab rows of the pair).
+// Rotation: Dq(m, chi, mu2) = sum_j G[m, mu2][j] dP[m, row j][chi + m], the d
+// column read at stride W; the padded G entries are zero, so the rows j >= 2m+1
+// (the next m's rows) add nothing.  The OZ product runs over the n blocks
+// n >= |chi| with the term offset of block n a compile-time constant past the
+// lane's base.  Inverse rotation: out(m, mup, mu2) = sum_j gamma[j] dP[m, mup][j]
+// (and dP[m, mup][2m - j] for -q; the leading zero row keeps those reads in
+// the array, and gamma's zero padding cancels them).
+template <int MMAX, int MRSO>
+struct OzShape {
+  static constexpr int W = 2 * MMAX + 1;
+  static constexpr int km(int m) { return m / MRSO + 1; }
+  static constexpr int mbase(int m) {
+    int a = 0;
+    for (int k = 0; k < m; ++k) a += (2 * k + 1) * km(k);
+    return a;
+  }
+  static constexpr int grow(int m) {  // G rows before m
+    int a = 0;
+    for (int k = 0; k < m; ++k) a += km(k);
+    return a;
+  }
+  static constexpr int tofs(int n) {  // OZ terms of the n blocks before n
+    int a = 0;
+    for (int k = 0; k < n; ++k) a += 2 * (k / MRSO) + 1;
+    return a;
+  }
+  static constexpr int NP = mbase(MMAX + 1), NG = grow(MMAX + 1), ND = (MMAX + 1) * (MMAX + 1);
+  static constexpr int NSLOT = (NP + kWave - 1) / kWave;
+};
+
+// runtime values of the OzShape offsets (m, n <= MMAX: unrolled sums)
+template <int MMAX, int MRSO>
+__device__ __forceinline__ int grow_rt(int m) {
+  int a = 0;
+#pragma unroll
+  for (int k = 0; k < MMAX; ++k) a += k < m ? k / MRSO + 1 : 0;
+  return a;
+}
+template <int MMAX, int MRSO>
+__device__ __forceinline__ int tofs_rt(int n) {
+  int a = 0;
+#pragma unroll
+  for (int k = 0; k < MMAX; ++k) a += k < n ? 2 * (k / MRSO) + 1 : 0;
+  return a;
+}
+
+template <typename T, int MMAX, int MRSO>
+__global__ __launch_bounds__(kWave) void oz_pair_ct_kernel(cplx<T>* __restrict__ p, const T* __restrict__ dtab,
+                                                           OzArgs<T> a) {
+  using S = OzShape<MMAX, MRSO>;
+  constexpr int W = S::W;
+  extern __shared__ unsigned char lds_raw[];
+  cplx<T>* G = reinterpret_cast<cplx<T>*>(lds_raw);
+  cplx<T>* Gm = G + S::NG * W;
+  cplx<T>* Dq = Gm + S::NG * W;
+  cplx<T>* Dmq = Dq + S::NP;
+  cplx<T>* ceff = Dmq + S::NP;
+  T* dP0 = reinterpret_cast<T*>(ceff + a.np_new);  // leading zero row, then ND rows
+  T* dP = dP0 + W;
+  __shared__ cplx<T> ph[W];  // e^{-i mup phi_q}
+  const int lane = threadIdx.x;
+
+  const long b = blockIdx.x;
+  const long pair = (b % kXcds) * a.chunk + b / kXcds;
+  if (pair >= a.npairs) return;  // uniform across the block
+  const int pk = a.pairs[pair];
+  const int ix = pk & 1023, iy = (pk >> 10) & 1023, iz = (pk >> 20) & 1023;
+  const int ixm = ix == 0 ? 0 : a.nx - ix, iym = iy == 0 ? 0 : a.ny - iy, izm = iz == 0 ? 0 : a.nz - iz;
+  const long sq = (static_cast<long>(ix) * a.ny + iy) * a.nz + iz;
+  const long smq = (static_cast<long>(ixm) * a.ny + iym) * a.nz + izm;
+  const double q0 = a.kxyz[ix], q1 = a.kxyz[a.nx + iy], q2 = a.kxyz[a.nx + a.ny + iz];
+
+  // gathers (:420-425) and lane metadata first (plain scalars: arrays of
+  // HIP's vector classes are kept in scratch memory)
+  T g0x[S::NSLOT], g0y[S::NSLOT], g1x[S::NSLOT], g1y[S::NSLOT];
+  int info[S::NSLOT], ias[S::NSLOT];
+#pragma unroll
+  for (int t = 0; t < S::NSLOT; ++t) {
+    const int ip = min(lane + t * kWave, S::NP - 1);
+    const cplx<T> u = p[static_cast<long>(ip) * a.N + sq], v = p[static_cast<long>(ip) * a.N + smq];
+    g0x[t] = u.x;
+    g0y[t] = u.y;
+    g1x[t] = v.x;
+    g1y[t] = v.y;
+    info[t] = a.ipinfo[ip];
+    ias[t] = a.ia_start[ip];
+  }
+  // padded d rows of this pair
+  {
+    constexpr int ND = S::ND * W, NU = (ND + kWave - 1) / kWave;
+    const T* src = dtab + pair * ND;
+    T dreg[NU];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) dreg[u] = src[min(lane + u * kWave, ND - 1)];
+#pragma unroll
+    for (int u = 0; u < NU; ++u)
+      if (lane + u * kWave < ND) dP[lane + u * kWave] = dreg[u];
+    if (lane < W) dP0[lane] = T(0);
+  }
+  // zero G and Gm (the padding past 2m+1 must stay zero; the valid entries
+  // are written after the barrier)
+  for (int i = lane; i < 2 * S::NG * W; i += kWave) G[i] = mkc<T>(0, 0);
+  if (lane < W) {
+    const double2 e = phase_pow(q0, q1, lane - MMAX);
+    ph[lane] = mkc<T>(static_cast<T>(e.x), static_cast<T>(-e.y));
+  }
+  // c(|q|) rows iq, iq+1 lerped into LDS, kCBatch rows per lane in flight
+  {
+    const double qn = sqrt(q0 * q0 + q1 * q1 + q2 * q2);
+    const double eff = qn / a.dq;
+    int iq = static_cast<int>(eff);
+    const double alpha = eff - static_cast<double>(iq);
+    if (iq > a.nq - 2) iq = a.nq - 2;  // unreachable for |q| <= qmax (nq has 1.5 rows of margin)
+    const T al = static_cast<T>(alpha), bl = static_cast<T>(1.0 - alpha);
+    const cplx<T>* c0 = a.ctab + static_cast<long>(iq) * a.np_new;
+    const cplx<T>* c1 = c0 + a.np_new;
+    for (int ib = lane; ib < a.np_new; ib += kCBatch * kWave) {
+      T lx[kCBatch], ly[kCBatch], hx[kCBatch], hy[kCBatch];
+#pragma unroll
+      for (int u = 0; u < kCBatch; ++u) {
+        const int ia = min(ib + u * kWave, a.np_new - 1);
+        const cplx<T> lo = c0[ia], hi = c1[ia];
+        lx[u] = lo.x;
+        ly[u] = lo.y;
+        hx[u] = hi.x;
+        hy[u] = hi.y;
+      }
+#pragma unroll
+      for (int u = 0; u < kCBatch; ++u) {
+        const int ia = ib + u * kWave;
+        if (ia < a.np_new) ceff[ia] = mkc<T>(al * hx[u] + bl * lx[u], al * hy[u] + bl * ly[u]);
+      }
+    }
+  }
+  __syncthreads();
+
+  // phased gathers g' = g e^{-i mup phi} into G[(m, mu2)][mup + m]
+#pragma unroll
+  for (int t = 0; t < S::NSLOT; ++t) {
+    const int ip = lane + t * kWave;
+    if (ip < S::NP) {
+      const int m = info[t] & 0xff, mupm = (info[t] >> 8) & 0xff, mu2 = info[t] >> 16;
+      const cplx<T> e = ph[mupm];
+      const int at = (grow_rt<MMAX, MRSO>(m) + mu2) * W + mupm - MMAX + m;
+      G[at] = cmul<T>(mkc<T>(g0x[t], g0y[t]), e);
+      Gm[at] = cmul<T>(mkc<T>(g1x[t], g1y[t]), e);
+    }
+  }
+  __syncthreads();
+
+  // rotation to the q frame (:427-443): Dq(m, chi, mu2) = sum_mup g'_q d_{mup,chi},
+  // Dmq(m, chi, mu2) = (-1)^m sum_mup g'_-q d_{mup,-chi}
+#pragma unroll
+  for (int t = 0; t < S::NSLOT; ++t) {
+    const int ip = lane + t * kWave;
+    if (ip < S::NP) {
+      const int m = info[t] & 0xff, chi = ((info[t] >> 8) & 0xff) - MMAX, mu2 = info[t] >> 16;
+      const int row = grow_rt<MMAX, MRSO>(m) + mu2;
+      const cplx<T>* g0 = G + row * W;
+      const cplx<T>* g1 = Gm + row * W;
+      const T* d0 = dP + m * m * W + (chi + m);
+      const T* d1 = dP + m * m * W + (m - chi);
+      T ar = 0, ai = 0, br = 0, bi = 0;
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        const cplx<T> x = g0[j], y = g1[j];
+        const T u = d0[j * W], v = d1[j * W];
+        ar = fma(x.x, u, ar);
+        ai = fma(x.y, u, ai);
+        br = fma(y.x, v, br);
+        bi = fma(y.y, v, bi);
+      }
+      Dq[ip] = mkc<T>(ar, ai);
+      Dmq[ip] = (m & 1) ? mkc<T>(-br, -bi) : mkc<T>(br, bi);
+    }
+  }
+  __syncthreads();
+
+  // OZ product (:468-496) and chop (:514-517): output (m, chi, mu2), terms
+  // (n, nu2) for n >= |chi| in the order of the table rows; gamma goes to
+  // G[(m, mu2)][chi + m]
+#pragma unroll
+  for (int t = 0; t < S::NSLOT; ++t) {
+    const int ip = lane + t * kWave;
+    if (ip < S::NP) {
+      const int m = info[t] & 0xff, chi = ((info[t] >> 8) & 0xff) - MMAX, mu2 = info[t] >> 16;
+      const int achi = chi < 0 ? -chi : chi;
+      const cplx<T>* ce = ceff + (ias[t] - tofs_rt<MMAX, MRSO>(achi));
+      cplx<T> s0 = mkc<T>(0, 0), s1 = mkc<T>(0, 0);
+#pragma unroll
+      for (int n = 0; n <= MMAX; ++n) {
+        if (n >= achi) {
+          const int h = n / MRSO, kn = n / MRSO + 1;
+          const cplx<T>* cn = ce + S::tofs(n);
+          const int srcb = S::mbase(n) + (chi + n) * kn;
+#pragma unroll
+          for (int k = 0; k <= 2 * h; ++k) {
+            const int nu2 = k - h;
+            const cplx<T> c = cn[k];
+            if (nu2 < 0) {
+              const int src = srcb - nu2;
+              s0 = cfma<T>(s0, c, Dq[src]);
+              s1 = cfma<T>(s1, c, Dmq[src]);
+            } else {
+              const int src = srcb + nu2;
+              s0 = cfma_conjb<T>(s0, c, Dmq[src]);
+              s1 = cfma_conjb<T>(s1, c, Dq[src]);
+            }
+          }
+        }
+      }
+      const int at = (grow_rt<MMAX, MRSO>(m) + mu2) * W + chi + m;
+      G[at] = chop<T>(s0, a.chop_eps);
+      Gm[at] = chop<T>(s1, a.chop_eps);
+    }
+  }
+  __syncthreads();
+
+  // inverse rotation and stores (:518-541): out(m, mup, mu2) =
+  // e^{+i mup phi} sum_chi gamma d_{mup,chi}
+  const bool self = (ixm == ix && iym == iy && izm == iz);
+  const bool singular = self && (ix == a.nx / 2 || iy == a.ny / 2 || iz == a.nz / 2);
+#pragma unroll
+  for (int t = 0; t < S::NSLOT; ++t) {
+    const int ip = lane + t * kWave;
+    if (ip < S::NP) {
+      const int m = info[t] & 0xff, mupm = (info[t] >> 8) & 0xff, mu2 = info[t] >> 16;
+      const int mup = mupm - MMAX;
+      const int row = grow_rt<MMAX, MRSO>(m) + mu2;
+      const cplx<T>* g0 = G + row * W;
+      const cplx<T>* g1 = Gm + row * W;
+      const T* dr = dP + (m * m + mup + m) * W;
+      const T* drm = dr + 2 * m;  // d_{mup,-chi} = drm[-j]; reads past the row start meet gamma's zeros
+      T ar = 0, ai = 0, br = 0, bi = 0;
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        const cplx<T> x = g0[j], y = g1[j];
+        const T u = dr[j], v = drm[-j];
+        ar = fma(x.x, u, ar);
+        ai = fma(x.y, u, ai);
+        br = fma(y.x, v, br);
+        bi = fma(y.y, v, bi);
+      }
+      const cplx<T> e = ph[mupm];  // e^{-i mup phi}; the output takes its conjugate
+      const cplx<T> o0 = cmul<T>(mkc<T>(ar, ai), cconj<T>(e));
+      cplx<T> o1 = cmul<T>(mkc<T>(br, bi), cconj<T>(e));
+      if (m & 1) o1 = mkc<T>(-o1.x, -o1.y);
+      p[static_cast<long>(ip) * a.N + sq] = o0;
+      p[static_cast<long>(ip) * a.N + smq] = singular ? cconj<T>(o1) : o1;
+    }
   }
 }
 
@@ -230,19 +513,35 @@ __global__ __launch_bounds__(kWave) void oz_pair_kernel(cplx<T>* __restrict__ p,
   const long smq = (static_cast<long>(ixm) * a.ny + iym) * a.nz + izm;
   const double q0 = a.kxyz[ix], q1 = a.kxyz[a.nx + iy], q2 = a.kxyz[a.nx + a.ny + iz];
 
-  // gathers of g(q), g(-q) (:420-425), issued first
-  cplx<T> g0[4], g1[4];
+  // gathers of g(q), g(-q) (:420-425) and this lane's projection metadata,
+  // issued first and together
+  // (unconditional loads at clamped indices; plain scalars: arrays of HIP's
+  // vector classes are kept in scratch memory)
+  T g0x[4], g0y[4], g1x[4], g1y[4];
+  int info[4], ias[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
-    const int ip = lane + t * kWave;
-    if (ip < np) {
-      g0[t] = p[static_cast<long>(ip) * a.N + sq];
-      g1[t] = p[static_cast<long>(ip) * a.N + smq];
+    const int ip = min(lane + t * kWave, np - 1);
+    const cplx<T> u = p[static_cast<long>(ip) * a.N + sq], v = p[static_cast<long>(ip) * a.N + smq];
+    g0x[t] = u.x;
+    g0y[t] = u.y;
+    g1x[t] = v.x;
+    g1y[t] = v.y;
+    info[t] = a.ipinfo[ip];
+    ias[t] = a.ia_start[ip];
+  }
+  // this pair's d matrices (coalesced, all loads in flight at once) and phases
+  {
+    const T* dsrc = dtab + pair * nR;
+    T dreg[kMaxR / kWave];
+#pragma unroll
+    for (int u = 0; u < kMaxR / kWave; ++u) dreg[u] = dsrc[min(lane + u * kWave, nR - 1)];
+#pragma unroll
+    for (int u = 0; u < kMaxR / kWave; ++u) {
+      const int it = lane + u * kWave;
+      if (it < nR) dm[it] = dreg[u];
     }
   }
-  // this pair's d matrices (coalesced) and phases
-  const T* dsrc = dtab + pair * nR;
-  for (int it = lane; it < nR; it += kWave) dm[it] = dsrc[it];
   if (lane <= 2 * mmax) {
     const double2 e = phase_pow(q0, q1, lane - mmax);
     ph[lane] = mkc<T>(static_cast<T>(e.x), static_cast<T>(-e.y));
@@ -265,9 +564,24 @@ __global__ __launch_bounds__(kWave) void oz_pair_kernel(cplx<T>* __restrict__ p,
   const cplx<T>* c0 = a.ctab + static_cast<long>(iq) * a.np_new;
   const cplx<T>* c1 = c0 + a.np_new;
   if (STAGE_C) {
-    for (int ia = lane; ia < a.np_new; ia += kWave) {
-      const cplx<T> lo = c0[ia], hi = c1[ia];
-      ceff[ia] = mkc<T>(al * hi.x + bl * lo.x, al * hi.y + bl * lo.y);
+    // batches of kCBatch rows per lane with all their loads in flight
+    for (int ib = lane; ib < a.np_new; ib += kCBatch * kWave) {
+      // plain scalars: arrays of HIP's vector classes are kept in scratch memory
+      T lx[kCBatch], ly[kCBatch], hx[kCBatch], hy[kCBatch];
+#pragma unroll
+      for (int u = 0; u < kCBatch; ++u) {
+        const int ia = min(ib + u * kWave, a.np_new - 1);
+        const cplx<T> lo = c0[ia], hi = c1[ia];
+        lx[u] = lo.x;
+        ly[u] = lo.y;
+        hx[u] = hi.x;
+        hy[u] = hi.y;
+      }
+#pragma unroll
+      for (int u = 0; u < kCBatch; ++u) {
+        const int ia = ib + u * kWave;
+        if (ia < a.np_new) ceff[ia] = mkc<T>(al * hx[u] + bl * lx[u], al * hy[u] + bl * ly[u]);
+      }
     }
   }
   __syncthreads();
@@ -277,19 +591,20 @@ __global__ __launch_bounds__(kWave) void oz_pair_kernel(cplx<T>* __restrict__ p,
   for (int t = 0; t < 4; ++t) {
     const int ip = lane + t * kWave;
     if (ip < np) {
-      const int info = a.ipinfo[ip];
-      const cplx<T> e = ph[(info >> 8) & 0xff];
-      gq[ip] = cmul<T>(g0[t], e);
-      gmq[ip] = cmul<T>(g1[t], e);
+      const cplx<T> e = ph[(info[t] >> 8) & 0xff];
+      gq[ip] = cmul<T>(mkc<T>(g0x[t], g0y[t]), e);
+      gmq[ip] = cmul<T>(mkc<T>(g1x[t], g1y[t]), e);
     }
   }
   __syncthreads();
 
   // rotation to the q frame: d_q[m,chi,mu2] = sum_mup g'_q d_{mup,chi};
   // d_-q[m,chi,mu2] = (-1)^m sum_mup g'_-q d_{mup,-chi}
-  for (int ip = lane; ip < np; ip += kWave) {
-    const int info = a.ipinfo[ip];
-    const int m = info & 0xff, chi = ((info >> 8) & 0xff) - mmax, mu2 = info >> 16;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int ip = lane + t * kWave;
+    if (ip >= np) continue;
+    const int m = info[t] & 0xff, chi = ((info[t] >> 8) & 0xff) - mmax, mu2 = info[t] >> 16;
     const int km = m / a.mrso + 1, w = 2 * m + 1;
     const cplx<T>* g0p = gq + mbase[m] + mu2;
     const cplx<T>* g1p = gmq + mbase[m] + mu2;
@@ -310,11 +625,13 @@ __global__ __launch_bounds__(kWave) void oz_pair_kernel(cplx<T>* __restrict__ p,
   __syncthreads();
 
   // OZ product: for output (m, chi, mu2), terms (n, nu2), n >= |chi|
-  for (int ip = lane; ip < np; ip += kWave) {
-    const int info = a.ipinfo[ip];
-    const int chi = ((info >> 8) & 0xff) - mmax;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int ip = lane + t * kWave;
+    if (ip >= np) continue;
+    const int chi = ((info[t] >> 8) & 0xff) - mmax;
     const int achi = chi < 0 ? -chi : chi;
-    int ia = a.ia_start[ip];
+    int ia = ias[t];
     cplx<T> s0 = mkc<T>(0, 0), s1 = mkc<T>(0, 0);
     for (int n = achi; n <= mmax; ++n) {
       const int kn = n / a.mrso + 1;
@@ -353,9 +670,11 @@ __global__ __launch_bounds__(kWave) void oz_pair_kernel(cplx<T>* __restrict__ p,
   // inverse rotation and stores: out[m,mup,mu2] = e^{+i mup phi} sum_chi gamma d_{mup,chi}
   const bool self = (ixm == ix && iym == iy && izm == iz);
   const bool singular = self && (ix == a.nx / 2 || iy == a.ny / 2 || iz == a.nz / 2);
-  for (int ip = lane; ip < np; ip += kWave) {
-    const int info = a.ipinfo[ip];
-    const int m = info & 0xff, mup = ((info >> 8) & 0xff) - mmax, mu2 = info >> 16;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int ip = lane + t * kWave;
+    if (ip >= np) continue;
+    const int m = info[t] & 0xff, mup = ((info[t] >> 8) & 0xff) - mmax, mu2 = info[t] >> 16;
     const int km = m / a.mrso + 1, w = 2 * m + 1;
     const cplx<T>* g0p = gq + mbase[m] + mu2;
     const cplx<T>* g1p = gmq + mbase[m] + mu2;
@@ -384,6 +703,22 @@ T* upload(const std::vector<T>& h) {
   MDFT_HIP_CHECK(hipMalloc(&d, std::max<size_t>(1, h.size()) * sizeof(T)));
   if (!h.empty()) MDFT_HIP_CHECK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
   return d;
+}
+
+// the angular grids with a compile-time OZ kernel: mmax 1..6 with mrso 1 or 2
+template <int MMAX = 1, int MRSO = 1, typename F>
+bool dispatch_oz(int mmax, int mrso, F&& f) {
+  if constexpr (MMAX > kMaxM) {
+    return false;
+  } else if constexpr (MRSO > 2) {
+    return dispatch_oz<MMAX + 1, 1>(mmax, mrso, f);
+  } else {
+    if (mmax == MMAX && mrso == MRSO) {
+      f(std::integral_constant<int, MMAX>{}, std::integral_constant<int, MRSO>{});
+      return true;
+    }
+    return dispatch_oz<MMAX, MRSO + 1>(mmax, mrso, f);
+  }
 }
 
 }  // namespace
@@ -457,6 +792,7 @@ ConvDevice::ConvDevice(int nx, int ny, int nz, double lx, double ly, double lz, 
   coef.insert(coef.end(), &rc.b[0][0][0], &rc.b[0][0][0] + 5 * 13 * 13);
   coef.insert(coef.end(), &rc.c[0][0], &rc.c[0][0] + 5 * 13);
   coef.insert(coef.end(), &rc.d[0][0], &rc.d[0][0] + 5 * 13);
+  if (dispatch_oz(mmax, mrso, [](auto, auto) {})) padw = 2 * mmax + 1;
   kxyz = upload(k3);
   pairs = upload(packed);
   ia_start = upload(tab.ia_start);
@@ -502,10 +838,11 @@ const T* ConvDevice::dtable(hipStream_t s) const {
   void*& slot = std::is_same<T, double>::value ? dtab64 : dtab32;
   if (!slot) {
     const size_t nR = static_cast<size_t>(tab.mmax + 1) * (4 * (tab.mmax + 1) * (tab.mmax + 1) - 1) / 3;
-    MDFT_HIP_CHECK(hipMalloc(&slot, std::max<size_t>(1, npairs * nR * sizeof(T))));
+    const size_t row = padw ? static_cast<size_t>(tab.mmax + 1) * (tab.mmax + 1) * padw : nR;
+    MDFT_HIP_CHECK(hipMalloc(&slot, std::max<size_t>(1, npairs * row * sizeof(T))));
     OzArgs<T> a = oz_args<T>(*this, 0.0);
     hipLaunchKernelGGL((oz_dtable_kernel<T>), dim3(static_cast<unsigned>(npairs)), dim3(kWave),
-                       nR * sizeof(double2), s, static_cast<T*>(slot), a);
+                       nR * sizeof(double2), s, static_cast<T*>(slot), a, padw);
     MDFT_HIP_CHECK(hipGetLastError());
   }
   return static_cast<const T*>(slot);
@@ -519,6 +856,19 @@ void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s) const {
   OzArgs<T> a = oz_args<T>(*this, chop_eps);
   const long nblk = a.chunk * kXcds;
   throw_if(nblk > 0x7fffffffL, "too many q pairs");
+  if (padw) {
+    dispatch_oz(tab.mmax, tab.mrso, [&](auto mm, auto mr) {
+      constexpr int MMAX = decltype(mm)::value, MRSO = decltype(mr)::value;
+      using Sh = OzShape<MMAX, MRSO>;
+      const size_t lds = (2 * static_cast<size_t>(Sh::NG) * Sh::W + 2 * Sh::NP + tab.np_new) * sizeof(cplx<T>) +
+                         static_cast<size_t>(Sh::W) * (Sh::ND + 1) * sizeof(T);
+      throw_if(lds > 64 * 1024, "angular grid too large for the OZ kernel");
+      hipLaunchKernelGGL((oz_pair_ct_kernel<T, MMAX, MRSO>), dim3(static_cast<unsigned>(nblk)), dim3(kWave), lds, s,
+                         p, dt, a);
+    });
+    MDFT_HIP_CHECK(hipGetLastError());
+    return;
+  }
   const size_t lds = oz_lds_bytes(sizeof(T));
   throw_if(lds > 64 * 1024, "angular grid too large for the OZ kernel");
   constexpr bool stage = std::is_same<T, float>::value;

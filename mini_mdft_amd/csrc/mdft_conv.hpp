@@ -30,6 +30,9 @@ struct ConvDevice {
   const T* dtable(hipStream_t s) const;
   mutable void* dtab64 = nullptr;
   mutable void* dtab32 = nullptr;
+  // row width of the padded d layout when the compile-time OZ kernel serves
+  // this (mmax, mrso) (2 mmax + 1), else 0 (compact rows, generic kernel)
+  int padw = 0;
 
   // the per-(q, -q) rotation / OZ / inverse-rotation pass on FFT'd data;
   // gamma components below chop_eps in magnitude are set to 0

@@ -30,13 +30,13 @@ def _np(t):
     return t.detach().cpu().numpy()
 
 
-def _problem(oracle, n, L, mmax, seed=1):
+def _problem(oracle, n, L, mmax, seed=1, mrso=MRSO):
     g = M.SpatialGrid(n, L)
-    ag = M.AngularGrid(mmax, MRSO)
+    ag = M.AngularGrid(mmax, mrso)
     mp = M.OrientationProjectionMap(ag)
     qmax = luc.qmax_of_grid(n, L)
-    tab = luc.synthetic_c(mmax, MRSO, luc.nq_for(qmax, DQ), seed=seed)
-    ocv = oracle.Conv(n, L, mmax, MRSO, tab.tuples, tab.c, tab.dq)
+    tab = luc.synthetic_c(mmax, mrso, luc.nq_for(qmax, DQ), seed=seed)
+    ocv = oracle.Conv(n, L, mmax, mrso, tab.tuples, tab.c, tab.dq)
     return g, ag, mp, tab, ocv
 
 
@@ -117,6 +117,22 @@ def test_conv_execute(oracle, n, L, mmax, prec, tol):
     # accessors
     assert np.abs(conv.mnmunukhi_q() - ocv.table()).max() <= 1e-14 * np.abs(ocv.table()).max()
     assert np.array_equal(conv.gamma_p_map(), oracle.qpairs(*n))
+
+
+# other molecular symmetry orders: the compile-time OZ kernel serves mmax <= 6
+# with mrso 1 or 2, the generic kernel the rest (mrso = 3 here)
+@pytest.mark.parametrize("n,L,mmax,mrso", [((6, 5, 4), (7.0, 7.0, 7.0), 3, 1), ((5, 5, 6), (6.0, 6.0, 6.0), 2, 1),
+                                           ((6, 6, 6), (8.0, 8.0, 8.0), 6, 2), ((4, 5, 4), (6.0, 6.0, 6.0), 3, 3)])
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-11), (FP32, 2e-5)])
+def test_conv_execute_mrso(oracle, n, L, mmax, mrso, prec, tol):
+    g, ag, mp, tab, ocv = _problem(oracle, n, L, mmax, seed=9, mrso=mrso)
+    conv = M.Convolution.from_table(tab, g, ag, mp, prec)
+    r = np.random.default_rng(6)
+    p = r.standard_normal((ag.m_np,) + n) + 1j * r.standard_normal((ag.m_np,) + n)
+    ref = ocv.execute(p)
+    t = _to(p, conv.complex_dtype)
+    conv.execute(t)
+    assert rel_err(_np(t), ref) < tol
 
 
 def test_conv_from_file_equals_from_table(oracle, tmp_path):
