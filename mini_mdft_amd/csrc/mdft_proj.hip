@@ -339,19 +339,47 @@ __global__ __launch_bounds__(64) void proj2angl_wave_kernel(const cplx<TP>* __re
 
   double acc_ff = 0.0;
   const long base = r0 * A::kNo + th * A::kNjk;
+  constexpr int kE = kWaveRows * A::kNjk / 64;  // elements per lane
+  if (xi != nullptr && r0 + kWaveRows <= N) {
+    // full block: batches of kB elements per lane with all their xi loads in
+    // flight before the arithmetic
+    constexpr int kB = 11;
+    for (int i0 = 0; i0 < kE; i0 += kB) {
+      double xv[kB];
+#pragma unroll
+      for (int b = 0; b < kB; ++b) {
+        const int e = lane + 64 * min(i0 + b, kE - 1);
+        const int rl = e / A::kNjk, jk = e - rl * A::kNjk;
+        xv[b] = static_cast<double>(xi[base + static_cast<long>(rl) * A::kNo + jk]);
+      }
+#pragma unroll
+      for (int b = 0; b < kB; ++b) {
+        if (i0 + b < kE) {
+          const int e = lane + 64 * (i0 + b);
+          const int rl = e / A::kNjk, jk = e - rl * A::kNjk;
+          const long g = base + static_cast<long>(rl) * A::kNo + jk;
+          const double x = xv[b];
+          const double vx = static_cast<double>(static_cast<TO>(stage[e]));
+          acc_ff += vx * wq[th * A::kNjk + jk] * (x * x - 1.0);
+          df[g] = static_cast<TO>(2.0 * rho0 * x * vx);
+        }
+      }
+    }
+  } else {
 #pragma unroll 5
-  for (int e = lane; e < kWaveRows * A::kNjk; e += 64) {
-    const int rl = e / A::kNjk, jk = e - rl * A::kNjk;
-    if (r0 + rl < N) {
-      const long g = base + static_cast<long>(rl) * A::kNo + jk;
-      const TC v = stage[e];
-      if (xi == nullptr) {
-        o[g] = static_cast<TO>(v);
-      } else {
-        const double x = static_cast<double>(xi[g]);
-        const double vx = static_cast<double>(static_cast<TO>(v));
-        acc_ff += vx * wq[th * A::kNjk + jk] * (x * x - 1.0);
-        df[g] = static_cast<TO>(2.0 * rho0 * x * vx);
+    for (int e = lane; e < kWaveRows * A::kNjk; e += 64) {
+      const int rl = e / A::kNjk, jk = e - rl * A::kNjk;
+      if (r0 + rl < N) {
+        const long g = base + static_cast<long>(rl) * A::kNo + jk;
+        const TC v = stage[e];
+        if (xi == nullptr) {
+          o[g] = static_cast<TO>(v);
+        } else {
+          const double x = static_cast<double>(xi[g]);
+          const double vx = static_cast<double>(static_cast<TO>(v));
+          acc_ff += vx * wq[th * A::kNjk + jk] * (x * x - 1.0);
+          df[g] = static_cast<TO>(2.0 * rho0 * x * vx);
+        }
       }
     }
   }
