@@ -6,27 +6,36 @@ Workload (the reference's 64^3 / mmax = 5 configuration): a 64^3 grid in a
 projections, 131 076 (q, -q) pairs), spce n0 = 0.0332891 A^-3 at 298.15 K,
 synthetic c(q) for all 2002 nmax = 5 Luc tuples, xi = 1 + 0.1 U(-1, 1).
 One step = one Solver::energy_cproj_mrso evaluation (F_exc and df) with xi
-already resident in HBM.  value = evaluations/s of the MDFT_MIXED form; the
-plain FP64 form is timed beside it, and |dF| and the gradient's relative L2
-error of the mixed form are stated against it.  The CPU baseline is the FP64
-oracle (C++/OpenMP restatement of the reference) on the host cores.
+already resident in HBM.
+
+value = evaluations/s of MDFT_MIXED_X6, the mixed form that meets |dF| <= 0.01
+kJ/mol (FFT, rotation, OZ in FP32; angl2proj as a BF16x6 MFMA GEMM; proj2angl
+in FP32; FP64 rotation tables and reductions).  The plain FP64 form and the
+BF16x3 form (projections as BF16x3 MFMA GEMMs both ways) are timed beside it;
+|dF| and the gradient's relative L2 error of each mixed form are stated against
+FP64.  The CPU baseline is the FP64 oracle (C++/OpenMP restatement of the
+reference) timed on the host cores.
 """
 from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
+import re
 import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+METRIC = "F+grad evals/s at 64³ mmax=5 (mixed vs FP64) at |ΔF|≤0.01 kJ/mol; p99 solve"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+FP32_PEAK_TFLOPS = 157.3
 N_SIDE, BOX, MMAX, MRSO = 64, 30.0, 5, 2
 TEMP, N0 = 298.15, 0.0332891
 C_SEED, XI_SEED, XI_AMP = 2024, 12345, 0.1
+OZ_FLOP_PER_PAIR = 52344  # rotations, OZ product, phases (both members of the pair)
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "oz_pair_64c_mmax5_traffic.md")
 
 
 def stage_bytes(N, no, np_, form):
@@ -63,8 +72,6 @@ def timed(solver, xi, df, ffd, steps, warmup):
 
 
 def cpu_baseline(tab, n, L, rho0, pref, evals):
-    import numpy as np
-
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
@@ -79,9 +86,20 @@ def cpu_baseline(tab, n, L, rho0, pref, evals):
     for _ in range(evals):
         ff, df = C.energy_cproj(xi, rho0, pref)
     dt = time.perf_counter() - t0
-    return dict(value=evals / dt, unit="evals/s", cores=threads, kind="port",
-                sample=f"{evals} full FP64 evaluations at 64^3 mmax=5 after 1 warm-up (oracle, OpenMP "
-                       f"{threads} threads, {dt:.1f} s)"), ff, float(np.linalg.norm(df))
+    return dict(value=round(evals / dt, 4), unit="evals/s", cores=threads, kind="port",
+                sample=f"{evals} full FP64 evaluations of the same 64^3 mmax=5 workload after 1 warm-up "
+                       f"(oracle: C++/OpenMP restatement of the reference, {threads} threads, {dt:.1f} s)"), ff, df
+
+
+def traffic_from_summary():
+    """HBM bytes per OZ launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (None if absent)."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", f.read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
 
 
 def main():
@@ -93,11 +111,10 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py runs on one GPU: the c-proj hot path of one solute is a single-GPU workload here "
-              "(multi-GPU sharding is a design note, not a measured configuration)", file=sys.stderr)
+        print("bench.py measures one GPU: the c-proj evaluation of one solute is a single-GPU workload here "
+              "(sharding it over GPUs is a design note, not a measured configuration)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
 
     from mini_mdft_amd import FP64, MIXED, MIXED_X6, luc
@@ -116,9 +133,15 @@ def main():
     xi = torch.empty((N, no), dtype=torch.float64, device=dev)
     M.fill_uniform(xi, XI_SEED, 1.0 - XI_AMP, 1.0 + XI_AMP)
     ffd = torch.zeros(1, dtype=torch.float64, device=dev)
+    # traffic calibration for counter runs: one elementwise pass of known
+    # size (delta_rho: one read and one write of N*no doubles)
+    calib = torch.empty_like(xi)
+    M.get_delta_rho(xi, calib, rho0)
+    torch.cuda.synchronize()
+    del calib
 
     results = {}
-    for form, prec in (("mixed", MIXED), ("mixed_x6", MIXED_X6), ("fp64", FP64)):
+    for form, prec in (("mixed_x6", MIXED_X6), ("bf16x3", MIXED), ("fp64", FP64)):
         conv = M.Convolution.from_table(tab, g, ag, mp, prec)
         solver = M.Solver(conv, rho0, pref, prec)
         df = torch.empty_like(xi)
@@ -129,22 +152,33 @@ def main():
               f"{ {k: round(v, 4) for k, v in stages.items()} }", file=sys.stderr, flush=True)
         del solver, conv
 
-    mix, f64, x6 = results["mixed"], results["fp64"], results["mixed_x6"]
-    dF = mix["ff"] - f64["ff"]
-    grad_rel = float(torch.linalg.norm(mix["df"] - f64["df"]) / torch.linalg.norm(f64["df"]))
-    dF_x6 = x6["ff"] - f64["ff"]
-    grad_rel_x6 = float(torch.linalg.norm(x6["df"] - f64["df"]) / torch.linalg.norm(f64["df"]))
-    value = args.steps / mix["sec"]
+    f64 = results["fp64"]
+    nrm64 = torch.linalg.norm(f64["df"])
+
+    def err(r):
+        return r["ff"] - f64["ff"], float(torch.linalg.norm(r["df"] - f64["df"]) / nrm64)
+
+    x6, x3 = results["mixed_x6"], results["bf16x3"]
+    dF, grad_rel = err(x6)
+    dF3, grad_rel3 = err(x3)
+    value = args.steps / x6["sec"]
     fp64_rate = args.steps / f64["sec"]
 
     by = stage_bytes(N, no, np_, "mixed")
-    stages = {}
-    for k, ms in mix["stages"].items():
-        stages[k] = dict(ms=round(ms, 5), GBs=round(by[k] / (ms * 1e-3) / 1e9, 1))
-    dom = max(mix["stages"], key=mix["stages"].get)
-    ach = by[dom] / (mix["stages"][dom] * 1e-3) / 1e9
-    roofline = dict(kernel=dom, bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, algorithmic_bytes=by[dom])
+    stages = {k: dict(ms=round(ms, 5), GBs=round(by[k] / (ms * 1e-3) / 1e9, 1)) for k, ms in x6["stages"].items()}
+    eval_bytes = sum(by.values())
+    # dominant kernel: the OZ pass (one launch per evaluation); its time is
+    # the HIP-event interval on the solver's stream around that launch
+    oz_ms = x6["stages"]["oz"]
+    oz_bytes = by["oz"]
+    npairs = 131076
+    achieved = oz_bytes / (oz_ms * 1e-3) / 1e9
+    oz_tflops = OZ_FLOP_PER_PAIR * npairs / (oz_ms * 1e-3) / 1e12
+    roofline = dict(kernel="oz_pair_ct_kernel<float,5,2>", bound="hbm", achieved=round(achieved, 1),
+                    peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
+                    traffic=traffic_from_summary(), algorithmic_bytes=oz_bytes, launch_ms=round(oz_ms, 5),
+                    fp32_tflops=round(oz_tflops, 2), fp32_frac=round(oz_tflops / FP32_PEAK_TFLOPS, 4),
+                    eval_hbm_frac=round(eval_bytes * value / 1e9 / HBM_PEAK_GBS, 4))
 
     cpu = None
     if args.cpu_evals > 0:
@@ -153,35 +187,35 @@ def main():
         cpu["fp64_gpu_vs_cpu_rel"] = abs(f64["ff"] - ff_cpu) / abs(ff_cpu)
 
     line = {
-        "metric": "F+grad evals/s at 64^3 mmax=5 (mixed vs FP64) at |dF|<=0.01 kJ/mol; p99 solve",
+        "metric": METRIC,
         "value": round(value, 2),
         "unit": "evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * mix["sec"] / args.steps, 4),
+        "ms_per_step": round(1e3 * x6["sec"] / args.steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+bf16x3 (f64 reductions)",
-        "data": "synthetic",
-        "config": {"workload": "c-proj F_exc + gradient, 64^3 grid, 30 A box, mmax=5 (330 orientations, "
-                               "88 projections, 131076 q-pairs), spce n0, synthetic c(q) nmax=5",
-                   "grid": N_SIDE, "box_A": BOX, "mmax": MMAX, "mrso": MRSO},
+        "dtype": "f32 (bf16x6 mfma angl2proj, f64 tables and reductions)",
+        "data": "synthetic (seeded xi and c(q); no external files)",
+        "config": {"workload": "c-proj F_exc + gradient, 64^3 grid, 30 A box, mmax=5, mrso=2 (330 orientations, "
+                               "88 projections, 131076 q-pairs), spce n0, T=298.15 K, synthetic c(q) nmax=5",
+                   "grid": N_SIDE, "box_A": BOX, "mmax": MMAX, "mrso": MRSO, "form": "MDFT_MIXED_X6"},
+        "dF_vs_fp64_kJmol": dF,
+        "grad_rel_l2_vs_fp64": grad_rel,
         "fp64_evals_per_s": round(fp64_rate, 2),
-        "speedup_mixed_vs_fp64": round(value / fp64_rate, 3),
-        "F_fp64": f64["ff"],
-        "dF_mixed_vs_fp64": dF,
-        "grad_rel_l2_mixed_vs_fp64": grad_rel,
-        "mixed_x6_evals_per_s": round(args.steps / x6["sec"], 2),
-        "dF_mixed_x6_vs_fp64": dF_x6,
-        "grad_rel_l2_mixed_x6_vs_fp64": grad_rel_x6,
-        "stages_mixed": stages,
+        "speedup_vs_fp64": round(value / fp64_rate, 3),
+        "F_fp64_kJmol": f64["ff"],
+        "bf16x3": {"evals_per_s": round(args.steps / x3["sec"], 2), "dF_vs_fp64_kJmol": dF3,
+                   "grad_rel_l2_vs_fp64": grad_rel3, "meets_dF_bar": abs(dF3) <= 0.01},
+        "stages_ms_GBs": stages,
         "stages_fp64_ms": {k: round(v, 5) for k, v in f64["stages"].items()},
+        "p99_solve": None,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line))
+    print(json.dumps(line, ensure_ascii=False))
 
 
 if __name__ == "__main__":
