@@ -4,7 +4,7 @@
  * a nonzero status on failure; mdft_last_error() then returns the message of
  * the calling thread's last failure (the reference throws std::runtime_error
  * from Impl::Throw_If, common/MDFT_Asserts.hpp:15-28; the C++ facade in
- * mini_mdft_amd/cpp rethrows it).  No exception crosses this boundary.
+ * include/mdft/ rethrows it).  No exception crosses this boundary.
  *
  * Device pointers are HIP device memory; `stream` is a hipStream_t (NULL =
  * the default stream).  Calls are stream ordered and do not synchronise,
