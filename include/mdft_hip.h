@@ -171,6 +171,58 @@ int mdft_solver_stage_ms(mdft_solver_t h, double* ms, long* count);
 int mdft_fill_uniform(void* x, size_t n, unsigned long long seed, unsigned long long offset, double lo,
                       double hi, int dtype, void* stream);
 
+/* ---- external potential -------------------------------------------------------
+ * AngularGrid's rotation tables (common/MDFT_Grid.hpp:265-333, m_rotxx ..
+ * m_rotzz): rot[io * 9 + 3 a + b] = R_ab of orientation io, Euler ZYZ with
+ * theta from the GL nodes, phi_j = j 2 pi / nphi, psi_k = k 2 pi / mrso / npsi
+ * (host output). */
+int mdft_rotation_matrices(int mmax, int mrso, double* rot);
+
+/* V_ext(r, o) in kJ/mol, (N, no) doubles on the device: the sum over the sites
+ * of the solvent molecule in orientation o at grid node r = (i lx/nx, j ly/ny,
+ * k lz/nz) and over the solute sites of Lennard-Jones (Lorentz-Berthelot
+ * mixing, between sites with sigma > 0) and Constants::qfact q q / d
+ * (common/MDFT_Constants.hpp:46), d the minimum-image distance floored at
+ * 1e-6 A, capped at vmax.  solute (nsolute x 6) and solvent (nsolvent x 6) are
+ * host rows (charge e, sigma A, epsilon kJ/mol, x, y, z A), the solvent's in
+ * the molecular frame (Solute sites: common/MDFT_Solute.hpp:126-191; solvent
+ * sites: common/MDFT_Solvent.hpp:417-532).  The reference builds these tables
+ * but never evaluates V_ext (common/MDFT_Solvent.hpp:160).  Synchronises. */
+int mdft_vext(int nx, int ny, int nz, double lx, double ly, double lz, int mmax, int mrso, int nsolute,
+              const double* solute, int nsolvent, const double* solvent, double vmax, double* vext,
+              void* stream);
+
+/* ---- full functional and minimiser (no counterpart in the reference: its
+ * Solver::run makes one excess evaluation, src/mini-mdft.hpp:129-133, and the
+ * iteration settings are unused, common/MDFT_System.hpp:188-189) --------------
+ *   F_id  = kT rho0 dv sum w (xi^2 ln xi^2 - xi^2 + 1)
+ *   F_ext = rho0 dv sum w xi^2 V
+ *   F_exc = (dv / N) ff              (ff of mdft_energy_cproj)
+ *   g     = dv w (2 rho0 xi (kT ln xi^2 + V) + (2 prefactor / N) df)
+ * dv = lx ly lz / N.  The functional takes its grid, rho0, prefactor and
+ * precision form from the solver (MDFT_FP64, MDFT_MIXED or MDFT_MIXED_X6: xi
+ * and g are doubles); vext (device, N x no) is borrowed and must outlive the
+ * handle.  F (host) receives F, F_id, F_ext, F_exc.  Both calls synchronise. */
+typedef struct mdft_functional_s* mdft_functional_t;
+int mdft_functional_create(mdft_solver_t solver, const double* vext, double kT, mdft_functional_t* out);
+int mdft_functional_destroy(mdft_functional_t h);
+int mdft_functional_eval(mdft_functional_t h, const double* xi, double* g, double* F, void* stream);
+
+/* L-BFGS (FP64 vectors, dot products and line search) from xi, in place:
+ * m stored pairs (1..8), diagonal initial inverse Hessian gamma h0 with
+ * h0 = 1 / (2 rho0 dv w kT) and gamma = s'y / y'(h0 y) of the newest pair,
+ * Armijo backtracking (c1 = 1e-4, halving from t = 1, at most 30 trials), a
+ * pair kept when s'y > 1e-12 |s| |y|, a restart from -h0 g on a non-descent
+ * direction; converged when max |h0 g| < gtol.  Outputs (host; history may be
+ * NULL, else max_iter + 1 doubles receiving F at the start and after every
+ * iteration): F[4] at the returned point, the iterations made, the
+ * functional evaluations, status (0 converged, 1 max_iter reached, 2 no
+ * Armijo point: the decrease is below what F resolves) and max |h0 g|.  The
+ * handle keeps its (2 m + 4) N no doubles of work space between calls. */
+int mdft_lbfgs_minimize(mdft_functional_t h, double* xi, int m, int max_iter, double gtol, double* F,
+                        int* iterations, int* evaluations, int* status, double* residual, double* history,
+                        void* stream);
+
 #ifdef __cplusplus
 }
 #endif

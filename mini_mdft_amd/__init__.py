@@ -7,6 +7,7 @@ from ._lib import DOUBLE, FLOAT, FP32, FP64, MIXED, MIXED_X6, MDFTError  # noqa:
 from .mdft import (  # noqa: F401
     AngularGrid,
     Convolution,
+    Functional,
     OrientationProjectionMap,
     OrientationProjectionTransform,
     SOLVENTS,
@@ -19,4 +20,6 @@ from .mdft import (  # noqa: F401
     kT,
     prefactor_of,
     rho0_of,
+    rotation_matrices,
+    vext,
 )

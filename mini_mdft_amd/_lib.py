@@ -60,6 +60,12 @@ _SIGS = {
     "mdft_solver_set_timing": (_i, [_vp, _i]),
     "mdft_solver_stage_ms": (_i, [_vp, _pd, ctypes.POINTER(ctypes.c_long)]),
     "mdft_fill_uniform": (_i, [_vp, _sz, _ull, _ull, _d, _d, _i, _vp]),
+    "mdft_rotation_matrices": (_i, [_i, _i, _pd]),
+    "mdft_vext": (_i, [_i, _i, _i, _d, _d, _d, _i, _i, _i, _pd, _i, _pd, _d, _vp, _vp]),
+    "mdft_functional_create": (_i, [_vp, _vp, _d, ctypes.POINTER(_vp)]),
+    "mdft_functional_destroy": (_i, [_vp]),
+    "mdft_functional_eval": (_i, [_vp, _vp, _vp, _pd, _vp]),
+    "mdft_lbfgs_minimize": (_i, [_vp, _vp, _i, _i, _d, _pd, _pi, _pi, _pi, _pd, _pd, _vp]),
 }
 
 
@@ -81,6 +87,14 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise MDFTError(f"{LIB_PATH} is not built: run __graft_entry__.build() (or make -C "
                             f"mini_mdft_amd/csrc)")
+        # torch first: its HIP runtime (NEEDED as libamdhip64.so, soname
+        # libamdhip64.so.7) is then the one the library binds to.  Loaded the
+        # other way round, the process holds two HIP runtimes and the second
+        # one sees no device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             fn = getattr(L, name)

@@ -10,6 +10,7 @@
 
 #include "../../include/mdft_hip.h"
 #include "mdft_conv.hpp"
+#include "mdft_functional.hpp"
 #include "mdft_gemm.hpp"
 #include "mdft_proj.hpp"
 
@@ -493,6 +494,79 @@ int mdft_fill_uniform(void* x, size_t n, unsigned long long seed, unsigned long 
       fill_uniform<double>(static_cast<double*>(x), n, seed, offset, lo, hi, as_stream(stream));
     else
       fill_uniform<float>(static_cast<float*>(x), n, seed, offset, lo, hi, as_stream(stream));
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// external potential, full functional and minimiser
+struct mdft_functional_s {
+  mdft_solver_t solver;  // borrowed
+  std::unique_ptr<Functional> f;
+};
+
+extern "C" {
+
+int mdft_rotation_matrices(int mmax, int mrso, double* rot) {
+  return guarded([&] {
+    throw_if(!rot, "rotation_matrices: null output");
+    const std::vector<double> R = rotation_matrices(make_angular_grid(mmax, mrso));
+    std::memcpy(rot, R.data(), R.size() * sizeof(double));
+  });
+}
+
+int mdft_vext(int nx, int ny, int nz, double lx, double ly, double lz, int mmax, int mrso, int nsolute,
+              const double* solute, int nsolvent, const double* solvent, double vmax, double* vext, void* stream) {
+  return guarded([&] {
+    compute_vext(nx, ny, nz, lx, ly, lz, make_angular_grid(mmax, mrso), solute, nsolute, solvent, nsolvent, vmax,
+                 vext, as_stream(stream));
+  });
+}
+
+int mdft_functional_create(mdft_solver_t solver, const double* vext, double kT, mdft_functional_t* out) {
+  return guarded([&] {
+    throw_if(!out || !solver || !vext, "functional_create: null argument");
+    *out = nullptr;
+    throw_if(solver->precision == MDFT_FP32, "functional: the FP32 form has float fields; use MDFT_FP64 or a mixed form");
+    const ConvTables& t = solver->conv->dev->tab;
+    FunctionalParams p{solver->N, t.ang.no, solver->rho0, solver->prefactor, kT,
+                       t.lx * t.ly * t.lz / static_cast<double>(solver->N)};
+    auto h = std::make_unique<mdft_functional_s>();
+    h->solver = solver;
+    h->f = std::make_unique<Functional>(p, t.ang.w, vext, [solver](const double* xi, double* df, double* ffd,
+                                                                   hipStream_t s) { energy_impl(solver, xi, df, ffd, s); });
+    *out = h.release();
+  });
+}
+
+int mdft_functional_destroy(mdft_functional_t h) {
+  return guarded([&] { delete h; });
+}
+
+int mdft_functional_eval(mdft_functional_t h, const double* xi, double* g, double* F, void* stream) {
+  return guarded([&] {
+    throw_if(!h, "functional_eval: null handle");
+    h->f->eval(xi, g, F, as_stream(stream));
+  });
+}
+
+int mdft_lbfgs_minimize(mdft_functional_t h, double* xi, int m, int max_iter, double gtol, double* F,
+                        int* iterations, int* evaluations, int* status, double* residual, double* history,
+                        void* stream) {
+  return guarded([&] {
+    throw_if(!h, "lbfgs: null handle");
+    LbfgsOptions o;
+    o.m = m;
+    o.max_iter = max_iter;
+    o.gtol = gtol;
+    const LbfgsResult r = h->f->minimize(xi, o, as_stream(stream));
+    if (F) std::memcpy(F, r.F, sizeof(r.F));
+    if (iterations) *iterations = r.iterations;
+    if (evaluations) *evaluations = r.evaluations;
+    if (status) *status = r.status;
+    if (residual) *residual = r.residual;
+    if (history) std::memcpy(history, r.history.data(), r.history.size() * sizeof(double));
   });
 }
 

@@ -1,0 +1,648 @@
+// The full functional, the solute's external potential and an FP64 L-BFGS
+// minimiser on the GPU.
+//
+// The reference makes one excess evaluation (src/mini-mdft.hpp:129-133) and
+// leaves the external potential and the iteration settings unused
+// (common/MDFT_Solvent.hpp:160, common/MDFT_System.hpp:188-189).  The terms
+// used here, per grid node r and orientation o with quadrature weight w_o:
+//   F_id  = kT rho0 dv sum w (xi^2 ln xi^2 - xi^2 + 1)
+//   F_ext = rho0 dv sum w xi^2 V
+//   F_exc = (dv / N) ff        ff from energy_cproj; its unnormalised FFTs
+//                              make ff N times the discrete integral
+//   g     = dv w (2 rho0 xi (kT ln xi^2 + V) + (2 prefactor / N) df)
+// with df = 2 rho0 xi vexc, the reference's gradient array
+// (common/MDFT_Solvent.hpp:643-657).  xi^2 is floored at 1e-300 inside ln.
+//
+// The minimiser is L-BFGS with the diagonal initial inverse Hessian gamma h0,
+// h0 = 1 / (2 rho0 dv w kT), Armijo backtracking and restarts on a
+// non-descent direction.  It runs in the vector-free form of Chen, Wang, Zhou
+// and Gao ("Large-scale L-BFGS using MapReduce", NIPS 2014): the two-loop
+// recursion is carried out on the host in the coefficients of the direction
+// over the basis {s_i, y_i, g}, from the Gram entries s_i.y_j, y_i'H y_j,
+// s_i.g, y_i'H g, g'H g (H = diag(h0)).  Every iteration then makes two
+// fused passes over HBM: one that forms the trial point x + t d (and its F_id,
+// F_ext partial sums), one that forms g, y = g_new - g and all new Gram
+// entries at once.  All sums are FP64 with fixed-order reductions.
+#include "mdft_functional.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <utility>
+
+namespace mdft {
+namespace {
+
+constexpr int kFThreads = 256;
+constexpr int kFWaves = kFThreads / 64;
+constexpr int kMaxBlocks = 2048;
+constexpr int kMaxSums = 5 * kLbfgsMaxPairs + 8;  // update pass: 5K + 7 sums and one max
+constexpr int kFfSlot = kMaxSums;                 // ff of the excess evaluation
+
+// Constants the element passes share.
+struct Elem {
+  const double* w;   // no
+  const double* h0;  // no
+  double rho0, kT, dv, gscale;  // gscale = 2 prefactor / N
+  long n;                       // N * no
+  int no, step;                 // step = (gridDim * threads) mod no
+};
+
+__device__ __forceinline__ double local_grad(double x, double v, double df, double wio, const Elem& e) {
+  const double x2 = fmax(x * x, 1e-300);
+  return (e.dv * wio) * (2.0 * e.rho0 * x * (e.kT * log(x2) + v) + e.gscale * df);
+}
+
+// accumulates w (x2 ln x2 - x2 + 1) and w x2 V
+__device__ __forceinline__ void local_f(double x, double v, double wio, double& fid, double& fext) {
+  const double x2 = fmax(x * x, 1e-300);
+  fid += wio * (x2 * log(x2) - x2 + 1.0);
+  fext += wio * x2 * v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+// One partial per block and accumulator, accumulator k at part[k * nb + block];
+// with LAST_MAX the last accumulator is a maximum.
+template <int NA, bool LAST_MAX>
+__device__ __forceinline__ void store_partials(const double (&acc)[NA], double* __restrict__ part, int nb) {
+  __shared__ double red[kFWaves][NA];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NA; ++k) {
+    const double v = (LAST_MAX && k == NA - 1) ? wave_max(acc[k]) : wave_sum(acc[k]);
+    if (lane == 0) red[wave][k] = v;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < NA; k += kFThreads) {
+    double v = red[0][k];
+    for (int w = 1; w < kFWaves; ++w) v = (LAST_MAX && k == NA - 1) ? fmax(v, red[w][k]) : v + red[w][k];
+    part[static_cast<long>(k) * nb + blockIdx.x] = v;
+  }
+}
+
+// the element loop of every pass: i over [0, n) with a grid stride, io = i mod no
+#define MDFT_ELEMENT_LOOP(e)                                                        \
+  const long stride_ = static_cast<long>(gridDim.x) * kFThreads;                  \
+  long i = static_cast<long>(blockIdx.x) * kFThreads + threadIdx.x;               \
+  int io = static_cast<int>(i % (e).no);                                          \
+  for (; i < (e).n; i += stride_, io = (io + (e).step >= (e).no) ? io + (e).step - (e).no : io + (e).step)
+
+// g, F_id and F_ext sums, g'H g and max |h0 g| at x
+__global__ __launch_bounds__(kFThreads) void local_kernel(const double* __restrict__ x, const double* __restrict__ V,
+                                                          const double* __restrict__ df, double* __restrict__ g,
+                                                          Elem e, double* __restrict__ part, int nb) {
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  MDFT_ELEMENT_LOOP(e) {
+    const double xv = x[i], v = V[i], wio = e.w[io], h = e.h0[io];
+    local_f(xv, v, wio, acc[0], acc[1]);
+    const double gv = local_grad(xv, v, df[i], wio, e);
+    g[i] = gv;
+    const double hg = h * gv;
+    acc[2] += gv * hg;
+    acc[3] = fmax(acc[3], fabs(hg));
+  }
+  store_partials<4, true>(acc, part, nb);
+}
+
+struct DirArgs {
+  const double* x;
+  const double* g;
+  const double* V;
+  const double* s[kLbfgsMaxPairs];
+  const double* y[kLbfgsMaxPairs];
+  double cs[kLbfgsMaxPairs], cq[kLbfgsMaxPairs];
+  double cg, gamma, t;
+  double* xt;  // x + t d
+  double* sc;  // xt - x (the candidate s)
+};
+
+// d = -(gamma h0 (cg g + sum cq_j y_j) + sum cs_j s_j); xt = x + t d; sc = xt - x;
+// F_id and F_ext sums at xt
+template <int K>
+__global__ __launch_bounds__(kFThreads) void direction_kernel(DirArgs a, Elem e, double* __restrict__ part, int nb) {
+  double acc[2] = {0.0, 0.0};
+  MDFT_ELEMENT_LOOP(e) {
+    double q = a.cg * a.g[i];
+#pragma unroll
+    for (int j = 0; j < K; ++j) q = fma(a.cq[j], a.y[j][i], q);
+    double r = a.gamma * e.h0[io] * q;
+#pragma unroll
+    for (int j = 0; j < K; ++j) r = fma(a.cs[j], a.s[j][i], r);
+    const double xv = a.x[i];
+    const double xn = xv + a.t * (-r);
+    a.xt[i] = xn;
+    a.sc[i] = xn - xv;
+    local_f(xn, a.V[i], e.w[io], acc[0], acc[1]);
+  }
+  store_partials<2, false>(acc, part, nb);
+}
+
+struct UpdArgs {
+  const double* xt;
+  const double* V;
+  const double* df;
+  double* g;  // in: g at x, out: g at xt
+  const double* sc;
+  double* yn;  // out: g_new - g_old
+  const double* s[kLbfgsMaxPairs];
+  const double* y[kLbfgsMaxPairs];
+};
+
+// g at the accepted point, y = g_new - g_old and the new Gram entries:
+//   [0, K) sc.y_j | K sc.yn | [K+1, 2K+1) s_j.yn | 2K+1 sc.sc | 2K+2 yn.yn |
+//   [2K+3, 3K+3) s_j.g | 3K+3 sc.g | [3K+4, 4K+4) yn'H y_j | 4K+4 yn'H yn |
+//   [4K+5, 5K+5) y_j'H g | 5K+5 yn'H g | 5K+6 g'H g | 5K+7 max |h0 g|
+template <int K>
+__global__ __launch_bounds__(kFThreads) void update_kernel(UpdArgs a, Elem e, double* __restrict__ part, int nb) {
+  constexpr int NA = 5 * K + 8;
+  constexpr int KA = K > 0 ? K : 1;
+  double acc[NA];
+#pragma unroll
+  for (int k = 0; k < NA; ++k) acc[k] = 0.0;
+  MDFT_ELEMENT_LOOP(e) {
+    const double wio = e.w[io], h = e.h0[io];
+    const double gn = local_grad(a.xt[i], a.V[i], a.df[i], wio, e);
+    const double yv = gn - a.g[i];
+    a.g[i] = gn;
+    a.yn[i] = yv;
+    const double scv = a.sc[i];
+    double sj[KA], yj[KA];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      sj[j] = a.s[j][i];
+      yj[j] = a.y[j][i];
+    }
+    const double hy = h * yv, hg = h * gn;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      acc[j] += scv * yj[j];
+      acc[K + 1 + j] += sj[j] * yv;
+      acc[2 * K + 3 + j] += sj[j] * gn;
+      acc[3 * K + 4 + j] += hy * yj[j];
+      acc[4 * K + 5 + j] += yj[j] * hg;
+    }
+    acc[K] += scv * yv;
+    acc[2 * K + 1] += scv * scv;
+    acc[2 * K + 2] += yv * yv;
+    acc[3 * K + 3] += scv * gn;
+    acc[4 * K + 4] += hy * yv;
+    acc[5 * K + 5] += yv * hg;
+    acc[5 * K + 6] += gn * hg;
+    acc[5 * K + 7] = fmax(acc[5 * K + 7], fabs(hg));
+  }
+  store_partials<NA, true>(acc, part, nb);
+}
+#undef MDFT_ELEMENT_LOOP
+
+// out[row] = sum (or, for row == max_row, max) of part[row * nb .. + nb), in a fixed order
+__global__ __launch_bounds__(kFThreads) void reduce_rows_kernel(const double* __restrict__ part, int nb, int max_row,
+                                                                double* __restrict__ out) {
+  const int row = blockIdx.x;
+  const bool is_max = row == max_row;
+  double v = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kFThreads) {
+    const double p = part[static_cast<long>(row) * nb + i];
+    v = is_max ? fmax(v, p) : v + p;
+  }
+  __shared__ double red[kFThreads];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int st = kFThreads / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] = is_max ? fmax(red[threadIdx.x], red[threadIdx.x + st])
+                                                    : red[threadIdx.x] + red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[row] = red[0];
+}
+
+// ---- external potential ----------------------------------------------------
+struct VextArgs {
+  long n;
+  int no, nyz, nz, nsolv, nsolute;
+  double dx, dy, dz, lx, ly, lz, ilx, ily, ilz, vmax;
+};
+
+// V(r, o) = min(vmax, sum_k sum_j [LJ_kj(d) + qq_kj / d]) with d the
+// minimum-image distance between solvent site k of orientation o at node r
+// and solute site j; pair[(k nsolute + j) * 3] = (sigma, 4 eps, qfact q q),
+// sigma = 0 where either site has no LJ.
+__global__ __launch_bounds__(kFThreads) void vext_kernel(VextArgs a, const double* __restrict__ off,
+                                                         const double* __restrict__ pair,
+                                                         const double* __restrict__ solute, double* __restrict__ V) {
+  for (long idx = static_cast<long>(blockIdx.x) * kFThreads + threadIdx.x; idx < a.n;
+       idx += static_cast<long>(gridDim.x) * kFThreads) {
+    const long r = idx / a.no;
+    const int io = static_cast<int>(idx - r * a.no);
+    const int ix = static_cast<int>(r / a.nyz);
+    const int rem = static_cast<int>(r - static_cast<long>(ix) * a.nyz);
+    const int iy = rem / a.nz, iz = rem - iy * a.nz;
+    const double gx = ix * a.dx, gy = iy * a.dy, gz = iz * a.dz;
+    double v = 0.0;
+    for (int k = 0; k < a.nsolv; ++k) {
+      const double* o = off + (static_cast<long>(io) * a.nsolv + k) * 3;
+      const double px = gx + o[0], py = gy + o[1], pz = gz + o[2];
+      for (int j = 0; j < a.nsolute; ++j) {
+        double ddx = px - solute[3 * j], ddy = py - solute[3 * j + 1], ddz = pz - solute[3 * j + 2];
+        ddx -= a.lx * rint(ddx * a.ilx);
+        ddy -= a.ly * rint(ddy * a.ily);
+        ddz -= a.lz * rint(ddz * a.ilz);
+        const double d = fmax(sqrt(ddx * ddx + ddy * ddy + ddz * ddz), 1e-6);
+        const double inv = 1.0 / d;
+        const double* pp = pair + (static_cast<long>(k) * a.nsolute + j) * 3;
+        if (pp[0] > 0.0) {
+          const double sr = pp[0] * inv, sr2 = sr * sr, sr6 = sr2 * sr2 * sr2;
+          v += pp[1] * (sr6 * sr6 - sr6);
+        }
+        v += pp[2] * inv;
+      }
+    }
+    V[idx] = fmin(v, a.vmax);
+  }
+}
+
+// Constants::qfact (common/MDFT_Constants.hpp:36-47): qfact q q / r in kJ/mol
+constexpr double kQunit = 1.602176565e-19, kNavo = 6.02214129e23, kEps0 = 8.854187817e-12;
+constexpr double kPi = 3.14159265358979323846;
+const double kQfact = kQunit * kQunit * 1.0e-3 * kNavo / (4.0 * kPi * kEps0 * 1.0e-10);
+
+template <typename F>
+void dispatch_k(int k, F&& f) {
+  switch (k) {
+    case 0: f(std::integral_constant<int, 0>{}); break;
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 5: f(std::integral_constant<int, 5>{}); break;
+    case 6: f(std::integral_constant<int, 6>{}); break;
+    case 7: f(std::integral_constant<int, 7>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    default: throw Error("L-BFGS: too many stored pairs");
+  }
+}
+static_assert(kLbfgsMaxPairs == 8, "dispatch_k covers 0 .. 8 pairs");
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+std::vector<double> rotation_matrices(const AngularGrid& g) {
+  // dphi, dpsi and phi_j = j dphi, psi_k = k dpsi as AngularGrid sets them up
+  // (common/MDFT_Grid.hpp:141-142, :181-182)
+  const double dphi = 2.0 * kPi / g.nphi, dpsi = 2.0 * kPi / g.mrso / g.npsi;
+  std::vector<double> R(static_cast<size_t>(g.no) * 9);
+  int io = 0;
+  for (int it = 0; it < g.ntheta; ++it)
+    for (int ip = 0; ip < g.nphi; ++ip)
+      for (int ik = 0; ik < g.npsi; ++ik, ++io) {
+        const double ct = std::cos(g.theta[it]), st = std::sin(g.theta[it]);
+        const double cp = std::cos(ip * dphi), sp = std::sin(ip * dphi);
+        const double cs = std::cos(ik * dpsi), ss = std::sin(ik * dpsi);
+        double* r = &R[static_cast<size_t>(io) * 9];
+        r[0] = ct * cp * cs - sp * ss;
+        r[1] = -ct * cp * ss - sp * cs;
+        r[2] = st * cp;
+        r[3] = ct * sp * cs + cp * ss;
+        r[4] = -ct * sp * ss + cp * cs;
+        r[5] = st * sp;
+        r[6] = -st * cs;
+        r[7] = st * ss;
+        r[8] = ct;
+      }
+  return R;
+}
+
+void compute_vext(int nx, int ny, int nz, double lx, double ly, double lz, const AngularGrid& g,
+                  const double* solute, int nsolute, const double* solvent, int nsolvent, double vmax, double* vext,
+                  hipStream_t s) {
+  throw_if(nx <= 0 || ny <= 0 || nz <= 0 || !(lx > 0) || !(ly > 0) || !(lz > 0), "vext: bad grid");
+  throw_if(nsolute < 0 || nsolvent <= 0 || (nsolute > 0 && !solute) || !solvent || !vext, "vext: bad sites");
+  const std::vector<double> R = rotation_matrices(g);
+  std::vector<double> off(static_cast<size_t>(g.no) * nsolvent * 3);
+  for (int io = 0; io < g.no; ++io)
+    for (int k = 0; k < nsolvent; ++k)
+      for (int c = 0; c < 3; ++c) {
+        const double* r = &R[static_cast<size_t>(io) * 9 + 3 * c];
+        const double* sv = solvent + 6 * k + 3;
+        off[(static_cast<size_t>(io) * nsolvent + k) * 3 + c] = r[0] * sv[0] + r[1] * sv[1] + r[2] * sv[2];
+      }
+  std::vector<double> pair(static_cast<size_t>(nsolvent) * std::max(nsolute, 1) * 3, 0.0);
+  std::vector<double> pos(static_cast<size_t>(std::max(nsolute, 1)) * 3, 0.0);
+  for (int j = 0; j < nsolute; ++j)
+    for (int c = 0; c < 3; ++c) pos[3 * j + c] = solute[6 * j + 3 + c];
+  for (int k = 0; k < nsolvent; ++k)
+    for (int j = 0; j < nsolute; ++j) {
+      const double qk = solvent[6 * k], sk = solvent[6 * k + 1], ek = solvent[6 * k + 2];
+      const double qj = solute[6 * j], sj = solute[6 * j + 1], ej = solute[6 * j + 2];
+      double* p = &pair[(static_cast<size_t>(k) * nsolute + j) * 3];
+      if (sk > 0 && sj > 0) {  // Lorentz-Berthelot mixing
+        p[0] = 0.5 * (sk + sj);
+        p[1] = 4.0 * std::sqrt(ek * ej);
+      }
+      p[2] = kQfact * qk * qj;
+    }
+  const long n = static_cast<long>(nx) * ny * nz * g.no;
+  // one-time tables: plain allocations, freed on every exit path
+  struct Buf {
+    double* p = nullptr;
+    ~Buf() {
+      if (p) (void)hipFree(p);
+    }
+  } doff, dpair, dpos;
+  MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&doff.p), off.size() * sizeof(double)));
+  MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&dpair.p), pair.size() * sizeof(double)));
+  MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&dpos.p), pos.size() * sizeof(double)));
+  MDFT_HIP_CHECK(hipMemcpyAsync(doff.p, off.data(), off.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  MDFT_HIP_CHECK(hipMemcpyAsync(dpair.p, pair.data(), pair.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  MDFT_HIP_CHECK(hipMemcpyAsync(dpos.p, pos.data(), pos.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  VextArgs a{n, g.no, ny * nz, nz, nsolvent, nsolute, lx / nx, ly / ny, lz / nz, lx, ly, lz,
+             1.0 / lx, 1.0 / ly, 1.0 / lz, vmax};
+  hipLaunchKernelGGL(vext_kernel, dim3(grid_for(n, kFThreads, 65536)), dim3(kFThreads), 0, s, a, doff.p, dpair.p,
+                     dpos.p, vext);
+  MDFT_HIP_CHECK(hipGetLastError());
+  // the tables (host and device) are released on return
+  MDFT_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------------------
+Functional::Functional(const FunctionalParams& p, const std::vector<double>& w, const double* vext, ExcessEval excess)
+    : p_(p), excess_(std::move(excess)), vext_(vext) {
+  throw_if(p.N <= 0 || p.no <= 0 || static_cast<int>(w.size()) != p.no, "functional: bad sizes");
+  throw_if(!vext, "functional: null vext");
+  throw_if(!(p.kT > 0) || !(p.dv > 0) || !(p.rho0 > 0), "functional: kT, dv and rho0 must be positive");
+  n_ = p.N * p.no;
+  const long want = (n_ + kFThreads * 16L - 1) / (kFThreads * 16L);
+  nb_ = static_cast<int>(std::min<long>(std::max<long>(want, 1), kMaxBlocks));
+  step_ = static_cast<int>((static_cast<long>(nb_) * kFThreads) % p.no);
+  std::vector<double> tab(2 * static_cast<size_t>(p.no));
+  for (int io = 0; io < p.no; ++io) {
+    tab[io] = w[io];
+    tab[p.no + io] = 1.0 / (2 * p.rho0 * p.dv * w[io] * p.kT);
+  }
+  try {
+    MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&tab_), tab.size() * sizeof(double)));
+    MDFT_HIP_CHECK(hipMemcpy(tab_, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&df_), n_ * sizeof(double)));
+    MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&part_), static_cast<size_t>(kMaxSums) * nb_ * sizeof(double)));
+    MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&res_), (kMaxSums + 1) * sizeof(double)));
+    MDFT_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_), (kMaxSums + 1) * sizeof(double)));
+  } catch (...) {
+    release();
+    throw;
+  }
+}
+
+Functional::~Functional() { release(); }
+
+void Functional::release() {
+  if (tab_) (void)hipFree(tab_);
+  if (df_) (void)hipFree(df_);
+  if (part_) (void)hipFree(part_);
+  if (res_) (void)hipFree(res_);
+  if (host_) (void)hipHostFree(host_);
+  if (work_) (void)hipFree(work_);
+  tab_ = df_ = part_ = res_ = host_ = work_ = nullptr;
+}
+
+size_t Functional::bytes() const {
+  return (2 * static_cast<size_t>(p_.no) + n_ + static_cast<size_t>(kMaxSums) * nb_ + kMaxSums + 1) * sizeof(double) +
+         static_cast<size_t>(2 * m_alloc_ + 4) * (m_alloc_ ? n_ : 0) * sizeof(double);
+}
+
+size_t Functional::minimize_bytes(int m) const { return static_cast<size_t>(2 * m + 4) * n_ * sizeof(double); }
+
+void Functional::reserve(int m) {
+  if (m <= m_alloc_ && work_) return;
+  if (work_) (void)hipFree(work_);
+  work_ = nullptr;
+  m_alloc_ = 0;
+  MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&work_), minimize_bytes(m)));
+  m_alloc_ = m;
+}
+
+const double* Functional::read_results(int count, hipStream_t s) {
+  (void)count;
+  MDFT_HIP_CHECK(hipMemcpyAsync(host_, res_, (kMaxSums + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
+  MDFT_HIP_CHECK(hipStreamSynchronize(s));
+  return host_;
+}
+
+namespace {
+void reduce_rows(const double* part, int nb, int rows, int max_row, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(reduce_rows_kernel, dim3(rows), dim3(kFThreads), 0, s, part, nb, max_row, out);
+  MDFT_HIP_CHECK(hipGetLastError());
+}
+}  // namespace
+
+void Functional::eval(const double* xi, double* g, double F[4], hipStream_t s) {
+  throw_if(!xi || !g || !F, "functional eval: null argument");
+  excess_(xi, df_, res_ + kFfSlot, s);
+  const Elem e{tab_, tab_ + p_.no, p_.rho0, p_.kT, p_.dv, 2.0 * p_.prefactor / p_.N, n_, p_.no, step_};
+  hipLaunchKernelGGL(local_kernel, dim3(nb_), dim3(kFThreads), 0, s, xi, vext_, df_, g, e, part_, nb_);
+  MDFT_HIP_CHECK(hipGetLastError());
+  reduce_rows(part_, nb_, 4, 3, res_, s);
+  const double* h = read_results(4, s);
+  F[1] = p_.kT * p_.rho0 * p_.dv * h[0];
+  F[2] = p_.rho0 * p_.dv * h[1];
+  F[3] = p_.dv / p_.N * h[kFfSlot];
+  F[0] = F[1] + F[2] + F[3];
+}
+
+LbfgsResult Functional::minimize(double* xi, const LbfgsOptions& o, hipStream_t s) {
+  throw_if(!xi, "lbfgs: null xi");
+  throw_if(o.m < 1 || o.m > kLbfgsMaxPairs, "lbfgs: m must be in 1 .. 8");
+  throw_if(o.max_iter < 0 || o.max_backtrack < 1 || !(o.gtol >= 0) || !(o.c1 > 0 && o.c1 < 1),
+           "lbfgs: bad options");
+  reserve(o.m);
+  const int m = o.m, nslot = m + 1;
+  const Elem e{tab_, tab_ + p_.no, p_.rho0, p_.kT, p_.dv, 2.0 * p_.prefactor / p_.N, n_, p_.no, step_};
+  double* x = xi;
+  double* xt = work_;
+  double* g = work_ + n_;
+  std::vector<double*> S(nslot), Y(nslot);
+  for (int j = 0; j < nslot; ++j) {
+    S[j] = work_ + (2 + 2 * static_cast<long>(j)) * n_;
+    Y[j] = work_ + (3 + 2 * static_cast<long>(j)) * n_;
+  }
+  auto fparts = [&](double sid, double sext, double ff, double* F) {
+    F[1] = p_.kT * p_.rho0 * p_.dv * sid;
+    F[2] = p_.rho0 * p_.dv * sext;
+    F[3] = p_.dv / p_.N * ff;
+    F[0] = F[1] + F[2] + F[3];
+  };
+
+  LbfgsResult r;
+  // the start point
+  excess_(x, df_, res_ + kFfSlot, s);
+  hipLaunchKernelGGL(local_kernel, dim3(nb_), dim3(kFThreads), 0, s, x, vext_, df_, g, e, part_, nb_);
+  MDFT_HIP_CHECK(hipGetLastError());
+  reduce_rows(part_, nb_, 4, 3, res_, s);
+  const double* h = read_results(4, s);
+  double F[4];
+  fparts(h[0], h[1], h[kFfSlot], F);
+  double GHG = h[2], gmax = h[3];
+  r.evaluations = 1;
+  r.history.push_back(F[0]);
+
+  // stored pairs, oldest first, as slot indices; Gram entries by position
+  std::vector<int> order;
+  std::vector<double> SY(m * m), YHY(m * m), SG(m), YHG(m);
+  auto sy = [&](int i, int j) -> double& { return SY[i * m + j]; };
+  auto yhy = [&](int i, int j) -> double& { return YHY[i * m + j]; };
+
+  int it = 0;
+  r.status = 1;
+  for (; it < o.max_iter; ++it) {
+    if (gmax < o.gtol) {
+      r.status = 0;
+      break;
+    }
+    int k = static_cast<int>(order.size());
+    // two-loop recursion in coefficients: q = cg g + sum cq_j y_j,
+    // r = gamma h0 q + sum cs_j s_j, d = -r
+    const double cg = 1.0;
+    double cq[kLbfgsMaxPairs] = {}, cs[kLbfgsMaxPairs] = {}, al[kLbfgsMaxPairs] = {};
+    for (int i = k - 1; i >= 0; --i) {
+      double sq = cg * SG[i];
+      for (int j = 0; j < k; ++j) sq += cq[j] * sy(i, j);
+      al[i] = sq / sy(i, i);
+      cq[i] -= al[i];
+    }
+    double gamma = k ? sy(k - 1, k - 1) / yhy(k - 1, k - 1) : 1.0;
+    for (int i = 0; i < k; ++i) {
+      double yq = cg * YHG[i];
+      for (int j = 0; j < k; ++j) yq += cq[j] * yhy(i, j);
+      double yr = gamma * yq;
+      for (int j = 0; j < k; ++j) yr += cs[j] * sy(j, i);
+      cs[i] += al[i] - yr / sy(i, i);
+    }
+    double gq = cg * GHG;
+    for (int j = 0; j < k; ++j) gq += cq[j] * YHG[j];
+    double gd = gamma * gq;
+    for (int j = 0; j < k; ++j) gd += cs[j] * SG[j];
+    gd = -gd;
+    if (!(gd < 0)) {  // not a descent direction: restart from -h0 g
+      order.clear();
+      k = 0;
+      gamma = 1.0;
+      gd = -GHG;
+    }
+    int c = 0;
+    while (std::find(order.begin(), order.end(), c) != order.end()) ++c;
+
+    DirArgs da{};
+    da.x = x;
+    da.g = g;
+    da.V = vext_;
+    for (int j = 0; j < k; ++j) {
+      da.s[j] = S[order[j]];
+      da.y[j] = Y[order[j]];
+      da.cs[j] = cs[j];
+      da.cq[j] = cq[j];
+    }
+    da.cg = cg;
+    da.gamma = gamma;
+    da.xt = xt;
+    da.sc = S[c];
+    double t = 1.0, Fn[4] = {0, 0, 0, 0};
+    bool accepted = false;
+    for (int b = 0; b < o.max_backtrack; ++b) {
+      da.t = t;
+      dispatch_k(k, [&](auto kc) {
+        hipLaunchKernelGGL((direction_kernel<decltype(kc)::value>), dim3(nb_), dim3(kFThreads), 0, s, da, e, part_,
+                           nb_);
+      });
+      MDFT_HIP_CHECK(hipGetLastError());
+      reduce_rows(part_, nb_, 2, -1, res_, s);
+      excess_(xt, df_, res_ + kFfSlot, s);
+      h = read_results(2, s);
+      fparts(h[0], h[1], h[kFfSlot], Fn);
+      ++r.evaluations;
+      if (std::isfinite(Fn[0]) && Fn[0] <= F[0] + o.c1 * t * gd) {
+        accepted = true;
+        break;
+      }
+      t *= 0.5;
+    }
+    if (!accepted) {
+      r.status = 2;
+      break;
+    }
+
+    UpdArgs ua{};
+    ua.xt = xt;
+    ua.V = vext_;
+    ua.df = df_;
+    ua.g = g;
+    ua.sc = S[c];
+    ua.yn = Y[c];
+    for (int j = 0; j < k; ++j) {
+      ua.s[j] = S[order[j]];
+      ua.y[j] = Y[order[j]];
+    }
+    dispatch_k(k, [&](auto kc) {
+      hipLaunchKernelGGL((update_kernel<decltype(kc)::value>), dim3(nb_), dim3(kFThreads), 0, s, ua, e, part_, nb_);
+    });
+    MDFT_HIP_CHECK(hipGetLastError());
+    const int K = k;
+    reduce_rows(part_, nb_, 5 * K + 8, 5 * K + 7, res_, s);
+    h = read_results(5 * K + 8, s);
+    std::vector<double> sn_y(h, h + K), s_yn(h + K + 1, h + 2 * K + 1), yn_hy(h + 3 * K + 4, h + 4 * K + 4);
+    const double snyn = h[K], snsn = h[2 * K + 1], ynyn = h[2 * K + 2], sn_g = h[3 * K + 3], ynhyn = h[4 * K + 4],
+                 yn_hg = h[5 * K + 5];
+    for (int j = 0; j < K; ++j) {
+      SG[j] = h[2 * K + 3 + j];
+      YHG[j] = h[4 * K + 5 + j];
+    }
+    GHG = h[5 * K + 6];
+    gmax = h[5 * K + 7];
+    if (snyn > 1e-12 * std::sqrt(snsn * ynyn)) {  // keep the pair
+      if (k == m) {  // drop the oldest
+        order.erase(order.begin());
+        for (int i = 0; i + 1 < m; ++i) {
+          for (int j = 0; j + 1 < m; ++j) {
+            sy(i, j) = sy(i + 1, j + 1);
+            yhy(i, j) = yhy(i + 1, j + 1);
+          }
+          SG[i] = SG[i + 1];
+          YHG[i] = YHG[i + 1];
+        }
+        sn_y.erase(sn_y.begin());
+        s_yn.erase(s_yn.begin());
+        yn_hy.erase(yn_hy.begin());
+        k = m - 1;
+      }
+      order.push_back(c);
+      for (int j = 0; j < k; ++j) {
+        sy(k, j) = sn_y[j];
+        sy(j, k) = s_yn[j];
+        yhy(k, j) = yhy(j, k) = yn_hy[j];
+      }
+      sy(k, k) = snyn;
+      yhy(k, k) = ynhyn;
+      SG[k] = sn_g;
+      YHG[k] = yn_hg;
+    }
+    std::swap(x, xt);
+    std::memcpy(F, Fn, sizeof(F));
+    r.history.push_back(F[0]);
+  }
+  if (it == o.max_iter && r.status == 1 && gmax < o.gtol) r.status = 0;
+  r.iterations = it;
+  r.residual = gmax;
+  std::memcpy(r.F, F, sizeof(F));
+  if (x != xi) MDFT_HIP_CHECK(hipMemcpyAsync(xi, x, n_ * sizeof(double), hipMemcpyDeviceToDevice, s));
+  MDFT_HIP_CHECK(hipStreamSynchronize(s));
+  return r;
+}
+
+}  // namespace mdft

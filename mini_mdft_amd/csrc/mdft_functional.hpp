@@ -1,0 +1,89 @@
+// The full functional F = F_id + F_ext + F_exc on the GPU, the solute's
+// external potential and an FP64 L-BFGS minimiser (see mdft_functional.hip).
+#pragma once
+
+#include <functional>
+#include <vector>
+
+#include "mdft_device.hpp"
+
+namespace mdft {
+
+// (no x 9) rotation matrix of every orientation, row major
+// (common/MDFT_Grid.hpp:295-318)
+std::vector<double> rotation_matrices(const AngularGrid& g);
+
+// V_ext (N x no, device, FP64) of the solute sites on the grid.  solute and
+// solvent are host arrays of rows (charge e, sigma A, epsilon kJ/mol, x, y, z A);
+// the solvent positions are in the molecular frame.
+void compute_vext(int nx, int ny, int nz, double lx, double ly, double lz, const AngularGrid& g,
+                  const double* solute, int nsolute, const double* solvent, int nsolvent, double vmax, double* vext,
+                  hipStream_t s);
+
+struct FunctionalParams {
+  long N;  // grid points
+  int no;  // orientations
+  double rho0, prefactor, kT, dv;
+};
+
+struct LbfgsOptions {
+  int m = 5;  // stored pairs, 1 .. kLbfgsMaxPairs
+  int max_iter = 200;
+  double gtol = 1e-5;  // converged when max |h0 g| < gtol
+  double c1 = 1e-4;    // Armijo constant
+  int max_backtrack = 30;
+};
+constexpr int kLbfgsMaxPairs = 8;
+
+struct LbfgsResult {
+  double F[4] = {0, 0, 0, 0};  // F, F_id, F_ext, F_exc at the returned point
+  int iterations = 0, evaluations = 0;
+  int status = 1;  // 0 converged, 1 max_iter reached, 2 no Armijo point found
+  double residual = 0.0;  // max |h0 g| at the returned point
+  std::vector<double> history;  // F of the start point, then after every iteration
+};
+
+class Functional {
+ public:
+  // the excess (c-proj) evaluation: fills df (device, N x no doubles) for xi
+  // and writes ff to ff_dev (device), stream ordered, without synchronising
+  using ExcessEval = std::function<void(const double* xi, double* df, double* ff_dev, hipStream_t s)>;
+
+  // w: quadrature weights (no); vext: device, N x no, borrowed (kept alive by the caller)
+  Functional(const FunctionalParams& p, const std::vector<double>& w, const double* vext, ExcessEval excess);
+  ~Functional();
+  Functional(const Functional&) = delete;
+  Functional& operator=(const Functional&) = delete;
+
+  // F[0..3] = F, F_id, F_ext, F_exc (host); g = dF/dxi (device, N x no)
+  void eval(const double* xi, double* g, double F[4], hipStream_t s);
+
+  // minimises in place from xi (device, N x no doubles)
+  LbfgsResult minimize(double* xi, const LbfgsOptions& o, hipStream_t s);
+
+  // device bytes held by the functional, and those a minimisation with m pairs adds
+  size_t bytes() const;
+  size_t minimize_bytes(int m) const;
+
+ private:
+  void reserve(int m);
+  void release();
+  const double* read_results(int count, hipStream_t s);
+
+  FunctionalParams p_;
+  ExcessEval excess_;
+  const double* vext_;       // borrowed
+  long n_ = 0;               // N * no
+  int nb_ = 0;               // blocks (= partials per sum) of the element passes
+  int step_ = 0;             // (blocks * threads) mod no
+  double* tab_ = nullptr;    // device: w (no), h0 (no)
+  double* df_ = nullptr;     // device, N x no
+  double* part_ = nullptr;   // device partials, kMaxSums x nb
+  double* res_ = nullptr;    // device results, kMaxSums + 1
+  double* host_ = nullptr;   // pinned copy of res_
+  // minimiser workspace (m + 1 slots of s and y, the trial point and g)
+  int m_alloc_ = 0;
+  double* work_ = nullptr;
+};
+
+}  // namespace mdft

@@ -88,6 +88,34 @@ def synthetic_c(nmax: int, mrso: int, nq: int, seed: int, dq: float = DQ_TIP3P,
     return LucTable(tup, c.astype(np.complex128), dq)
 
 
+def model_c_symmetric(nmax: int, mrso: int, nq: int, seed: int, dq: float = DQ_TIP3P,
+                      amplitude: float = 1.0) -> LucTable:
+    """A model c(q) for minimisations: exchange symmetric and stabilising.
+
+    Only the tuples with m = n and mu = nu are nonzero, with equal values at
+    khi and -khi; for this family the excess operator (proj2angl . conv .
+    angl2proj) is self-adjoint in the w-weighted orientation inner product, so
+    F_exc is a quadratic form whose gradient is 2 prefactor w vexc d(Delta rho)
+    (tests/test_functional.py checks <a, K b>_w = <b, K a>_w).  Each value is
+    -a exp(-(q r0)^2 / 4) with a = amplitude U[0.2, 1] / (1 + m) and r0 ~
+    U[1.5, 3] A (the stream of (seed, 2 tuple + {0, 1})): negative like the
+    core of a physical c, which keeps F bounded below.  The physical direct
+    correlation function has the same exchange symmetry; the independent
+    draws of synthetic_c do not."""
+    tup = luc_tuples(nmax, mrso)
+    u = uniform(seed, 2 * tup.shape[0]).reshape(-1, 2)
+    q = np.arange(nq, dtype=np.float64) * dq
+    c = np.zeros((tup.shape[0], nq), dtype=np.complex128)
+    index = {tuple(int(v) for v in t): i for i, t in enumerate(tup)}
+    for i, (m, n, mu, nu, khi) in enumerate(tup):
+        if m == n and mu == nu and khi >= 0:
+            a = amplitude * (0.2 + 0.8 * u[i, 0]) / (1.0 + m)
+            r0 = 1.5 + 1.5 * u[i, 1]
+            c[i] = -a * np.exp(-(q * r0) ** 2 / 4.0)
+            c[index[(int(m), int(n), int(mu), int(nu), -int(khi))]] = c[i]
+    return LucTable(tup, c, dq)
+
+
 def write_luc(path, table: LucTable, header: str = "synthetic c(q)") -> None:
     """Write ``table`` in the Luc text format described in the module docstring."""
     tup = table.tuples

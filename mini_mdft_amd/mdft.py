@@ -352,3 +352,80 @@ def fill_uniform(t, seed: int, lo: float, hi: float, offset: int = 0, stream=Non
     """t[i] = lo + (hi - lo) u(seed, offset + i), the stream of mini_mdft_amd.rng.uniform."""
     check(lib().mdft_fill_uniform(ptr(t), t.numel(), seed & 0xFFFFFFFFFFFFFFFF, offset, ctypes.c_double(lo),
                                   ctypes.c_double(hi), _dtype_code(t), stream_handle(stream)))
+
+
+def rotation_matrices(angular_grid: AngularGrid) -> np.ndarray:
+    """AngularGrid's rotation tables m_rotxx .. m_rotzz (common/MDFT_Grid.hpp:295-318)
+    as an (no, 3, 3) array."""
+    out = np.zeros((angular_grid.m_no, 3, 3))
+    check(lib().mdft_rotation_matrices(angular_grid.m_mmax, angular_grid.m_molrotsymorder,
+                                       out.ctypes.data_as(_lib._pd)))
+    return out
+
+
+def vext(spatial_grid: SpatialGrid, angular_grid: AngularGrid, solute, solvent, vmax: float, out=None, stream=None):
+    """External potential (kJ/mol) of the solute sites on the grid, an (N, no)
+    float64 CUDA tensor: Lennard-Jones (Lorentz-Berthelot) plus qfact q q / d
+    over solvent x solute site pairs at the minimum-image distance, capped at
+    vmax.  solute, solvent: (n, 6) rows (charge, sigma, epsilon, x, y, z), the
+    solvent's in the molecular frame (mini_mdft_amd.solute)."""
+    torch = _torch()
+    g = spatial_grid
+    solute = np.ascontiguousarray(solute, dtype=np.float64).reshape(-1, 6)
+    solvent = np.ascontiguousarray(solvent, dtype=np.float64).reshape(-1, 6)
+    if out is None:
+        out = torch.empty((g.N, angular_grid.m_no), dtype=torch.float64, device="cuda")
+    _require(out, torch.float64, g.N * angular_grid.m_no, "vext")
+    check(lib().mdft_vext(g.m_nx, g.m_ny, g.m_nz, g.m_lx, g.m_ly, g.m_lz, angular_grid.m_mmax,
+                          angular_grid.m_molrotsymorder, solute.shape[0], solute.ctypes.data_as(_lib._pd),
+                          solvent.shape[0], solvent.ctypes.data_as(_lib._pd), ctypes.c_double(vmax), ptr(out),
+                          stream_handle(stream)))
+    return out
+
+
+class Functional:
+    """F = F_id + F_ext + F_exc and its gradient on the GPU, with an FP64
+    L-BFGS minimiser (the reference's Solver makes one excess evaluation,
+    src/mini-mdft.hpp:129-133; its iteration settings are unused,
+    common/MDFT_System.hpp:188-189).  The excess term is the solver's c-proj
+    evaluation in its precision form (MDFT_FP64 or a mixed form)."""
+
+    STATUS = {0: "converged", 1: "max_iter", 2: "line search"}
+
+    def __init__(self, solver: Solver, vext_tensor, kT_: float):
+        torch = _torch()
+        _require(vext_tensor, torch.float64, solver.conv.sg.N * solver.conv.ag.m_no, "vext")
+        h = ctypes.c_void_p()
+        check(lib().mdft_functional_create(solver._h, ptr(vext_tensor), ctypes.c_double(kT_), ctypes.byref(h)))
+        self._h, self._destroy = h, lib().mdft_functional_destroy
+        self.solver, self.vext, self.kT = solver, vext_tensor, kT_
+        self.n = solver.conv.sg.N * solver.conv.ag.m_no
+
+    def __del__(self):
+        h, destroy = getattr(self, "_h", None), getattr(self, "_destroy", None)
+        if h and destroy is not None:
+            destroy(h)
+
+    def eval(self, xi, g, stream=None):
+        """(F, F_id, F_ext, F_exc) in kJ/mol; g = dF/dxi written in place."""
+        torch = _torch()
+        _require(xi, torch.float64, self.n, "xi")
+        _require(g, torch.float64, self.n, "g")
+        F = (ctypes.c_double * 4)()
+        check(lib().mdft_functional_eval(self._h, ptr(xi), ptr(g), F, stream_handle(stream)))
+        return tuple(F)
+
+    def minimize(self, xi, m: int = 5, max_iter: int = 200, gtol: float = 1e-5, stream=None) -> dict:
+        """L-BFGS from xi, in place.  Returns F (4 terms), iterations,
+        evaluations, status, residual (max |h0 g|) and the F history."""
+        torch = _torch()
+        _require(xi, torch.float64, self.n, "xi")
+        F = (ctypes.c_double * 4)()
+        it, nev, st = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        res = ctypes.c_double()
+        hist = np.zeros(max_iter + 1)
+        check(lib().mdft_lbfgs_minimize(self._h, ptr(xi), m, max_iter, ctypes.c_double(gtol), F, ctypes.byref(it),
+                                        ctypes.byref(nev), ctypes.byref(st), ctypes.byref(res),
+                                        hist.ctypes.data_as(_lib._pd), stream_handle(stream)))
+        return dict(F=tuple(F), iterations=it.value, evaluations=nev.value, status=st.value,
+                    residual=res.value, history=hist[:it.value + 1].copy())

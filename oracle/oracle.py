@@ -300,3 +300,138 @@ class Conv:
         _check(lib().orc_energy_cproj(self._h, _d(xi), ctypes.c_double(rho0), ctypes.c_double(prefactor),
                                       _d(df), ctypes.byref(ff), _d(st)))
         return (ff.value, df, st) if stages else (ff.value, df)
+
+
+# ===========================================================================
+# Full functional, external potential and L-BFGS (the "next" rows N1 / N4).
+# The reference has no such code (src/mini-mdft.hpp:129-133 makes one
+# evaluation; Solvent::m_vext is unused, common/MDFT_Solvent.hpp:160), so these
+# restate the definitions of this build; their pins are a finite-difference
+# gradient, the c = 0 limit xi^2 = exp(-V / kT), self-adjointness of the
+# excess operator for the symmetric model c(q), monotone decrease and an
+# analytic single-site potential (tests/test_functional.py).
+
+#: Constants::qfact (common/MDFT_Constants.hpp:36-47): qfact q q / r in kJ/mol
+QFACT = 1.602176565e-19 ** 2 * 1.0e-3 * 6.02214129e23 / (4.0 * np.pi * 8.854187817e-12 * 1.0e-10)
+
+
+def rotation_matrices(mmax: int, mrso: int) -> np.ndarray:
+    """(no, 3, 3) rotation of every orientation (common/MDFT_Grid.hpp:295-318):
+    theta from the GL nodes, phi_j = j dphi and psi_k = k dpsi with dphi = 2 pi
+    / nphi and dpsi = 2 pi / mrso / npsi (:141-142, uniform_mesh of
+    common/MDFT_Math_Utils.hpp:76-83)."""
+    s = angular_sizes(mmax, mrso)
+    t = angular_tables(mmax, mrso)
+    nt, nph, nps = s["ntheta"], s["nphi"], s["npsi"]
+    dphi, dpsi = 2.0 * np.pi / nph, 2.0 * np.pi / mrso / nps
+    th = np.repeat(t["theta"], nph * nps)
+    ph = np.tile(np.repeat(np.arange(nph) * dphi, nps), nt)
+    ps = np.tile(np.arange(nps) * dpsi, nt * nph)
+    ct, st, cp, sp, cs, ss = np.cos(th), np.sin(th), np.cos(ph), np.sin(ph), np.cos(ps), np.sin(ps)
+    return np.stack([np.stack([ct * cp * cs - sp * ss, -ct * cp * ss - sp * cs, st * cp], -1),
+                     np.stack([ct * sp * cs + cp * ss, -ct * sp * ss + cp * cs, st * sp], -1),
+                     np.stack([-st * cs, st * ss, ct], -1)], -2)
+
+
+def vext(n, length, mmax: int, mrso: int, solute: np.ndarray, solvent: np.ndarray, vmax: float) -> np.ndarray:
+    """(N, no) external potential in kJ/mol: for every grid node x_r = (i dx, j dy,
+    k dz) and orientation, the sum over solvent sites (position x_r + R s) and
+    solute sites of Lennard-Jones (Lorentz-Berthelot mixing, only between sites
+    with sigma > 0) and qfact q q / d, with d the minimum-image distance, capped
+    at vmax (and d floored at 1e-6 A).  solute, solvent: (n, 6) rows of
+    (charge, sigma, epsilon, x, y, z)."""
+    nx, ny, nz = n
+    L = np.asarray(length, dtype=np.float64)
+    R = rotation_matrices(mmax, mrso)
+    grid = np.indices(n).reshape(3, -1).T * (L / np.asarray(n))
+    V = np.zeros((nx * ny * nz, R.shape[0]))
+    for qk, sk, ek, *sxyz in solvent:
+        pos = grid[:, None, :] + np.einsum("oij,j->oi", R, np.asarray(sxyz))[None]
+        for qj, sj, ej, *xj in solute:
+            d = pos - np.asarray(xj)
+            d -= L * np.round(d / L)
+            r = np.maximum(np.sqrt((d * d).sum(-1)), 1e-6)
+            if sk > 0 and sj > 0:
+                sig, eps = 0.5 * (sk + sj), np.sqrt(ek * ej)
+                V += 4.0 * eps * ((sig / r) ** 12 - (sig / r) ** 6)
+            V += QFACT * qk * qj / r
+    return np.minimum(V, vmax)
+
+
+def functional(conv: "Conv", xi: np.ndarray, V: np.ndarray, w: np.ndarray, rho0: float, prefactor: float,
+               kT: float, dv: float):
+    """F = F_id + F_ext + F_exc and its gradient with respect to xi (FP64):
+      F_id  = kT rho0 dv sum w (xi^2 ln xi^2 - xi^2 + 1)
+      F_ext = rho0 dv sum w xi^2 V
+      F_exc = (dv / N) ff, ff the c-proj value of energy_cproj (the unnormalised
+              FFTs make the reference's ff N times the discrete integral)
+      g     = dv w (2 rho0 xi (kT ln xi^2 + V) + (2 prefactor / N) df)
+    with df = 2 rho0 xi vexc the reference's gradient array (exact when the
+    excess operator is self-adjoint).  Returns ((F, F_id, F_ext, F_exc), g)."""
+    N = xi.shape[0]
+    ff, df = conv.energy_cproj(xi, rho0, prefactor)
+    x2 = np.maximum(xi * xi, 1e-300)
+    fid = kT * rho0 * dv * np.sum(w * (x2 * np.log(x2) - x2 + 1.0))
+    fext = rho0 * dv * np.sum(w * x2 * V)
+    fexc = dv / N * ff
+    g = dv * w * (2.0 * rho0 * xi * (kT * np.log(x2) + V) + (2.0 * prefactor / N) * df)
+    return (fid + fext + fexc, fid, fext, fexc), g
+
+
+def lbfgs(fg, x0: np.ndarray, h0: np.ndarray, m: int = 5, max_iter: int = 200, gtol: float = 1e-5,
+          c1: float = 1e-4, max_backtrack: int = 30):
+    """The L-BFGS of this build (FP64): two-loop recursion with the diagonal
+    initial inverse Hessian gamma h0, gamma = s'y / y'(h0 y) of the newest
+    pair; Armijo backtracking from step 1 (halving, at most max_backtrack
+    trials); a pair is kept when s'y > 1e-12 |s| |y|; a non-descent direction
+    restarts from -h0 g.  fg(x) -> (F, g).  Returns (x, F, g, iterations,
+    evaluations, F history, status) with status 0 = converged (max |h0 g| <
+    gtol), 1 = max_iter iterations made, 2 = no Armijo point found (x is the
+    last accepted point: the decrease is below what F resolves)."""
+    x = x0.copy()
+    F, g = fg(x)
+    S, Y = [], []
+    hist, nev = [F], 1
+    for it in range(max_iter):
+        if np.abs(h0 * g).max() < gtol:
+            return x, F, g, it, nev, hist, 0
+        q = g.copy()
+        alpha = []
+        for s, y in reversed(list(zip(S, Y))):
+            a = (s @ q) / (y @ s)
+            alpha.append(a)
+            q -= a * y
+        gamma = (S[-1] @ Y[-1]) / (Y[-1] @ (h0 * Y[-1])) if S else 1.0
+        r = gamma * h0 * q
+        for (s, y), a in zip(zip(S, Y), reversed(alpha)):
+            b = (y @ r) / (y @ s)
+            r += (a - b) * s
+        d = -r
+        gd = g @ d
+        if not gd < 0:
+            S.clear()
+            Y.clear()
+            d = -h0 * g
+            gd = g @ d
+        t = 1.0
+        for _ in range(max_backtrack):
+            xn = x + t * d
+            Fn, gn = fg(xn)
+            nev += 1
+            if np.isfinite(Fn) and Fn <= F + c1 * t * gd:
+                break
+            t *= 0.5
+        else:
+            return x, F, g, it, nev, hist, 2
+        s, y = xn - x, gn - g
+        if s @ y > 1e-12 * np.sqrt((s @ s) * (y @ y)):
+            S.append(s)
+            Y.append(y)
+            if len(S) > m:
+                S.pop(0)
+                Y.pop(0)
+        x, F, g = xn, Fn, gn
+        hist.append(F)
+    if np.abs(h0 * g).max() < gtol:
+        return x, F, g, max_iter, nev, hist, 0
+    return x, F, g, max_iter, nev, hist, 1
