@@ -208,17 +208,21 @@ int mdft_functional_create(mdft_solver_t solver, const double* vext, double kT, 
 int mdft_functional_destroy(mdft_functional_t h);
 int mdft_functional_eval(mdft_functional_t h, const double* xi, double* g, double* F, void* stream);
 
-/* L-BFGS (FP64 vectors, dot products and line search) from xi, in place:
- * m stored pairs (1..8), diagonal initial inverse Hessian gamma h0 with
- * h0 = 1 / (2 rho0 dv w kT) and gamma = s'y / y'(h0 y) of the newest pair,
- * Armijo backtracking (c1 = 1e-4, halving from t = 1, at most 30 trials), a
- * pair kept when s'y > 1e-12 |s| |y|, a restart from -h0 g on a non-descent
- * direction; converged when max |h0 g| < gtol.  Outputs (host; history may be
+/* L-BFGS (FP64 iterate, gradient, dot products and line search) from xi, in
+ * place: m stored pairs (1..8; FP32 storage for the mixed forms), diagonal
+ * initial inverse Hessian gamma h0 with h0 = 1 / (2 rho0 dv w kT) and
+ * gamma = s'y / y'(h0 y) of the newest pair, backtracking by halving from
+ * t = 1 (at most 30 trials) until Armijo (c1 = 1e-4) or the approximate Wolfe
+ * test of Hager and Zhang holds (g(t).d <= (2 c1 - 1) g.d with F(t) <= F +
+ * 1e-6 (|F_id| + |F_ext| + |F_exc|): Armijo read off the slopes, which stays
+ * decisive where F's rounding hides the decrease), a pair kept when
+ * s'y > 1e-12 |s| |y|, a restart from -h0 g on a non-descent direction;
+ * converged when max |h0 g| < gtol.  Outputs (host; history may be
  * NULL, else max_iter + 1 doubles receiving F at the start and after every
  * iteration): F[4] at the returned point, the iterations made, the
  * functional evaluations, status (0 converged, 1 max_iter reached, 2 no
- * Armijo point: the decrease is below what F resolves) and max |h0 g|.  The
- * handle keeps its (2 m + 4) N no doubles of work space between calls. */
+ * acceptable point within 30 trials) and max |h0 g|.  The handle keeps its
+ * work space (3 N no doubles and 2 (m + 1) N no pair values) between calls. */
 int mdft_lbfgs_minimize(mdft_functional_t h, double* xi, int m, int max_iter, double gtol, double* F,
                         int* iterations, int* evaluations, int* status, double* residual, double* history,
                         void* stream);

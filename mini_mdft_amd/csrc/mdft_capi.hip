@@ -534,8 +534,11 @@ int mdft_functional_create(mdft_solver_t solver, const double* vext, double kT, 
                        t.lx * t.ly * t.lz / static_cast<double>(solver->N)};
     auto h = std::make_unique<mdft_functional_s>();
     h->solver = solver;
-    h->f = std::make_unique<Functional>(p, t.ang.w, vext, [solver](const double* xi, double* df, double* ffd,
-                                                                   hipStream_t s) { energy_impl(solver, xi, df, ffd, s); });
+    // the mixed forms keep their L-BFGS pairs in FP32
+    h->f = std::make_unique<Functional>(
+        p, t.ang.w, vext,
+        [solver](const double* xi, double* df, double* ffd, hipStream_t s) { energy_impl(solver, xi, df, ffd, s); },
+        solver->precision != MDFT_FP64);
     *out = h.release();
   });
 }

@@ -14,15 +14,17 @@
 // (common/MDFT_Solvent.hpp:643-657).  xi^2 is floored at 1e-300 inside ln.
 //
 // The minimiser is L-BFGS with the diagonal initial inverse Hessian gamma h0,
-// h0 = 1 / (2 rho0 dv w kT), Armijo backtracking and restarts on a
-// non-descent direction.  It runs in the vector-free form of Chen, Wang, Zhou
+// h0 = 1 / (2 rho0 dv w kT), a backtracking line search that accepts Armijo
+// or the approximate Wolfe test of Hager and Zhang (SIAM J. Optim. 16 (2005)
+// 170-192), and restarts on a non-descent direction.  It runs in the vector-free form of Chen, Wang, Zhou
 // and Gao ("Large-scale L-BFGS using MapReduce", NIPS 2014): the two-loop
 // recursion is carried out on the host in the coefficients of the direction
 // over the basis {s_i, y_i, g}, from the Gram entries s_i.y_j, y_i'H y_j,
 // s_i.g, y_i'H g, g'H g (H = diag(h0)).  Every iteration then makes two
-// fused passes over HBM: one that forms the trial point x + t d (and its F_id,
-// F_ext partial sums), one that forms g, y = g_new - g and all new Gram
-// entries at once.  All sums are FP64 with fixed-order reductions.
+// fused passes over HBM: one that forms the trial point x + t d, one (after the
+// excess evaluation there) that forms g, y = g_new - g, the F_id and F_ext sums
+// and all new Gram entries at once.  All sums are FP64 with fixed-order
+// reductions.
 #include "mdft_functional.hpp"
 
 #include <algorithm>
@@ -36,8 +38,6 @@ namespace {
 constexpr int kFThreads = 256;
 constexpr int kFWaves = kFThreads / 64;
 constexpr int kMaxBlocks = 2048;
-constexpr int kMaxSums = 5 * kLbfgsMaxPairs + 8;  // update pass: 5K + 7 sums and one max
-constexpr int kFfSlot = kMaxSums;                 // ff of the excess evaluation
 
 // Constants the element passes share.
 struct Elem {
@@ -114,73 +114,81 @@ __global__ __launch_bounds__(kFThreads) void local_kernel(const double* __restri
   store_partials<4, true>(acc, part, nb);
 }
 
+// The stored pairs s_j, y_j and the candidate pair are TS (double, or float
+// for the mixed forms, whose pairs then take half the traffic); every dot
+// product and the direction are formed in FP64 from the stored values.
+template <typename TS>
 struct DirArgs {
   const double* x;
   const double* g;
-  const double* V;
-  const double* s[kLbfgsMaxPairs];
-  const double* y[kLbfgsMaxPairs];
+  const TS* s[kLbfgsMaxPairs];
+  const TS* y[kLbfgsMaxPairs];
   double cs[kLbfgsMaxPairs], cq[kLbfgsMaxPairs];
   double cg, gamma, t;
   double* xt;  // x + t d
-  double* sc;  // xt - x (the candidate s)
+  TS* sc;      // xt - x (the candidate s)
 };
 
-// d = -(gamma h0 (cg g + sum cq_j y_j) + sum cs_j s_j); xt = x + t d; sc = xt - x;
-// F_id and F_ext sums at xt
-template <int K>
-__global__ __launch_bounds__(kFThreads) void direction_kernel(DirArgs a, Elem e, double* __restrict__ part, int nb) {
-  double acc[2] = {0.0, 0.0};
+// d = -(gamma h0 (cg g + sum cq_j y_j) + sum cs_j s_j); xt = x + t d; sc = xt - x
+template <typename TS, int K>
+__global__ __launch_bounds__(kFThreads) void direction_kernel(DirArgs<TS> a, Elem e) {
   MDFT_ELEMENT_LOOP(e) {
     double q = a.cg * a.g[i];
 #pragma unroll
-    for (int j = 0; j < K; ++j) q = fma(a.cq[j], a.y[j][i], q);
+    for (int j = 0; j < K; ++j) q = fma(a.cq[j], static_cast<double>(a.y[j][i]), q);
     double r = a.gamma * e.h0[io] * q;
 #pragma unroll
-    for (int j = 0; j < K; ++j) r = fma(a.cs[j], a.s[j][i], r);
+    for (int j = 0; j < K; ++j) r = fma(a.cs[j], static_cast<double>(a.s[j][i]), r);
     const double xv = a.x[i];
     const double xn = xv + a.t * (-r);
     a.xt[i] = xn;
-    a.sc[i] = xn - xv;
-    local_f(xn, a.V[i], e.w[io], acc[0], acc[1]);
+    a.sc[i] = static_cast<TS>(xn - xv);
   }
-  store_partials<2, false>(acc, part, nb);
 }
 
+template <typename TS>
 struct UpdArgs {
   const double* xt;
   const double* V;
   const double* df;
-  double* g;  // in: g at x, out: g at xt
-  const double* sc;
-  double* yn;  // out: g_new - g_old
-  const double* s[kLbfgsMaxPairs];
-  const double* y[kLbfgsMaxPairs];
+  const double* g;  // g at x
+  double* gt;       // out: g at xt
+  const TS* sc;
+  TS* yn;  // out: g at xt - g at x
+  const TS* s[kLbfgsMaxPairs];
+  const TS* y[kLbfgsMaxPairs];
 };
 
-// g at the accepted point, y = g_new - g_old and the new Gram entries:
+// At the trial point: g, y = g_t - g, the F_id and F_ext sums and the Gram
+// entries of the candidate pair:
 //   [0, K) sc.y_j | K sc.yn | [K+1, 2K+1) s_j.yn | 2K+1 sc.sc | 2K+2 yn.yn |
-//   [2K+3, 3K+3) s_j.g | 3K+3 sc.g | [3K+4, 4K+4) yn'H y_j | 4K+4 yn'H yn |
-//   [4K+5, 5K+5) y_j'H g | 5K+5 yn'H g | 5K+6 g'H g | 5K+7 max |h0 g|
-template <int K>
-__global__ __launch_bounds__(kFThreads) void update_kernel(UpdArgs a, Elem e, double* __restrict__ part, int nb) {
-  constexpr int NA = 5 * K + 8;
+//   [2K+3, 3K+3) s_j.g_t | 3K+3 sc.g_t | [3K+4, 4K+4) yn'H y_j | 4K+4 yn'H yn |
+//   [4K+5, 5K+5) y_j'H g_t | 5K+5 yn'H g_t | 5K+6 g_t'H g_t | 5K+7 F_id sum |
+//   5K+8 F_ext sum | 5K+9 max |h0 g_t|
+// yn enters the sums as stored (rounded to TS).
+template <typename TS, int K>
+__global__ __launch_bounds__(kFThreads) void update_kernel(UpdArgs<TS> a, Elem e, double* __restrict__ part,
+                                                           int nb) {
+  constexpr int NA = 5 * K + 10;
   constexpr int KA = K > 0 ? K : 1;
   double acc[NA];
 #pragma unroll
   for (int k = 0; k < NA; ++k) acc[k] = 0.0;
   MDFT_ELEMENT_LOOP(e) {
     const double wio = e.w[io], h = e.h0[io];
-    const double gn = local_grad(a.xt[i], a.V[i], a.df[i], wio, e);
-    const double yv = gn - a.g[i];
-    a.g[i] = gn;
-    a.yn[i] = yv;
-    const double scv = a.sc[i];
+    const double xv = a.xt[i], v = a.V[i];
+    local_f(xv, v, wio, acc[5 * K + 7], acc[5 * K + 8]);
+    const double gn = local_grad(xv, v, a.df[i], wio, e);
+    const TS ys = static_cast<TS>(gn - a.g[i]);
+    a.gt[i] = gn;
+    a.yn[i] = ys;
+    const double yv = static_cast<double>(ys);
+    const double scv = static_cast<double>(a.sc[i]);
     double sj[KA], yj[KA];
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-      sj[j] = a.s[j][i];
-      yj[j] = a.y[j][i];
+      sj[j] = static_cast<double>(a.s[j][i]);
+      yj[j] = static_cast<double>(a.y[j][i]);
     }
     const double hy = h * yv, hg = h * gn;
 #pragma unroll
@@ -198,16 +206,17 @@ __global__ __launch_bounds__(kFThreads) void update_kernel(UpdArgs a, Elem e, do
     acc[4 * K + 4] += hy * yv;
     acc[5 * K + 5] += yv * hg;
     acc[5 * K + 6] += gn * hg;
-    acc[5 * K + 7] = fmax(acc[5 * K + 7], fabs(hg));
+    acc[5 * K + 9] = fmax(acc[5 * K + 9], fabs(hg));
   }
   store_partials<NA, true>(acc, part, nb);
 }
 #undef MDFT_ELEMENT_LOOP
 
-// out[row] = sum (or, for row == max_row, max) of part[row * nb .. + nb), in a fixed order
-__global__ __launch_bounds__(kFThreads) void reduce_rows_kernel(const double* __restrict__ part, int nb, int max_row,
-                                                                double* __restrict__ out) {
-  const int row = blockIdx.x;
+// out[row] = sum (or, for row == max_row, max) of part[row * nb .. + nb), in a
+// fixed order, for rows first .. first + gridDim.x
+__global__ __launch_bounds__(kFThreads) void reduce_rows_kernel(const double* __restrict__ part, int nb, int first,
+                                                                int max_row, double* __restrict__ out) {
+  const int row = first + blockIdx.x;
   const bool is_max = row == max_row;
   double v = 0.0;
   for (int i = threadIdx.x; i < nb; i += kFThreads) {
@@ -374,8 +383,22 @@ void compute_vext(int nx, int ny, int nz, double lx, double ly, double lz, const
 }
 
 // ---------------------------------------------------------------------------
-Functional::Functional(const FunctionalParams& p, const std::vector<double>& w, const double* vext, ExcessEval excess)
-    : p_(p), excess_(std::move(excess)), vext_(vext) {
+namespace {
+// rows of the partial sums and slots of the results: [0, kRows) the update
+// pass (and the local pass), kFfSlot the excess evaluation's ff
+constexpr int kRows = 5 * kLbfgsMaxPairs + 10;
+constexpr int kFfSlot = kRows;
+constexpr int kResults = kRows + 1;
+
+void reduce_rows(const double* part, int nb, int first, int rows, int max_row, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(reduce_rows_kernel, dim3(rows), dim3(kFThreads), 0, s, part, nb, first, max_row, out);
+  MDFT_HIP_CHECK(hipGetLastError());
+}
+}  // namespace
+
+Functional::Functional(const FunctionalParams& p, const std::vector<double>& w, const double* vext, ExcessEval excess,
+                       bool fp32_pairs)
+    : p_(p), excess_(std::move(excess)), vext_(vext), fp32_pairs_(fp32_pairs) {
   throw_if(p.N <= 0 || p.no <= 0 || static_cast<int>(w.size()) != p.no, "functional: bad sizes");
   throw_if(!vext, "functional: null vext");
   throw_if(!(p.kT > 0) || !(p.dv > 0) || !(p.rho0 > 0), "functional: kT, dv and rho0 must be positive");
@@ -392,9 +415,9 @@ Functional::Functional(const FunctionalParams& p, const std::vector<double>& w, 
     MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&tab_), tab.size() * sizeof(double)));
     MDFT_HIP_CHECK(hipMemcpy(tab_, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
     MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&df_), n_ * sizeof(double)));
-    MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&part_), static_cast<size_t>(kMaxSums) * nb_ * sizeof(double)));
-    MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&res_), (kMaxSums + 1) * sizeof(double)));
-    MDFT_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_), (kMaxSums + 1) * sizeof(double)));
+    MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&part_), static_cast<size_t>(kRows) * nb_ * sizeof(double)));
+    MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&res_), kResults * sizeof(double)));
+    MDFT_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_), kResults * sizeof(double)));
   } catch (...) {
     release();
     throw;
@@ -410,38 +433,43 @@ void Functional::release() {
   if (res_) (void)hipFree(res_);
   if (host_) (void)hipHostFree(host_);
   if (work_) (void)hipFree(work_);
-  tab_ = df_ = part_ = res_ = host_ = work_ = nullptr;
+  tab_ = df_ = part_ = res_ = host_ = nullptr;
+  work_ = nullptr;
+  m_alloc_ = 0;
 }
 
 size_t Functional::bytes() const {
-  return (2 * static_cast<size_t>(p_.no) + n_ + static_cast<size_t>(kMaxSums) * nb_ + kMaxSums + 1) * sizeof(double) +
-         static_cast<size_t>(2 * m_alloc_ + 4) * (m_alloc_ ? n_ : 0) * sizeof(double);
+  return (2 * static_cast<size_t>(p_.no) + n_ + static_cast<size_t>(kRows) * nb_ + kResults) * sizeof(double) +
+         (m_alloc_ ? minimize_bytes(m_alloc_) : 0);
 }
 
-size_t Functional::minimize_bytes(int m) const { return static_cast<size_t>(2 * m + 4) * n_ * sizeof(double); }
+size_t Functional::minimize_bytes(int m) const {
+  // the trial point and two gradients in FP64, m + 1 slots of s and y
+  const size_t ts = fp32_pairs_ ? sizeof(float) : sizeof(double);
+  return 3 * static_cast<size_t>(n_) * sizeof(double) + 2 * static_cast<size_t>(m + 1) * n_ * ts;
+}
 
 void Functional::reserve(int m) {
   if (m <= m_alloc_ && work_) return;
   if (work_) (void)hipFree(work_);
   work_ = nullptr;
   m_alloc_ = 0;
-  MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&work_), minimize_bytes(m)));
+  MDFT_HIP_CHECK(hipMalloc(&work_, minimize_bytes(m)));
   m_alloc_ = m;
 }
 
-const double* Functional::read_results(int count, hipStream_t s) {
-  (void)count;
-  MDFT_HIP_CHECK(hipMemcpyAsync(host_, res_, (kMaxSums + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
+const double* Functional::read_results(hipStream_t s) {
+  MDFT_HIP_CHECK(hipMemcpyAsync(host_, res_, kResults * sizeof(double), hipMemcpyDeviceToHost, s));
   MDFT_HIP_CHECK(hipStreamSynchronize(s));
   return host_;
 }
 
-namespace {
-void reduce_rows(const double* part, int nb, int rows, int max_row, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(reduce_rows_kernel, dim3(rows), dim3(kFThreads), 0, s, part, nb, max_row, out);
-  MDFT_HIP_CHECK(hipGetLastError());
+void Functional::fparts(double sid, double sext, double ff, double F[4]) const {
+  F[1] = p_.kT * p_.rho0 * p_.dv * sid;
+  F[2] = p_.rho0 * p_.dv * sext;
+  F[3] = p_.dv / p_.N * ff;
+  F[0] = F[1] + F[2] + F[3];
 }
-}  // namespace
 
 void Functional::eval(const double* xi, double* g, double F[4], hipStream_t s) {
   throw_if(!xi || !g || !F, "functional eval: null argument");
@@ -449,44 +477,43 @@ void Functional::eval(const double* xi, double* g, double F[4], hipStream_t s) {
   const Elem e{tab_, tab_ + p_.no, p_.rho0, p_.kT, p_.dv, 2.0 * p_.prefactor / p_.N, n_, p_.no, step_};
   hipLaunchKernelGGL(local_kernel, dim3(nb_), dim3(kFThreads), 0, s, xi, vext_, df_, g, e, part_, nb_);
   MDFT_HIP_CHECK(hipGetLastError());
-  reduce_rows(part_, nb_, 4, 3, res_, s);
-  const double* h = read_results(4, s);
-  F[1] = p_.kT * p_.rho0 * p_.dv * h[0];
-  F[2] = p_.rho0 * p_.dv * h[1];
-  F[3] = p_.dv / p_.N * h[kFfSlot];
-  F[0] = F[1] + F[2] + F[3];
+  reduce_rows(part_, nb_, 0, 4, 3, res_, s);
+  const double* h = read_results(s);
+  fparts(h[0], h[1], h[kFfSlot], F);
 }
 
 LbfgsResult Functional::minimize(double* xi, const LbfgsOptions& o, hipStream_t s) {
   throw_if(!xi, "lbfgs: null xi");
   throw_if(o.m < 1 || o.m > kLbfgsMaxPairs, "lbfgs: m must be in 1 .. 8");
-  throw_if(o.max_iter < 0 || o.max_backtrack < 1 || !(o.gtol >= 0) || !(o.c1 > 0 && o.c1 < 1),
+  throw_if(o.max_iter < 0 || o.max_backtrack < 1 || !(o.gtol >= 0) || !(o.c1 > 0 && o.c1 < 0.5) ||
+               !(o.eps_f >= 0),
            "lbfgs: bad options");
   reserve(o.m);
+  return fp32_pairs_ ? minimize_t<float>(xi, o, s) : minimize_t<double>(xi, o, s);
+}
+
+template <typename TS>
+LbfgsResult Functional::minimize_t(double* xi, const LbfgsOptions& o, hipStream_t s) {
   const int m = o.m, nslot = m + 1;
   const Elem e{tab_, tab_ + p_.no, p_.rho0, p_.kT, p_.dv, 2.0 * p_.prefactor / p_.N, n_, p_.no, step_};
   double* x = xi;
-  double* xt = work_;
-  double* g = work_ + n_;
-  std::vector<double*> S(nslot), Y(nslot);
+  double* xt = static_cast<double*>(work_);
+  double* g = xt + n_;
+  double* gt = g + n_;
+  TS* pairs = reinterpret_cast<TS*>(gt + n_);
+  std::vector<TS*> S(nslot), Y(nslot);
   for (int j = 0; j < nslot; ++j) {
-    S[j] = work_ + (2 + 2 * static_cast<long>(j)) * n_;
-    Y[j] = work_ + (3 + 2 * static_cast<long>(j)) * n_;
+    S[j] = pairs + 2 * static_cast<long>(j) * n_;
+    Y[j] = pairs + (2 * static_cast<long>(j) + 1) * n_;
   }
-  auto fparts = [&](double sid, double sext, double ff, double* F) {
-    F[1] = p_.kT * p_.rho0 * p_.dv * sid;
-    F[2] = p_.rho0 * p_.dv * sext;
-    F[3] = p_.dv / p_.N * ff;
-    F[0] = F[1] + F[2] + F[3];
-  };
 
   LbfgsResult r;
   // the start point
   excess_(x, df_, res_ + kFfSlot, s);
   hipLaunchKernelGGL(local_kernel, dim3(nb_), dim3(kFThreads), 0, s, x, vext_, df_, g, e, part_, nb_);
   MDFT_HIP_CHECK(hipGetLastError());
-  reduce_rows(part_, nb_, 4, 3, res_, s);
-  const double* h = read_results(4, s);
+  reduce_rows(part_, nb_, 0, 4, 3, res_, s);
+  const double* h = read_results(s);
   double F[4];
   fparts(h[0], h[1], h[kFfSlot], F);
   double GHG = h[2], gmax = h[3];
@@ -539,13 +566,20 @@ LbfgsResult Functional::minimize(double* xi, const LbfgsOptions& o, hipStream_t 
     int c = 0;
     while (std::find(order.begin(), order.end(), c) != order.end()) ++c;
 
-    DirArgs da{};
+    DirArgs<TS> da{};
     da.x = x;
     da.g = g;
-    da.V = vext_;
+    UpdArgs<TS> ua{};
+    ua.xt = xt;
+    ua.V = vext_;
+    ua.df = df_;
+    ua.g = g;
+    ua.gt = gt;
+    ua.sc = S[c];
+    ua.yn = Y[c];
     for (int j = 0; j < k; ++j) {
-      da.s[j] = S[order[j]];
-      da.y[j] = Y[order[j]];
+      da.s[j] = ua.s[j] = S[order[j]];
+      da.y[j] = ua.y[j] = Y[order[j]];
       da.cs[j] = cs[j];
       da.cq[j] = cq[j];
     }
@@ -553,21 +587,32 @@ LbfgsResult Functional::minimize(double* xi, const LbfgsOptions& o, hipStream_t 
     da.gamma = gamma;
     da.xt = xt;
     da.sc = S[c];
+    const int K = k;
+    // each trial: x + t d, the excess evaluation there, then g, the F_id and
+    // F_ext sums and the candidate pair's Gram entries (kept if accepted)
+    const double fscale = std::fabs(F[1]) + std::fabs(F[2]) + std::fabs(F[3]);
     double t = 1.0, Fn[4] = {0, 0, 0, 0};
     bool accepted = false;
     for (int b = 0; b < o.max_backtrack; ++b) {
       da.t = t;
-      dispatch_k(k, [&](auto kc) {
-        hipLaunchKernelGGL((direction_kernel<decltype(kc)::value>), dim3(nb_), dim3(kFThreads), 0, s, da, e, part_,
+      dispatch_k(K, [&](auto kc) {
+        hipLaunchKernelGGL((direction_kernel<TS, decltype(kc)::value>), dim3(nb_), dim3(kFThreads), 0, s, da, e);
+      });
+      MDFT_HIP_CHECK(hipGetLastError());
+      excess_(xt, df_, res_ + kFfSlot, s);
+      dispatch_k(K, [&](auto kc) {
+        hipLaunchKernelGGL((update_kernel<TS, decltype(kc)::value>), dim3(nb_), dim3(kFThreads), 0, s, ua, e, part_,
                            nb_);
       });
       MDFT_HIP_CHECK(hipGetLastError());
-      reduce_rows(part_, nb_, 2, -1, res_, s);
-      excess_(xt, df_, res_ + kFfSlot, s);
-      h = read_results(2, s);
-      fparts(h[0], h[1], h[kFfSlot], Fn);
+      reduce_rows(part_, nb_, 0, 5 * K + 10, 5 * K + 9, res_, s);
+      h = read_results(s);
+      fparts(h[5 * K + 7], h[5 * K + 8], h[kFfSlot], Fn);
       ++r.evaluations;
-      if (std::isfinite(Fn[0]) && Fn[0] <= F[0] + o.c1 * t * gd) {
+      // slope at the trial point: g_t . d = (sc . g_t) / t
+      const double gtd = h[3 * K + 3] / t;
+      if (std::isfinite(Fn[0]) && (Fn[0] <= F[0] + o.c1 * t * gd ||
+                                   (Fn[0] <= F[0] + o.eps_f * fscale && gtd <= (2 * o.c1 - 1) * gd))) {
         accepted = true;
         break;
       }
@@ -578,24 +623,6 @@ LbfgsResult Functional::minimize(double* xi, const LbfgsOptions& o, hipStream_t 
       break;
     }
 
-    UpdArgs ua{};
-    ua.xt = xt;
-    ua.V = vext_;
-    ua.df = df_;
-    ua.g = g;
-    ua.sc = S[c];
-    ua.yn = Y[c];
-    for (int j = 0; j < k; ++j) {
-      ua.s[j] = S[order[j]];
-      ua.y[j] = Y[order[j]];
-    }
-    dispatch_k(k, [&](auto kc) {
-      hipLaunchKernelGGL((update_kernel<decltype(kc)::value>), dim3(nb_), dim3(kFThreads), 0, s, ua, e, part_, nb_);
-    });
-    MDFT_HIP_CHECK(hipGetLastError());
-    const int K = k;
-    reduce_rows(part_, nb_, 5 * K + 8, 5 * K + 7, res_, s);
-    h = read_results(5 * K + 8, s);
     std::vector<double> sn_y(h, h + K), s_yn(h + K + 1, h + 2 * K + 1), yn_hy(h + 3 * K + 4, h + 4 * K + 4);
     const double snyn = h[K], snsn = h[2 * K + 1], ynyn = h[2 * K + 2], sn_g = h[3 * K + 3], ynhyn = h[4 * K + 4],
                  yn_hg = h[5 * K + 5];
@@ -604,7 +631,7 @@ LbfgsResult Functional::minimize(double* xi, const LbfgsOptions& o, hipStream_t 
       YHG[j] = h[4 * K + 5 + j];
     }
     GHG = h[5 * K + 6];
-    gmax = h[5 * K + 7];
+    gmax = h[5 * K + 9];
     if (snyn > 1e-12 * std::sqrt(snsn * ynyn)) {  // keep the pair
       if (k == m) {  // drop the oldest
         order.erase(order.begin());
@@ -633,6 +660,7 @@ LbfgsResult Functional::minimize(double* xi, const LbfgsOptions& o, hipStream_t 
       YHG[k] = yn_hg;
     }
     std::swap(x, xt);
+    std::swap(g, gt);
     std::memcpy(F, Fn, sizeof(F));
     r.history.push_back(F[0]);
   }
@@ -644,5 +672,8 @@ LbfgsResult Functional::minimize(double* xi, const LbfgsOptions& o, hipStream_t 
   MDFT_HIP_CHECK(hipStreamSynchronize(s));
   return r;
 }
+
+template LbfgsResult Functional::minimize_t<double>(double*, const LbfgsOptions&, hipStream_t);
+template LbfgsResult Functional::minimize_t<float>(double*, const LbfgsOptions&, hipStream_t);
 
 }  // namespace mdft

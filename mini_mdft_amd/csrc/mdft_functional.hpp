@@ -32,6 +32,8 @@ struct LbfgsOptions {
   double gtol = 1e-5;  // converged when max |h0 g| < gtol
   double c1 = 1e-4;    // Armijo constant
   int max_backtrack = 30;
+  // noise allowance of the approximate Wolfe test, relative to |F_id| + |F_ext| + |F_exc|
+  double eps_f = 1e-6;
 };
 constexpr int kLbfgsMaxPairs = 8;
 
@@ -49,8 +51,10 @@ class Functional {
   // and writes ff to ff_dev (device), stream ordered, without synchronising
   using ExcessEval = std::function<void(const double* xi, double* df, double* ff_dev, hipStream_t s)>;
 
-  // w: quadrature weights (no); vext: device, N x no, borrowed (kept alive by the caller)
-  Functional(const FunctionalParams& p, const std::vector<double>& w, const double* vext, ExcessEval excess);
+  // w: quadrature weights (no); vext: device, N x no, borrowed (kept alive by
+  // the caller); fp32_pairs: store the L-BFGS pairs in FP32 (dot products stay FP64)
+  Functional(const FunctionalParams& p, const std::vector<double>& w, const double* vext, ExcessEval excess,
+             bool fp32_pairs = false);
   ~Functional();
   Functional(const Functional&) = delete;
   Functional& operator=(const Functional&) = delete;
@@ -68,11 +72,15 @@ class Functional {
  private:
   void reserve(int m);
   void release();
-  const double* read_results(int count, hipStream_t s);
+  const double* read_results(hipStream_t s);
+  void fparts(double sid, double sext, double ff, double F[4]) const;
+  template <typename TS>
+  LbfgsResult minimize_t(double* xi, const LbfgsOptions& o, hipStream_t s);
 
   FunctionalParams p_;
   ExcessEval excess_;
   const double* vext_;       // borrowed
+  bool fp32_pairs_ = false;
   long n_ = 0;               // N * no
   int nb_ = 0;               // blocks (= partials per sum) of the element passes
   int step_ = 0;             // (blocks * threads) mod no
@@ -81,9 +89,10 @@ class Functional {
   double* part_ = nullptr;   // device partials, kMaxSums x nb
   double* res_ = nullptr;    // device results, kMaxSums + 1
   double* host_ = nullptr;   // pinned copy of res_
-  // minimiser workspace (m + 1 slots of s and y, the trial point and g)
+  // minimiser workspace: the trial point, g and g at the trial (FP64), then
+  // m + 1 slots of s and y
   int m_alloc_ = 0;
-  double* work_ = nullptr;
+  void* work_ = nullptr;
 };
 
 }  // namespace mdft

@@ -379,17 +379,27 @@ def functional(conv: "Conv", xi: np.ndarray, V: np.ndarray, w: np.ndarray, rho0:
 
 
 def lbfgs(fg, x0: np.ndarray, h0: np.ndarray, m: int = 5, max_iter: int = 200, gtol: float = 1e-5,
-          c1: float = 1e-4, max_backtrack: int = 30):
+          c1: float = 1e-4, max_backtrack: int = 30, eps_f: float = 1e-6):
     """The L-BFGS of this build (FP64): two-loop recursion with the diagonal
     initial inverse Hessian gamma h0, gamma = s'y / y'(h0 y) of the newest
-    pair; Armijo backtracking from step 1 (halving, at most max_backtrack
-    trials); a pair is kept when s'y > 1e-12 |s| |y|; a non-descent direction
-    restarts from -h0 g.  fg(x) -> (F, g).  Returns (x, F, g, iterations,
-    evaluations, F history, status) with status 0 = converged (max |h0 g| <
-    gtol), 1 = max_iter iterations made, 2 = no Armijo point found (x is the
-    last accepted point: the decrease is below what F resolves)."""
+    pair; a pair is kept when s'y > 1e-12 |s| |y|; a non-descent direction
+    restarts from -h0 g.  Line search: backtracking by halving from t = 1 (at
+    most max_backtrack trials) until the trial point x + t d satisfies Armijo,
+    F(t) <= F + c1 t g.d, or the approximate Wolfe test of Hager and Zhang
+    (SIAM J. Optim. 16 (2005) 170-192, their (23) with delta = c1):
+    g(t).d <= (2 c1 - 1) g.d and F(t) <= F + eps_f S, S the scale of F.  The
+    second test is Armijo read off the slopes, which stay accurate where F's
+    rounding hides the decrease.  fg(x) -> (F, g) or (F, g, S); S = |F| when
+    not given.  Converged when max |h0 g| < gtol.  Returns (x, F, g,
+    iterations, evaluations, F history, status) with status 0 = converged,
+    1 = max_iter iterations made, 2 = no acceptable point (x is the last
+    accepted point)."""
+    def call(x):
+        r = fg(x)
+        return (r[0], r[1], abs(r[0])) if len(r) == 2 else r
+
     x = x0.copy()
-    F, g = fg(x)
+    F, g, scale = call(x)
     S, Y = [], []
     hist, nev = [F], 1
     for it in range(max_iter):
@@ -416,9 +426,10 @@ def lbfgs(fg, x0: np.ndarray, h0: np.ndarray, m: int = 5, max_iter: int = 200, g
         t = 1.0
         for _ in range(max_backtrack):
             xn = x + t * d
-            Fn, gn = fg(xn)
+            Fn, gn, sn = call(xn)
             nev += 1
-            if np.isfinite(Fn) and Fn <= F + c1 * t * gd:
+            if np.isfinite(Fn) and (Fn <= F + c1 * t * gd or
+                                    (Fn <= F + eps_f * scale and gn @ d <= (2 * c1 - 1) * gd)):
                 break
             t *= 0.5
         else:
@@ -430,7 +441,7 @@ def lbfgs(fg, x0: np.ndarray, h0: np.ndarray, m: int = 5, max_iter: int = 200, g
             if len(S) > m:
                 S.pop(0)
                 Y.pop(0)
-        x, F, g = xn, Fn, gn
+        x, F, g, scale = xn, Fn, gn, sn
         hist.append(F)
     if np.abs(h0 * g).max() < gtol:
         return x, F, g, max_iter, nev, hist, 0

@@ -188,8 +188,8 @@ def test_lbfgs_c0_limit_is_boltzmann(oracle):
     s, w, conv, V = _setup(oracle, amp=0.0, vmax_kt=10.0)
     h0 = np.broadcast_to(1.0 / (2 * RHO0 * DV * w * KT), (NPTS, s["no"])).ravel()
     def fg(x):
-        (F, *_), g = oracle.functional(conv, x.reshape(NPTS, -1), V, w, RHO0, PREF, KT, DV)
-        return F, g.ravel()
+        (F, *parts), g = oracle.functional(conv, x.reshape(NPTS, -1), V, w, RHO0, PREF, KT, DV)
+        return F, g.ravel(), sum(abs(p) for p in parts)
     x, F, g, it, nev, hist, status = oracle.lbfgs(fg, np.ones(NPTS * s["no"]), h0, m=5, max_iter=300, gtol=1e-10)
     assert status == 0
     assert np.abs(np.abs(x) - np.exp(-V / (2 * KT)).ravel()).max() <= 1e-9
@@ -203,17 +203,40 @@ def test_lbfgs_monotone_and_matches_scipy(oracle):
     s, w, conv, V = _setup(oracle)
     h0 = np.broadcast_to(1.0 / (2 * RHO0 * DV * w * KT), (NPTS, s["no"])).ravel()
     def fg(x):
-        (F, *_), g = oracle.functional(conv, x.reshape(NPTS, -1), V, w, RHO0, PREF, KT, DV)
-        return F, g.ravel()
+        (F, *parts), g = oracle.functional(conv, x.reshape(NPTS, -1), V, w, RHO0, PREF, KT, DV)
+        return F, g.ravel(), sum(abs(p) for p in parts)
     x, F, g, it, nev, hist, status = oracle.lbfgs(fg, np.ones(NPTS * s["no"]), h0, m=5, max_iter=300, gtol=1e-6)
     assert status == 0 and np.abs(h0 * g).max() < 1e-6
-    assert all(b <= a for a, b in zip(hist, hist[1:]))
+    # F decreases, up to the approximate Wolfe test's allowance of 1e-6 of its scale
+    tol = 2e-6 * abs(hist[0])
+    assert all(b <= a + tol for a, b in zip(hist, hist[1:]))
     # scipy's L-BFGS-B in the scaled variable y = x / sqrt(h0) reaches the same minimum
     sc = np.sqrt(h0)
-    res = minimize(lambda y: (lambda F, g: (F, g * sc))(*fg(y * sc)), 1.0 / sc, jac=True, method="L-BFGS-B",
+    res = minimize(lambda y: (lambda F, g, _: (F, g * sc))(*fg(y * sc)), 1.0 / sc, jac=True, method="L-BFGS-B",
                    options=dict(maxiter=1000, gtol=1e-12, ftol=1e-16))
     assert abs(res.fun - F) <= 1e-10 * abs(F)
     assert np.abs(np.abs(res.x * sc) - np.abs(x)).max() <= 1e-4
+
+
+def test_lbfgs_approximate_wolfe_carries_a_noisy_f(oracle):
+    """With F perturbed by a deterministic noise of 1e-9 of its scale (the
+    rounding of a lower-precision evaluation) and the gradient exact, the
+    slope test keeps the line search decisive and the solve converges; with
+    that test switched off (eps_f = -inf) Armijo alone stalls."""
+    s, w, conv, V = _setup(oracle)
+    h0 = np.broadcast_to(1.0 / (2 * RHO0 * DV * w * KT), (NPTS, s["no"])).ravel()
+
+    def fg(x):
+        (F, *parts), g = oracle.functional(conv, x.reshape(NPTS, -1), V, w, RHO0, PREF, KT, DV)
+        scale = sum(abs(p) for p in parts)
+        noise = 1e-9 * scale * np.sin(1e7 * float(np.sum(x[::97])))
+        return F + noise, g.ravel(), scale
+
+    x0 = np.ones(NPTS * s["no"])
+    res = oracle.lbfgs(fg, x0, h0, m=5, max_iter=400, gtol=1e-7)
+    assert res[6] == 0
+    res = oracle.lbfgs(fg, x0, h0, m=5, max_iter=400, gtol=1e-7, eps_f=-np.inf)
+    assert res[6] == 2 and np.abs(h0 * res[2]).max() > 1e-7
 
 
 # ---- the library (C ABI) against the oracle -----------------------------------
@@ -284,8 +307,8 @@ def test_gpu_lbfgs_trajectory_matches_oracle(oracle):
     h0 = np.broadcast_to(1.0 / (2 * RHO0 * P["dv"] * w * KT), (P["N"], no)).ravel()
 
     def fg(x):
-        (F, *_), g = oracle.functional(P["oconv"], x.reshape(P["N"], -1), Vh, w, RHO0, PREF, KT, P["dv"])
-        return F, g.ravel()
+        (F, *parts), g = oracle.functional(P["oconv"], x.reshape(P["N"], -1), Vh, w, RHO0, PREF, KT, P["dv"])
+        return F, g.ravel(), sum(abs(p) for p in parts)
 
     x0 = np.ones(P["N"] * no)
     ref = oracle.lbfgs(fg, x0, h0, m=5, max_iter=10, gtol=0.0)
@@ -312,7 +335,9 @@ def test_gpu_lbfgs_c0_limit(oracle):
     assert r["status"] == 0
     exact = torch.exp(-P["V"] / (2 * KT))
     assert float((xi.abs() - exact).abs().max()) <= 1e-9
-    assert all(b <= a for a, b in zip(r["history"], r["history"][1:]))
+    # F decreases, up to the approximate Wolfe test's allowance of 1e-6 of its scale
+    tol = 2e-6 * sum(abs(p) for p in r["F"][1:])
+    assert all(b <= a + tol for a, b in zip(r["history"], r["history"][1:]))
 
 
 @pytest.mark.gpu
@@ -329,7 +354,7 @@ def test_gpu_lbfgs_mixed_vs_fp64(oracle, prec):
     r64 = P64["fun"].minimize(x64, max_iter=300, gtol=1e-5)
     xm = torch.ones(shape, dtype=torch.float64, device="cuda")
     rm = Pm["fun"].minimize(xm, max_iter=300, gtol=1e-5)
-    assert r64["status"] == 0 and rm["status"] in (0, 2)
+    assert r64["status"] == 0 and rm["status"] == 0
     assert abs(rm["F"][0] - r64["F"][0]) <= 0.01
     g = torch.empty(shape, dtype=torch.float64, device="cuda")
     F_at_m = P64["fun"].eval(xm, g)
