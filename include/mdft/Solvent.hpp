@@ -82,6 +82,7 @@ class ExcessFunctional {
     Impl::check(mdft_solver_workspace_bytes(m_h, &b));
     return b;
   }
+  mdft_solver_t handle() const { return m_h; }
 
  private:
   template <typename View4DType>

@@ -344,6 +344,98 @@ void test_excess_functional(const std::string& path, int np_luc) {
   }
 }
 
+
+// The full functional through the facade: V_ext of one ion seen by a one-site
+// solvent against the closed form, F and g against host sums over the oracle's
+// excess evaluation, and a few L-BFGS iterations lowering F.
+void test_functional(const std::string& path, int np_luc) {
+  const int n = 8;
+  const double L = 10.0;
+  MDFT::SpatialGrid<execution_space, double> grid({n, n, n}, {L, L, L});
+  MDFT::AngularGrid<execution_space, double> ag(3, 2);
+  MDFT::OrientationProjectionMap<execution_space, double> map(ag);
+  execution_space exec;
+  const double n0 = 0.0332891, T = 298.15, kT = 1.38064852e-23 * 6.02214129e23 * T * 1e-3;
+  const double rho0 = n0 * 2 / (8 * M_PI * M_PI), pref = -2 * kT * 4 * M_PI * M_PI / 2 / n0;
+  const double qfact = 1.602176565e-19 * 1.602176565e-19 * 1.0e-3 * 6.02214129e23 /
+                       (4.0 * M_PI * 8.854187817e-12 * 1.0e-10);
+  const int no = ag.m_no;
+  const size_t N = static_cast<size_t>(n) * n * n, total = N * no;
+  // one ion at (4.1, 5.3, 6.2) and a one-site solvent
+  const std::vector<MDFT::Site> ion{{0.3, 2.5, 0.5, 4.1, 5.3, 6.2}};
+  const std::vector<MDFT::Site> site{{-0.8, 3.15, 0.64, 0.0, 0.0, 0.0}};
+  MDFT::View<double, 4> vd("vext", n, n, n, no);
+  const double vmax = 50.0;
+  MDFT::compute_vext(exec, grid, ag, ion, site, vmax, vd);
+  auto vh = MDFT::create_mirror_view_and_copy(vd);
+  const double sig = 0.5 * (2.5 + 3.15), eps4 = 4.0 * std::sqrt(0.5 * 0.64);
+  double verr = 0.0;
+  for (size_t r = 0; r < N; ++r) {
+    const double p[3] = {double(r / (n * n)) * L / n, double((r / n) % n) * L / n, double(r % n) * L / n};
+    const double c[3] = {4.1, 5.3, 6.2};
+    double d2 = 0.0;
+    for (int a = 0; a < 3; ++a) {
+      double d = p[a] - c[a];
+      d -= L * std::nearbyint(d / L);
+      d2 += d * d;
+    }
+    const double d = std::sqrt(d2), sr6 = std::pow(sig / d, 6);
+    const double ref = std::min(eps4 * (sr6 * sr6 - sr6) + qfact * 0.3 * -0.8 / d, vmax);
+    for (int io = 0; io < no; ++io) verr = std::max(verr, std::abs(vh[r * no + io] - ref));
+  }
+  std::printf("    V_ext max |err| = %.3g kJ/mol\n", verr);
+  EXPECT_TRUE(verr <= 1e-11);
+  auto rot = MDFT::rotation_matrices(ag);
+  EXPECT_TRUE(rot.size() == static_cast<size_t>(no) && std::abs(rot[0][8] - std::cos(ag.m_thetaofntheta[0])) < 1e-15);
+
+  std::mt19937_64 gen(9);
+  std::uniform_real_distribution<double> U(-1.0, 1.0);
+  std::vector<double> xi(total);
+  for (double& v : xi) v = 1.0 + 0.1 * U(gen);
+  const double qmax = std::sqrt(3.0) * M_PI * n / L;
+  LucFile f = read_luc(path, np_luc, qmax);
+  std::vector<int> cols[5];
+  for (int j = 0; j < 5; ++j)
+    for (int i = 0; i < np_luc; ++i) cols[j].push_back(f.tuples[i * 5 + j]);
+  void* oc = orc_conv_create(n, n, n, L, L, L, 3, 2, np_luc, cols[0].data(), cols[1].data(), cols[2].data(),
+                             cols[3].data(), cols[4].data(), reinterpret_cast<const double*>(f.c.data()), f.nq, f.dq);
+  std::vector<double> df_ref(total);
+  double ff_ref = 0.0;
+  orc_energy_cproj(oc, xi.data(), rho0, pref, df_ref.data(), &ff_ref, nullptr);
+  orc_conv_destroy(oc);
+  const double dv = L * L * L / N;
+  double fid = 0.0, fext = 0.0;
+  std::vector<double> g_ref(total);
+  for (size_t i = 0; i < total; ++i) {
+    const double w = ag.m_w[i % no], x2 = xi[i] * xi[i];
+    fid += w * (x2 * std::log(x2) - x2 + 1.0);
+    fext += w * x2 * vh[i];
+    g_ref[i] = dv * w * (2 * rho0 * xi[i] * (kT * std::log(x2) + vh[i]) + 2 * pref / N * df_ref[i]);
+  }
+  const double F_ref[4] = {kT * rho0 * dv * fid + rho0 * dv * fext + dv / N * ff_ref, kT * rho0 * dv * fid,
+                           rho0 * dv * fext, dv / N * ff_ref};
+  MDFT::Convolution<execution_space, double> conv(path, grid, ag, map, np_luc);
+  MDFT::ExcessFunctional<execution_space, double> excess(conv, rho0, pref);
+  MDFT::Functional<execution_space, double> fun(excess, vd, kT);
+  MDFT::View<double, 4> xd("xi", n, n, n, no), gd("g", n, n, n, no);
+  MDFT::deep_copy(xd, xi);
+  const auto F = fun.eval(xd, gd);
+  auto gh = MDFT::create_mirror_view_and_copy(gd);
+  g_where = "Functional::eval";
+  std::printf("    F = %.12g (F_id %.6g, F_ext %.6g, F_exc %.6g) kJ/mol, host sums %.12g\n", F[0], F[1], F[2], F[3],
+              F_ref[0]);
+  for (int k = 0; k < 4; ++k) EXPECT_TRUE(std::abs(F[k] - F_ref[k]) <= 1e-10 * std::abs(F_ref[1] + F_ref[2]));
+  EXPECT_TRUE(rel_err(gh, g_ref) < 1e-10);
+  // a few iterations from xi = 1 lower F
+  std::vector<double> ones(total, 1.0);
+  MDFT::deep_copy(xd, ones);
+  const auto r = fun.minimize(xd, 5, 5, 1e-5);
+  std::printf("    minimize: %d iterations, %d evaluations, status %d, F %.8g -> %.8g\n", r.iterations,
+              r.evaluations, r.status, r.history.front(), r.F[0]);
+  EXPECT_TRUE(r.iterations == 5 && r.status == 1 && r.F[0] < r.history.front());
+  expect_throw([&] { fun.minimize(xd, 9, 5, 1e-5); }, "m must be in 1 .. 8");
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -379,6 +471,7 @@ int main(int argc, char** argv) {
     run("get_delta_rho/get_delta_f<float>", [] { test_solvent_functions<float>(); });
     run("get_delta_rho/get_delta_f<double>", [] { test_solvent_functions<double>(); });
     run("ExcessFunctional energy_cproj_mrso 8^3", [&] { test_excess_functional(path, np_luc); });
+    run("Functional: V_ext, eval, minimize 8^3", [&] { test_functional(path, np_luc); });
   }
   std::printf("[==========] %d tests, %d failed\n", g_run, g_failed);
   return g_failed == 0 ? 0 : 1;
