@@ -155,6 +155,20 @@ int mdft_energy_cproj(mdft_solver_t h, const void* xi, void* df, double* ff, voi
 /* ff_device: one device double written on the stream, no synchronisation */
 int mdft_energy_cproj_async(mdft_solver_t h, const void* xi, void* df, double* ff_device, void* stream);
 
+/* Same-shape batches (no counterpart in the reference, which evaluates one
+ * field per Solver): a solver created with capacity B evaluates up to B
+ * fields in one pass of every stage.  xi and df hold nbatch fields back to
+ * back (nbatch x N x no elements); ff (host, synchronising) or ff_device
+ * receives nbatch values.  Each field's result is bitwise the one it gets
+ * alone (same per-point arithmetic, same partial sums in the same order).
+ * Needs N = nx ny nz to be a multiple of 64 when capacity > 1;
+ * mdft_solver_create is capacity 1. */
+int mdft_solver_create_batch(mdft_conv_t conv, double rho0, double prefactor, int precision, int capacity,
+                             mdft_solver_t* out);
+int mdft_energy_cproj_batch(mdft_solver_t h, int nbatch, const void* xi, void* df, double* ff, void* stream);
+int mdft_energy_cproj_batch_async(mdft_solver_t h, int nbatch, const void* xi, void* df, double* ff_device,
+                                  void* stream);
+
 /* Stage timers (the Solver's per-stage Kokkos::Timer printout,
  * src/mini-mdft.hpp:140-192), as hipEvents recorded on the work stream
  * without synchronising between evaluations.  set_timing(1) starts (and

@@ -60,6 +60,9 @@ _SIGS = {
     "mdft_solver_set_timing": (_i, [_vp, _i]),
     "mdft_solver_stage_ms": (_i, [_vp, _pd, ctypes.POINTER(ctypes.c_long)]),
     "mdft_fill_uniform": (_i, [_vp, _sz, _ull, _ull, _d, _d, _i, _vp]),
+    "mdft_solver_create_batch": (_i, [_vp, _d, _d, _i, _i, ctypes.POINTER(_vp)]),
+    "mdft_energy_cproj_batch": (_i, [_vp, _i, _vp, _vp, _pd, _vp]),
+    "mdft_energy_cproj_batch_async": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     "mdft_rotation_matrices": (_i, [_i, _i, _pd]),
     "mdft_vext": (_i, [_i, _i, _i, _d, _d, _d, _i, _i, _i, _pd, _i, _pd, _d, _vp, _vp]),
     "mdft_functional_create": (_i, [_vp, _vp, _d, ctypes.POINTER(_vp)]),

@@ -74,6 +74,8 @@ struct mdft_solver_s {
   int precision;
   double rho0, prefactor;
   long N;
+  int capacity = 1;       // fields one call may evaluate together
+  int field_parts = 0;    // partial sums per field
   std::unique_ptr<ProjTables> tab;
   std::unique_ptr<ProjGemm> gemm;
   void* pbuf = nullptr;  // projections, np x N complex
@@ -372,11 +374,13 @@ int mdft_delta_f(const void* xi, const void* vexc, const double* w, void* delta_
 }
 
 // ---------------------------------------------------------------------------
-int mdft_solver_create(mdft_conv_t conv, double rho0, double prefactor, int precision, mdft_solver_t* out) {
+int mdft_solver_create_batch(mdft_conv_t conv, double rho0, double prefactor, int precision, int capacity,
+                             mdft_solver_t* out) {
   return guarded([&] {
     throw_if(!out || !conv, "solver_create: null argument");
     *out = nullptr;
     check_precision(precision);
+    throw_if(capacity < 1, "solver_create: capacity must be at least 1");
     throw_if((precision == MDFT_FP64) != (conv->precision == MDFT_FP64),
              "solver and convolution must agree on FP64 vs FP32 projections");
     auto h = std::make_unique<mdft_solver_s>();
@@ -384,18 +388,25 @@ int mdft_solver_create(mdft_conv_t conv, double rho0, double prefactor, int prec
     h->precision = precision;
     h->rho0 = rho0;
     h->prefactor = prefactor;
+    h->capacity = capacity;
     const ConvTables& t = conv->dev->tab;
     h->N = static_cast<long>(t.nx) * t.ny * t.nz;
+    throw_if(capacity > 1 && h->N % 64 != 0, "batched evaluation needs nx ny nz to be a multiple of 64");
     h->tab = std::make_unique<ProjTables>(t.ang);
     h->gemm = make_gemm(t.ang, precision);
-    const size_t pbytes = static_cast<size_t>(h->N) * t.ang.np * (precision == MDFT_FP64 ? 16 : 8);
-    h->nparts = std::max(proj2angl_blocks(*h->tab, h->N), h->gemm ? h->gemm->p2a_blocks(h->N) : 0);
+    h->field_parts = h->gemm ? h->gemm->p2a_blocks(h->N) : proj2angl_blocks(*h->tab, h->N);
+    const size_t pbytes = static_cast<size_t>(h->N) * t.ang.np * capacity * (precision == MDFT_FP64 ? 16 : 8);
+    h->nparts = h->field_parts * capacity;
     MDFT_HIP_CHECK(hipMalloc(&h->pbuf, pbytes));
     MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&h->partials), h->nparts * sizeof(double)));
-    MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&h->ffdev), sizeof(double)));
-    h->bytes = pbytes + h->nparts * sizeof(double) + sizeof(double);
+    MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&h->ffdev), capacity * sizeof(double)));
+    h->bytes = pbytes + (h->nparts + capacity) * sizeof(double);
     *out = h.release();
   });
+}
+
+int mdft_solver_create(mdft_conv_t conv, double rho0, double prefactor, int precision, mdft_solver_t* out) {
+  return mdft_solver_create_batch(conv, rho0, prefactor, precision, 1, out);
 }
 
 int mdft_solver_workspace_bytes(mdft_solver_t h, size_t* bytes) {
@@ -409,45 +420,50 @@ int mdft_solver_destroy(mdft_solver_t h) {
   return guarded([&] { delete h; });
 }
 
-static void energy_impl(mdft_solver_t h, const void* xi, void* df, double* ffd, hipStream_t s) {
+// nb fields back to back in xi and df (N no each); ff of field j into
+// ffd[j ffstride].  Every stage runs once over all nb fields: the pointwise
+// projections over nb N points, the FFTs over np nb transforms, the OZ pass
+// over the nb fields of each pair.  Each field's arithmetic (and its partial
+// sums, reduced in the same order) is the same as when evaluated alone.
+static void energy_impl(mdft_solver_t h, int nb, const void* xi, void* df, double* ffd, long ffstride,
+                        hipStream_t s) {
   throw_if(!h || !xi || !df || !ffd, "energy_cproj: null argument");
+  throw_if(nb < 1 || nb > h->capacity, "energy_cproj: batch size exceeds the solver's capacity");
   const double rho0 = h->rho0;
   const double eps = chop_eps_of(h->precision);
+  const long Nb = h->N * nb;
   StageTimer* tm = h->timer.get();
   if (tm) {
     tm->begin_eval();
     tm->mark(s, kStageStart);
   }
-  int nb = 0;
   switch (h->precision) {
     case MDFT_FP64: {
       auto* p = static_cast<cplx<double>*>(h->pbuf);
-      angl2proj_factorized<double, double, double>(*h->tab, h->N, static_cast<const double*>(xi), p, &rho0, s);
-      h->conv->dev->execute<double>(p, eps, s, tm);
-      proj2angl_factorized<double, double, double>(*h->tab, h->N, p, nullptr, static_cast<const double*>(xi),
+      angl2proj_factorized<double, double, double>(*h->tab, Nb, static_cast<const double*>(xi), p, &rho0, s);
+      h->conv->dev->execute<double>(p, eps, s, tm, nb);
+      proj2angl_factorized<double, double, double>(*h->tab, Nb, p, nullptr, static_cast<const double*>(xi),
                                                    static_cast<double*>(df), rho0, h->partials, s);
-      nb = proj2angl_blocks(*h->tab, h->N);
       break;
     }
     case MDFT_FP32: {
       auto* p = static_cast<cplx<float>*>(h->pbuf);
-      angl2proj_factorized<float, float, float>(*h->tab, h->N, static_cast<const float*>(xi), p, &rho0, s);
-      h->conv->dev->execute<float>(p, eps, s, tm);
-      proj2angl_factorized<float, float, float>(*h->tab, h->N, p, nullptr, static_cast<const float*>(xi),
+      angl2proj_factorized<float, float, float>(*h->tab, Nb, static_cast<const float*>(xi), p, &rho0, s);
+      h->conv->dev->execute<float>(p, eps, s, tm, nb);
+      proj2angl_factorized<float, float, float>(*h->tab, Nb, p, nullptr, static_cast<const float*>(xi),
                                                 static_cast<float*>(df), rho0, h->partials, s);
-      nb = proj2angl_blocks(*h->tab, h->N);
       break;
     }
     default: {
       auto* p = static_cast<cplx<float>*>(h->pbuf);
-      h->gemm->angl2proj(h->N, static_cast<const double*>(xi), p, &rho0, s);
-      h->conv->dev->execute<float>(p, eps, s, tm);
-      nb = h->gemm->proj2angl(h->N, p, nullptr, static_cast<const double*>(xi), static_cast<double*>(df), rho0,
-                              h->partials, s);
+      h->gemm->angl2proj(Nb, static_cast<const double*>(xi), p, &rho0, s);
+      h->conv->dev->execute<float>(p, eps, s, tm, nb);
+      h->gemm->proj2angl(Nb, p, nullptr, static_cast<const double*>(xi), static_cast<double*>(df), rho0, h->partials,
+                         s);
       break;
     }
   }
-  reduce_partials(h->partials, nb, rho0 * h->prefactor, ffd, s);
+  reduce_partials(h->partials, h->field_parts, rho0 * h->prefactor, ffd, s, nb, ffstride);
   if (tm) tm->mark(s, kStageEnd);
 }
 
@@ -472,17 +488,26 @@ int mdft_solver_stage_ms(mdft_solver_t h, double* ms, long* count) {
 }
 
 int mdft_energy_cproj(mdft_solver_t h, const void* xi, void* df, double* ff, void* stream) {
+  return mdft_energy_cproj_batch(h, 1, xi, df, ff, stream);
+}
+
+int mdft_energy_cproj_async(mdft_solver_t h, const void* xi, void* df, double* ff_device, void* stream) {
+  return guarded([&] { energy_impl(h, 1, xi, df, ff_device, 1, as_stream(stream)); });
+}
+
+int mdft_energy_cproj_batch(mdft_solver_t h, int nbatch, const void* xi, void* df, double* ff, void* stream) {
   return guarded([&] {
     throw_if(!ff, "energy_cproj: null ff");
     hipStream_t s = as_stream(stream);
-    energy_impl(h, xi, df, h ? h->ffdev : nullptr, s);
-    MDFT_HIP_CHECK(hipMemcpyAsync(ff, h->ffdev, sizeof(double), hipMemcpyDeviceToHost, s));
+    energy_impl(h, nbatch, xi, df, h ? h->ffdev : nullptr, 1, s);
+    MDFT_HIP_CHECK(hipMemcpyAsync(ff, h->ffdev, nbatch * sizeof(double), hipMemcpyDeviceToHost, s));
     MDFT_HIP_CHECK(hipStreamSynchronize(s));
   });
 }
 
-int mdft_energy_cproj_async(mdft_solver_t h, const void* xi, void* df, double* ff_device, void* stream) {
-  return guarded([&] { energy_impl(h, xi, df, ff_device, as_stream(stream)); });
+int mdft_energy_cproj_batch_async(mdft_solver_t h, int nbatch, const void* xi, void* df, double* ff_device,
+                                  void* stream) {
+  return guarded([&] { energy_impl(h, nbatch, xi, df, ff_device, 1, as_stream(stream)); });
 }
 
 int mdft_fill_uniform(void* x, size_t n, unsigned long long seed, unsigned long long offset, double lo, double hi,
@@ -537,7 +562,7 @@ int mdft_functional_create(mdft_solver_t solver, const double* vext, double kT, 
     // the mixed forms keep their L-BFGS pairs in FP32
     h->f = std::make_unique<Functional>(
         p, t.ang.w, vext,
-        [solver](const double* xi, double* df, double* ffd, hipStream_t s) { energy_impl(solver, xi, df, ffd, s); },
+        [solver](const double* xi, double* df, double* ffd, hipStream_t s) { energy_impl(solver, 1, xi, df, ffd, 1, s); },
         solver->precision != MDFT_FP64);
     *out = h.release();
   });

@@ -44,6 +44,8 @@ struct OzArgs {
   int nx, ny, nz, mmax, mrso, np, np_new, nq;
   double dq;
   long npairs, chunk, N;
+  long pstride;  // elements between projections: N, or nbatch N for a batch of fields
+  int nbatch;    // fields in the batch (job j at offset j N within each projection)
   const double* kxyz;
   const int* pairs;
   const int* ipinfo;    // m | (mup + mmax) << 8 | mu2 << 16
@@ -52,6 +54,20 @@ struct OzArgs {
   const double* rc;     // a (5x13x13) | b (5x13x13) | c (5x13) | d (5x13)
   T chop_eps;
 };
+
+// The block -> (pair, field) map of the OZ pass.  Launch index L = 8 rest +
+// xcd runs on XCD xcd; rest = nbatch k + job.  The single-field block
+// b0 = 8 k + xcd takes pair (b0 mod 8) chunk + b0 / 8, so each XCD owns a
+// contiguous pair range and the nbatch fields of one pair run back to back on
+// the same XCD (their d-table rows are read once into its L2).
+template <typename A>
+__device__ __forceinline__ long oz_block_pair(const A& a, int& job) {
+  const long L = blockIdx.x;
+  const long xcd = L % kXcds, rest = L / kXcds;
+  job = static_cast<int>(rest % a.nbatch);
+  const long b0 = (rest / a.nbatch) * kXcds + xcd;
+  return (b0 % kXcds) * a.chunk + b0 / kXcds;
+}
 
 __device__ __forceinline__ int roff(int l) { return l * (4 * l * l - 1) / 3; }
 
@@ -295,9 +311,10 @@ __global__ __launch_bounds__(kWave) void oz_pair_ct_kernel(cplx<T>* __restrict__
   __shared__ cplx<T> ph[W];  // e^{-i mup phi_q}
   const int lane = threadIdx.x;
 
-  const long b = blockIdx.x;
-  const long pair = (b % kXcds) * a.chunk + b / kXcds;
+  int job;
+  const long pair = oz_block_pair(a, job);
   if (pair >= a.npairs) return;  // uniform across the block
+  p += static_cast<long>(job) * a.N;
   const int pk = a.pairs[pair];
   const int ix = pk & 1023, iy = (pk >> 10) & 1023, iz = (pk >> 20) & 1023;
   const int ixm = ix == 0 ? 0 : a.nx - ix, iym = iy == 0 ? 0 : a.ny - iy, izm = iz == 0 ? 0 : a.nz - iz;
@@ -312,7 +329,7 @@ __global__ __launch_bounds__(kWave) void oz_pair_ct_kernel(cplx<T>* __restrict__
 #pragma unroll
   for (int t = 0; t < S::NSLOT; ++t) {
     const int ip = min(lane + t * kWave, S::NP - 1);
-    const cplx<T> u = p[static_cast<long>(ip) * a.N + sq], v = p[static_cast<long>(ip) * a.N + smq];
+    const cplx<T> u = p[static_cast<long>(ip) * a.pstride + sq], v = p[static_cast<long>(ip) * a.pstride + smq];
     g0x[t] = u.x;
     g0y[t] = u.y;
     g1x[t] = v.x;
@@ -480,8 +497,8 @@ __global__ __launch_bounds__(kWave) void oz_pair_ct_kernel(cplx<T>* __restrict__
       const cplx<T> o0 = cmul<T>(mkc<T>(ar, ai), cconj<T>(e));
       cplx<T> o1 = cmul<T>(mkc<T>(br, bi), cconj<T>(e));
       if (m & 1) o1 = mkc<T>(-o1.x, -o1.y);
-      p[static_cast<long>(ip) * a.N + sq] = o0;
-      p[static_cast<long>(ip) * a.N + smq] = singular ? cconj<T>(o1) : o1;
+      p[static_cast<long>(ip) * a.pstride + sq] = o0;
+      p[static_cast<long>(ip) * a.pstride + smq] = singular ? cconj<T>(o1) : o1;
     }
   }
 }
@@ -503,9 +520,10 @@ __global__ __launch_bounds__(kWave) void oz_pair_kernel(cplx<T>* __restrict__ p,
   __shared__ int mbase[kMaxM + 2];       // first ip of each m
   const int lane = threadIdx.x;
 
-  const long b = blockIdx.x;
-  const long pair = (b % kXcds) * a.chunk + b / kXcds;
+  int job;
+  const long pair = oz_block_pair(a, job);
   if (pair >= a.npairs) return;  // uniform across the block
+  p += static_cast<long>(job) * a.N;
   const int pk = a.pairs[pair];
   const int ix = pk & 1023, iy = (pk >> 10) & 1023, iz = (pk >> 20) & 1023;
   const int ixm = ix == 0 ? 0 : a.nx - ix, iym = iy == 0 ? 0 : a.ny - iy, izm = iz == 0 ? 0 : a.nz - iz;
@@ -522,7 +540,7 @@ __global__ __launch_bounds__(kWave) void oz_pair_kernel(cplx<T>* __restrict__ p,
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const int ip = min(lane + t * kWave, np - 1);
-    const cplx<T> u = p[static_cast<long>(ip) * a.N + sq], v = p[static_cast<long>(ip) * a.N + smq];
+    const cplx<T> u = p[static_cast<long>(ip) * a.pstride + sq], v = p[static_cast<long>(ip) * a.pstride + smq];
     g0x[t] = u.x;
     g0y[t] = u.y;
     g1x[t] = v.x;
@@ -692,8 +710,8 @@ __global__ __launch_bounds__(kWave) void oz_pair_kernel(cplx<T>* __restrict__ p,
     cplx<T> o0 = cmul<T>(mkc<T>(ar, ai), cconj<T>(e));
     cplx<T> o1 = cmul<T>(mkc<T>(br, bi), cconj<T>(e));
     if (m & 1) o1 = mkc<T>(-o1.x, -o1.y);
-    p[static_cast<long>(ip) * a.N + sq] = o0;
-    p[static_cast<long>(ip) * a.N + smq] = singular ? cconj<T>(o1) : o1;
+    p[static_cast<long>(ip) * a.pstride + sq] = o0;
+    p[static_cast<long>(ip) * a.pstride + smq] = singular ? cconj<T>(o1) : o1;
   }
 }
 
@@ -826,6 +844,8 @@ static OzArgs<T> oz_args(const ConvDevice& c, double chop_eps) {
   a.npairs = c.npairs;
   a.chunk = (c.npairs + kXcds - 1) / kXcds;
   a.N = static_cast<long>(tab.nx) * tab.ny * tab.nz;
+  a.pstride = a.N;
+  a.nbatch = 1;
   a.kxyz = c.kxyz; a.pairs = c.pairs; a.ipinfo = c.ipinfo; a.ia_start = c.ia_start;
   a.ctab = std::is_same<T, double>::value ? reinterpret_cast<const cplx<T>*>(c.ctab64)
                                           : reinterpret_cast<const cplx<T>*>(c.ctab32);
@@ -849,12 +869,15 @@ const T* ConvDevice::dtable(hipStream_t s) const {
 }
 
 template <typename T>
-void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s) const {
+void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch) const {
   if (npairs == 0) return;
   throw_if(tab.ang.np > 4 * kWave, "the OZ kernel handles at most 256 projections");
+  throw_if(nbatch < 1, "OZ pass: bad batch");
   const T* dt = dtable<T>(s);
   OzArgs<T> a = oz_args<T>(*this, chop_eps);
-  const long nblk = a.chunk * kXcds;
+  a.nbatch = nbatch;
+  a.pstride = a.N * nbatch;
+  const long nblk = a.chunk * kXcds * nbatch;
   throw_if(nblk > 0x7fffffffL, "too many q pairs");
   if (padw) {
     dispatch_oz(tab.mmax, tab.mrso, [&](auto mm, auto mr) {
@@ -877,20 +900,20 @@ void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s) const {
 }
 
 template <typename T>
-void ConvDevice::execute(cplx<T>* p, double chop_eps, hipStream_t s, StageTimer* timer) const {
-  const long batch = tab.ang.np;
+void ConvDevice::execute(cplx<T>* p, double chop_eps, hipStream_t s, StageTimer* timer, int nbatch) const {
+  const long batch = static_cast<long>(tab.ang.np) * nbatch;
   if (timer) timer->mark(s, kStageConvStart);
   fft.execute<T>(p, batch, -1, s);
   if (timer) timer->mark(s, kStageFftFwd);
-  oz_pass<T>(p, chop_eps, s);
+  oz_pass<T>(p, chop_eps, s, nbatch);
   if (timer) timer->mark(s, kStageOz);
   fft.execute<T>(p, batch, +1, s);
   if (timer) timer->mark(s, kStageFftBwd);
 }
 
-template void ConvDevice::oz_pass<double>(cplx<double>*, double, hipStream_t) const;
-template void ConvDevice::oz_pass<float>(cplx<float>*, double, hipStream_t) const;
-template void ConvDevice::execute<double>(cplx<double>*, double, hipStream_t, StageTimer*) const;
-template void ConvDevice::execute<float>(cplx<float>*, double, hipStream_t, StageTimer*) const;
+template void ConvDevice::oz_pass<double>(cplx<double>*, double, hipStream_t, int) const;
+template void ConvDevice::oz_pass<float>(cplx<float>*, double, hipStream_t, int) const;
+template void ConvDevice::execute<double>(cplx<double>*, double, hipStream_t, StageTimer*, int) const;
+template void ConvDevice::execute<float>(cplx<float>*, double, hipStream_t, StageTimer*, int) const;
 
 }  // namespace mdft

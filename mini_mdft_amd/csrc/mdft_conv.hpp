@@ -35,13 +35,14 @@ struct ConvDevice {
   int padw = 0;
 
   // the per-(q, -q) rotation / OZ / inverse-rotation pass on FFT'd data;
-  // gamma components below chop_eps in magnitude are set to 0
+  // gamma components below chop_eps in magnitude are set to 0.  With
+  // nbatch > 1, p holds nbatch fields as (np, nbatch, nx, ny, nz).
   template <typename T>
-  void oz_pass(cplx<T>* p, double chop_eps, hipStream_t s) const;
-  // FFT, oz_pass, inverse FFT: in place on (np, nx, ny, nz); marks the
-  // FFT / OZ stage boundaries on `timer` when given
+  void oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch = 1) const;
+  // FFT, oz_pass, inverse FFT: in place on (np, nx, ny, nz) (or (np, nbatch,
+  // nx, ny, nz)); marks the FFT / OZ stage boundaries on `timer` when given
   template <typename T>
-  void execute(cplx<T>* p, double chop_eps, hipStream_t s, StageTimer* timer = nullptr) const;
+  void execute(cplx<T>* p, double chop_eps, hipStream_t s, StageTimer* timer = nullptr, int nbatch = 1) const;
 };
 
 // (ix, iy, iz) triples of the reference's (q, -q) list with the same q / -q

@@ -268,10 +268,7 @@ __global__ __launch_bounds__(64) void proj2angl_wave_kernel(const cplx<TP>* __re
   const int q = blockIdx.x / kXcds;
   const int th = q % A::kNth;
   const long rb = static_cast<long>(q / A::kNth) * kXcds + blockIdx.x % kXcds;
-  if (rb >= nrb) {  // padding block of the XCD ordering
-    if (partials && lane == 0) partials[blockIdx.x] = 0.0;
-    return;
-  }
+  if (rb >= nrb) return;  // padding block of the XCD ordering
   const long r0 = rb * kWaveRows;
   const long r = min(r0 + lane, N - 1);
   const TC* w = tb.wts + th * NP;  // d f_m of this theta
@@ -386,7 +383,9 @@ __global__ __launch_bounds__(64) void proj2angl_wave_kernel(const cplx<TP>* __re
   if (xi != nullptr) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) acc_ff += __shfl_xor(acc_ff, off);
-    if (lane == 0) partials[blockIdx.x] = acc_ff;
+    // indexed by the data (row block, theta), not by the launch order, so
+    // that the partials of consecutive fields of a batch are contiguous
+    if (lane == 0) partials[rb * A::kNth + th] = acc_ff;
   }
 }
 
@@ -421,8 +420,12 @@ __global__ __launch_bounds__(kThreads) void delta_f_kernel(const T* __restrict__
   block_sum_store<double>(acc, partials + blockIdx.x);
 }
 
+// block j: out[j out_stride] = scale * sum of partials[j count .. (j + 1) count)
 __global__ __launch_bounds__(kThreads) void reduce_partials_kernel(const double* __restrict__ partials, int count,
-                                                                   double scale, double* __restrict__ out) {
+                                                                   double scale, double* __restrict__ out,
+                                                                   long out_stride) {
+  partials += static_cast<long>(blockIdx.x) * count;
+  out += blockIdx.x * out_stride;
   double acc = 0.0;
   for (int i = threadIdx.x; i < count; i += kThreads) acc += partials[i];
   __shared__ double red[kThreads];
@@ -599,7 +602,7 @@ long wave_blocks(const ProjTables& t, long N) {
 }  // namespace
 
 int proj2angl_blocks(const ProjTables& t, long N) {
-  if (dispatch_angular(t.mmax, t.mrso, [](auto, auto) {})) return static_cast<int>(wave_blocks(t, N));
+  if (dispatch_angular(t.mmax, t.mrso, [](auto, auto) {})) return static_cast<int>(wave_row_blocks(N) * t.ntheta);
   return static_cast<int>((N + kRT - 1) / kRT);
 }
 
@@ -648,8 +651,10 @@ int delta_f(const T* xi, const T* vexc, const double* w, T* df, long nxyz, int n
   return nb;
 }
 
-void reduce_partials(const double* partials, int count, double scale, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(kThreads), 0, s, partials, count, scale, out);
+void reduce_partials(const double* partials, int count, double scale, double* out, hipStream_t s, int nbatch,
+                     long out_stride) {
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(nbatch), dim3(kThreads), 0, s, partials, count, scale, out,
+                     out_stride);
   MDFT_HIP_CHECK(hipGetLastError());
 }
 

@@ -42,6 +42,9 @@ void angl2proj_factorized(const ProjTables& t, long N, const TI* o, cplx<TP>* p,
 template <typename TP, typename TO, typename TC>
 void proj2angl_factorized(const ProjTables& t, long N, const cplx<TP>* p, TO* o, const TO* xi, TO* df,
                           double rho0, double* partials, hipStream_t s);
+// partial sums written by the fused proj2angl over N points (one per row
+// block, or per (row block, theta)); for N a multiple of 64 the partials of
+// point ranges [j N, (j + 1) N) of a batch are contiguous
 int proj2angl_blocks(const ProjTables& t, long N);
 
 template <typename T>
@@ -53,8 +56,11 @@ int delta_f(const T* xi, const T* vexc, const double* w, T* df, long nxyz, int n
             int max_partials, hipStream_t s);
 int delta_f_blocks(long nxyz);
 
-// ff = scale * sum(partials[0..count)), summed in a fixed order, into out[0]
-void reduce_partials(const double* partials, int count, double scale, double* out, hipStream_t s);
+// ff = scale * sum(partials[0..count)), summed in a fixed order, into out[0];
+// with nbatch > 1, field j's count partials follow field j - 1's and its sum
+// goes to out[j out_stride]
+void reduce_partials(const double* partials, int count, double scale, double* out, hipStream_t s, int nbatch = 1,
+                     long out_stride = 1);
 
 template <typename T>
 void fill_uniform(T* x, size_t n, unsigned long long seed, unsigned long long offset, double lo, double hi,

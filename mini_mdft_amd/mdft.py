@@ -288,11 +288,11 @@ class Solver:
     """One c-proj functional + gradient evaluation (src/mini-mdft.hpp:137-193)
     with delta_rho fused into angl2proj and get_delta_f into proj2angl."""
 
-    def __init__(self, conv: Convolution, rho0: float, prefactor: float, precision: int = FP64):
+    def __init__(self, conv: Convolution, rho0: float, prefactor: float, precision: int = FP64, capacity: int = 1):
         h = ctypes.c_void_p()
-        check(lib().mdft_solver_create(conv._h, ctypes.c_double(rho0), ctypes.c_double(prefactor), precision,
-                                       ctypes.byref(h)))
-        self._h, self.conv, self.precision = h, conv, precision
+        check(lib().mdft_solver_create_batch(conv._h, ctypes.c_double(rho0), ctypes.c_double(prefactor), precision,
+                                             capacity, ctypes.byref(h)))
+        self._h, self.conv, self.precision, self.capacity = h, conv, precision, capacity
         self._destroy = lib().mdft_solver_destroy
         self.rho0, self.prefactor = rho0, prefactor
         self.real_dtype = _dtypes(precision)[0]
@@ -314,6 +314,17 @@ class Solver:
         ff = ctypes.c_double()
         check(lib().mdft_energy_cproj(self._h, ptr(xi), ptr(df), ctypes.byref(ff), stream_handle(stream)))
         return ff.value
+
+    def energy_cproj_batch(self, xi, df, stream=None):
+        """Evaluate the nbatch fields of xi (nbatch, N, no) in one pass of every
+        stage (capacity >= nbatch); returns the nbatch ff values, df in place."""
+        n = self.conv.sg.N * self.conv.ag.m_no
+        nb = xi.shape[0]
+        _require(xi, self.real_dtype, nb * n, "xi")
+        _require(df, self.real_dtype, nb * n, "df")
+        ff = (ctypes.c_double * nb)()
+        check(lib().mdft_energy_cproj_batch(self._h, nb, ptr(xi), ptr(df), ff, stream_handle(stream)))
+        return list(ff)
 
     STAGES = ("angl2proj", "fft_forward", "oz", "fft_backward", "proj2angl_delta_f")
 

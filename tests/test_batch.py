@@ -1,0 +1,57 @@
+"""Same-shape batches: a solver of capacity B evaluates B fields in one pass of
+every stage, and each field's ff and df are bitwise the ones it gets alone."""
+import numpy as np
+import pytest
+
+from mini_mdft_amd import FP32, FP64, MIXED, MIXED_X6, luc
+from mini_mdft_amd import mdft as M
+
+RHO0, PREF = M.rho0_of(0.0332891, 2), M.prefactor_of(298.15, 2, 0.0332891)
+
+
+def _conv(n, L, mmax, prec):
+    sg, ag = M.SpatialGrid(n, L), M.AngularGrid(mmax, 2)
+    tab = luc.synthetic_c(mmax, 2, luc.nq_for(luc.qmax_of_grid(n, L), luc.DQ_TIP3P), seed=3)
+    return sg, ag, M.Convolution.from_table(tab, sg, ag, M.OrientationProjectionMap(ag),
+                                            FP64 if prec == FP64 else FP32)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [FP64, MIXED_X6, MIXED, FP32])
+@pytest.mark.parametrize("n,L,mmax", [((8, 8, 8), (9.0,) * 3, 3), ((16, 8, 4), (12.0, 7.0, 5.0), 5),
+                                      ((12, 12, 12), (10.0,) * 3, 2)])
+def test_batch_equals_alone_bitwise(prec, n, L, mmax):
+    import torch
+
+    sg, ag, conv = _conv(n, L, mmax, prec)
+    dt = torch.float32 if prec == FP32 else torch.float64
+    nb, shape = 3, (sg.N, ag.m_no)
+    xi = torch.stack([torch.from_numpy(1.0 + 0.1 * np.random.default_rng(s).uniform(-1, 1, shape)) for s in range(nb)])
+    xi = xi.to(dt).cuda().contiguous()
+    batch = M.Solver(conv, RHO0, PREF, prec, capacity=4)
+    df_b = torch.empty_like(xi)
+    ff_b = batch.energy_cproj_batch(xi, df_b)
+    alone = M.Solver(conv, RHO0, PREF, prec)
+    for j in range(nb):
+        df = torch.empty_like(xi[j])
+        ff = alone.energy_cproj(xi[j].contiguous(), df)
+        assert ff == ff_b[j], (j, ff, ff_b[j])
+        assert torch.equal(df, df_b[j])
+        # the batch solver on one field gives the same again
+        df1 = torch.empty_like(xi[j])
+        assert batch.energy_cproj(xi[j].contiguous(), df1) == ff
+        assert torch.equal(df1, df)
+
+
+@pytest.mark.gpu
+def test_batch_errors():
+    import torch
+
+    sg, ag, conv = _conv((9, 9, 9), (10.0,) * 3, 2, FP64)
+    with pytest.raises(M.MDFTError, match="multiple of 64"):
+        M.Solver(conv, RHO0, PREF, FP64, capacity=2)
+    sg, ag, conv = _conv((8, 8, 8), (10.0,) * 3, 2, FP64)
+    s = M.Solver(conv, RHO0, PREF, FP64, capacity=2)
+    xi = torch.ones((3, sg.N, ag.m_no), dtype=torch.float64, device="cuda")
+    with pytest.raises(M.MDFTError, match="capacity"):
+        s.energy_cproj_batch(xi, torch.empty_like(xi))
