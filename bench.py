@@ -47,6 +47,8 @@ TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "oz_pair_64c_mmax5_traffic.md")
 # capped at 100 kT, L-BFGS with 5 pairs to max |h0 g| < 1e-5
 SOLVE_SITES, SOLVE_QSCALE, SOLVE_VMAX_KT = 20, 0.1, 100.0
 SOLVE_M, SOLVE_GTOL, SOLVE_MAX_ITER = 5, 1e-5, 400
+# the mixed-size batch: slots per shape and the device-memory budget
+SERVE_CAPACITY, SERVE_BUDGET = 4, 200e9
 
 
 def stage_bytes(N, no, np_, form):
@@ -151,6 +153,64 @@ def solve_leg(g, ag, mp, rho0, pref, nsolves):
     return out
 
 
+def serve_leg(nsolutes, seed=7):
+    """The mixed-size batch: nsolutes solutes with grids drawn from
+    {32, 48, 64, 80}^3 (box = 0.47 A x grid side), mmax from {3, 5} and 5-40
+    sites, all submitted at once to one MDFT_MIXED_X6 server (engines of
+    SERVE_CAPACITY slots per shape, HBM budget SERVE_BUDGET); each solve runs to
+    max |h0 g| < 1e-5.  Latency is submit -> converged."""
+    import numpy as np
+    import torch
+
+    from mini_mdft_amd import MIXED_X6, luc, solute
+    from mini_mdft_amd import mdft as M
+
+    kT = M.kT(TEMP)
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(nsolutes):
+        side = int(rng.choice([32, 48, 64, 80]))
+        mm = int(rng.choice([3, 5]))
+        ns = int(rng.integers(5, 41))
+        box = 0.47 * side
+        reqs.append((side, box, mm, solute.synthetic_solute(ns, 1000 + i, (box,) * 3, charge_scale=SOLVE_QSCALE)))
+    nq = max(luc.nq_for(luc.qmax_of_grid((r[0],) * 3, (r[1],) * 3), luc.DQ_TIP3P) for r in reqs) + 4
+    tab = luc.model_c_symmetric(5, MRSO, nq, seed=C_SEED)
+    srv = M.Server(tab, MRSO, N0, TEMP, MIXED_X6, capacity=SERVE_CAPACITY, m=SOLVE_M, max_iter=SOLVE_MAX_ITER,
+                   gtol=SOLVE_GTOL, budget_bytes=SERVE_BUDGET)
+    water = solute.SOLVENT_SITES["tip3p"]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ids = {srv.submit((side,) * 3, (box,) * 3, mm, sol, water, SOLVE_VMAX_KT * kT): (side, mm)
+           for side, box, mm, sol in reqs}
+    done, peak_bytes, peak_active, ticks = {}, 0.0, 0, 0
+    while True:
+        a, p = srv.step()
+        ticks += 1
+        inf = srv.info()
+        peak_bytes, peak_active = max(peak_bytes, inf["bytes_in_use"]), max(peak_active, a)
+        while (r := srv.poll()) is not None:
+            done[r["id"]] = r
+        if a == 0 and p == 0:
+            break
+        if ticks % 200 == 0:
+            print(f"[bench] serve: {len(done)}/{nsolutes} done, {a} active, {p} waiting", file=sys.stderr, flush=True)
+    makespan = time.perf_counter() - t0
+    lat = [r["total_s"] for r in done.values()]
+    shapes = {}
+    for i, (side, mm) in ids.items():
+        shapes.setdefault(f"{side}^3/mmax{mm}", 0)
+        shapes[f"{side}^3/mmax{mm}"] += 1
+    return {"solutes": nsolutes, "form": "MDFT_MIXED_X6", "capacity_per_shape": SERVE_CAPACITY,
+            "hbm_budget_GB": SERVE_BUDGET / 1e9, "makespan_s": round(makespan, 3),
+            "solutes_per_hour": round(3600.0 * nsolutes / makespan, 1),
+            "p50_s": round(percentile(lat, 50), 4), "p99_s": round(percentile(lat, 99), 4),
+            "converged": sum(r["status"] == 0 for r in done.values()),
+            "peak_active": peak_active, "peak_GB": round(peak_bytes / 1e9, 2), "ticks": ticks, "shapes": shapes,
+            "workload": "grids {32,48,64,80}^3 (box 0.47 A x side), mmax {3,5}, 5-40 sites (charges x0.1), "
+                        "model c(q), all submitted at t=0, latency submit -> max|h0 g|<1e-5"}
+
+
 def traffic_from_summary():
     """HBM bytes per OZ launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary (None if absent)."""
@@ -169,6 +229,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-evals", type=int, default=2, help="timed CPU-baseline evaluations (0: skip)")
     ap.add_argument("--solves", type=int, default=8, help="seeded solutes of the p99_solve leg (0: skip)")
+    ap.add_argument("--serve", type=int, default=64, help="solutes of the mixed-size batch leg (0: skip)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -257,6 +318,11 @@ def main():
                "max_abs_dF_kJmol": max(abs(a["F"] - b["F"]) for a, b in zip(mx, f64s))}
         p99["p99_speedup_vs_fp64"] = round(p99["fp64"]["p99"] / p99["mixed_x6"]["p99"], 3)
 
+    served = None
+    if args.serve > 0:
+        served = serve_leg(args.serve)
+        print(f"[bench] serve: {served}", file=sys.stderr, flush=True)
+
     cpu = None
     if args.cpu_evals > 0:
         print("[bench] CPU baseline (FP64 oracle) ...", file=sys.stderr, flush=True)
@@ -289,6 +355,7 @@ def main():
         "stages_ms_GBs": stages,
         "stages_fp64_ms": {k: round(v, 5) for k, v in f64["stages"].items()},
         "p99_solve": p99,
+        "served_batch": served,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

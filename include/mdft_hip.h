@@ -241,6 +241,45 @@ int mdft_lbfgs_minimize(mdft_functional_t h, double* xi, int m, int max_iter, do
                         int* iterations, int* evaluations, int* status, double* residual, double* history,
                         void* stream);
 
+/* ---- batched and served minimisation (no counterpart in the reference) ----
+ * A server minimises many solutes (each: V_ext from its sites, then L-BFGS
+ * from xi = 1 as mdft_lbfgs_minimize does) under one HBM budget.  Solutes of
+ * one shape (grid, box, mmax) share an engine of `capacity` slots that
+ * advances all of them together: every tick gives each slot one trial point
+ * and evaluates the functional of all slots in one batched pass of every
+ * stage, the tick's kernels replayed as one HIP graph.  Requests wait in a
+ * queue and are admitted first-fit in arrival order while their footprint
+ * (the solute's fields, plus an engine's tables and slabs for a new shape)
+ * fits the budget; a request that could never fit ends with status 3.  Each
+ * solute's result is bitwise the one it gets alone (its arithmetic does not
+ * depend on the other slots).
+ *   create: the c(q) table (tuples, values as mdft_luc_read returns them) for
+ *     every mmax that will be requested, the solvent's mrso and n0 (A^-3),
+ *     the temperature (K), the precision form (MDFT_FP64 or a mixed form),
+ *     the slots per engine, the L-BFGS m, max_iter, gtol and the budget (B);
+ *   submit: one solute (host arrays as mdft_vext takes them, copied); out_xi
+ *     (device, N no doubles, may be NULL) receives the final xi; returns its id;
+ *   step: admits what fits, then one tick of every engine (engines of
+ *     different shapes run concurrently on their own streams); synchronises;
+ *   poll: one finished solute per call (*got = 0 when none): F[4], the
+ *     iterations and evaluations, status (0 converged, 1 max_iter, 2 no
+ *     acceptable step, 3 rejected by the budget), max |h0 g|, and the seconds
+ *     from submit to admission and to the end;
+ *   use_graphs: graph replay on (default) or off, before the first step. */
+typedef struct mdft_server_s* mdft_server_t;
+int mdft_server_create(int np_luc, const int* tuples, const double* c, int nq_table, double dq, int mrso, double n0,
+                       double temperature, int precision, int capacity, int m, int max_iter, double gtol,
+                       double hbm_budget_bytes, mdft_server_t* out);
+int mdft_server_destroy(mdft_server_t h);
+int mdft_server_use_graphs(mdft_server_t h, int on);
+int mdft_server_submit(mdft_server_t h, int nx, int ny, int nz, double lx, double ly, double lz, int mmax,
+                       int nsolute, const double* solute, int nsolvent, const double* solvent, double vmax,
+                       double* out_xi, long long* id);
+int mdft_server_step(mdft_server_t h, int* active, int* pending);
+int mdft_server_poll(mdft_server_t h, long long* id, double* F, int* iterations, int* evaluations, int* status,
+                     double* residual, double* queued_s, double* total_s, int* got);
+int mdft_server_info(mdft_server_t h, double* bytes_in_use, int* engines, int* active, int* pending);
+
 #ifdef __cplusplus
 }
 #endif

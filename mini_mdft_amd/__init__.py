@@ -11,6 +11,7 @@ from .mdft import (  # noqa: F401
     OrientationProjectionMap,
     OrientationProjectionTransform,
     SOLVENTS,
+    Server,
     Solver,
     SpatialGrid,
     fft3d,

@@ -69,6 +69,14 @@ _SIGS = {
     "mdft_functional_destroy": (_i, [_vp]),
     "mdft_functional_eval": (_i, [_vp, _vp, _vp, _pd, _vp]),
     "mdft_lbfgs_minimize": (_i, [_vp, _vp, _i, _i, _d, _pd, _pi, _pi, _pi, _pd, _pd, _vp]),
+    "mdft_server_create": (_i, [_i, _pi, _pd, _i, _d, _i, _d, _d, _i, _i, _i, _i, _d, _d, ctypes.POINTER(_vp)]),
+    "mdft_server_destroy": (_i, [_vp]),
+    "mdft_server_use_graphs": (_i, [_vp, _i]),
+    "mdft_server_submit": (_i, [_vp, _i, _i, _i, _d, _d, _d, _i, _i, _pd, _i, _pd, _d, _vp,
+                                ctypes.POINTER(ctypes.c_longlong)]),
+    "mdft_server_step": (_i, [_vp, _pi, _pi]),
+    "mdft_server_poll": (_i, [_vp, ctypes.POINTER(ctypes.c_longlong), _pd, _pi, _pi, _pi, _pd, _pd, _pd, _pi]),
+    "mdft_server_info": (_i, [_vp, _pd, _pi, _pi, _pi]),
 }
 
 

@@ -10,6 +10,7 @@
 
 #include "../../include/mdft_hip.h"
 #include "mdft_conv.hpp"
+#include "mdft_engine.hpp"
 #include "mdft_functional.hpp"
 #include "mdft_gemm.hpp"
 #include "mdft_proj.hpp"
@@ -595,6 +596,179 @@ int mdft_lbfgs_minimize(mdft_functional_t h, double* xi, int m, int max_iter, do
     if (status) *status = r.status;
     if (residual) *residual = r.residual;
     if (history) std::memcpy(history, r.history.data(), r.history.size() * sizeof(double));
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batched and served minimisation
+namespace {
+
+// the convolution and solver of one engine's shape; the engine's excess
+// callback holds them for as long as the engine lives
+struct ShapeHold {
+  std::unique_ptr<mdft_conv_s> conv;
+  mdft_solver_t solver = nullptr;
+  ~ShapeHold() { delete solver; }
+};
+
+constexpr double kBoltz = 1.38064852e-23, kNavo = 6.02214129e23, kPiC = 3.14159265358979323846;
+
+}  // namespace
+
+struct mdft_server_s {
+  LucTable luc;
+  int mrso = 2, precision = MDFT_FP64, capacity = 1;
+  double kT = 0, rho0 = 0, prefactor = 0;
+  LbfgsOptions opt;
+  bool graphs = true;
+  std::unique_ptr<Server> server;
+
+  // device bytes of one solute, and an estimate of an engine's tables,
+  // plans and slabs (d table, c(q), projections, trial and df slabs)
+  std::pair<size_t, size_t> footprint(const Server::Shape& sh) const {
+    const AngularGrid ag = make_angular_grid(sh.mmax, mrso);
+    const size_t N = static_cast<size_t>(sh.nx) * sh.ny * sh.nz, n = N * ag.no;
+    const size_t ts = precision == MDFT_FP64 ? 8 : 4;
+    const size_t job = 4 * n * 8 + 2 * static_cast<size_t>(opt.m + 1) * n * ts;
+    const size_t proj = N * ag.np * capacity * (precision == MDFT_FP64 ? 16 : 8);
+    const size_t dtab = (N / 2 + 8) * static_cast<size_t>((sh.mmax + 1) * (sh.mmax + 1)) * (2 * sh.mmax + 1) * ts;
+    const size_t slabs = 2 * static_cast<size_t>(capacity) * n * 8 + n * ts;
+    return {job, proj + dtab + slabs + (size_t(64) << 20)};
+  }
+};
+
+extern "C" {
+
+int mdft_server_create(int np_luc, const int* tuples, const double* c, int nq_table, double dq, int mrso, double n0,
+                       double temperature, int precision, int capacity, int m, int max_iter, double gtol,
+                       double hbm_budget_bytes, mdft_server_t* out) {
+  return guarded([&] {
+    throw_if(!out || !tuples || !c, "server_create: null argument");
+    *out = nullptr;
+    check_precision(precision);
+    throw_if(precision == MDFT_FP32, "server: the FP32 form has float fields; use MDFT_FP64 or a mixed form");
+    throw_if(np_luc <= 0 || nq_table <= 0 || !(dq > 0) || mrso < 1 || !(n0 > 0) || !(temperature > 0),
+             "server_create: bad arguments");
+    throw_if(capacity < 1, "server_create: capacity must be at least 1");
+    throw_if(m < 1 || m > kLbfgsMaxPairs, "lbfgs: m must be in 1 .. 8");
+    auto h = std::make_unique<mdft_server_s>();
+    h->luc.np_luc = np_luc;
+    h->luc.nq = nq_table;
+    h->luc.dq = dq;
+    h->luc.tuples.assign(tuples, tuples + static_cast<size_t>(np_luc) * 5);
+    h->luc.c.resize(static_cast<size_t>(np_luc) * nq_table);
+    std::memcpy(h->luc.c.data(), c, h->luc.c.size() * sizeof(cd));
+    h->mrso = mrso;
+    h->precision = precision;
+    h->capacity = capacity;
+    // Thermo::m_kbt and the Solvent's rho0 and prefactor
+    // (common/MDFT_Thermo.hpp:60, common/MDFT_Solvent.hpp:366-368, :430-431)
+    h->kT = kBoltz * kNavo * temperature * 1e-3;
+    h->rho0 = n0 / (8 * kPiC * kPiC / mrso);
+    h->prefactor = -2 * h->kT * 4 * kPiC * kPiC / mrso / n0;
+    h->opt.m = m;
+    h->opt.max_iter = max_iter;
+    h->opt.gtol = gtol;
+    mdft_server_s* sp = h.get();
+    auto make = [sp](const Server::Shape& sh, size_t& table_bytes) {
+      auto hold = std::make_shared<ShapeHold>();
+      // the tuples this mmax needs (m, n <= mmax), in the table's order
+      LucTable luc;
+      luc.nq = sp->luc.nq;
+      luc.dq = sp->luc.dq;
+      for (int i = 0; i < sp->luc.np_luc; ++i) {
+        const int* t = &sp->luc.tuples[static_cast<size_t>(i) * 5];
+        if (t[0] > sh.mmax || t[1] > sh.mmax) continue;
+        luc.tuples.insert(luc.tuples.end(), t, t + 5);
+        luc.c.insert(luc.c.end(), sp->luc.c.begin() + static_cast<long>(i) * luc.nq,
+                     sp->luc.c.begin() + static_cast<long>(i + 1) * luc.nq);
+        ++luc.np_luc;
+      }
+      hold->conv.reset(make_conv(sh.nx, sh.ny, sh.nz, sh.lx, sh.ly, sh.lz, sh.mmax, sp->mrso, sp->precision, luc));
+      mdft_solver_t sv = nullptr;
+      if (mdft_solver_create_batch(hold->conv.get(), sp->rho0, sp->prefactor, sp->precision, sp->capacity, &sv))
+        throw Error(g_last_error);
+      hold->solver = sv;
+      const ConvTables& t = hold->conv->dev->tab;
+      FunctionalParams p{sv->N, t.ang.no, sp->rho0, sp->prefactor, sp->kT,
+                         t.lx * t.ly * t.lz / static_cast<double>(sv->N)};
+      auto eng = std::make_unique<Engine>(
+          p, t.ang.w, sp->capacity, sp->opt, sp->precision != MDFT_FP64,
+          [hold, sv](int nb, const double* x, double* df, double* ff, long st, hipStream_t s) {
+            energy_impl(sv, nb, x, df, ff, st, s);
+          });
+      eng->use_graphs(sp->graphs);
+      table_bytes = sp->footprint(sh).second;
+      return eng;
+    };
+    auto fp = [sp](const Server::Shape& sh) { return sp->footprint(sh); };
+    h->server = std::make_unique<Server>(make, fp, hbm_budget_bytes, mrso);
+    *out = h.release();
+  });
+}
+
+int mdft_server_destroy(mdft_server_t h) {
+  return guarded([&] { delete h; });
+}
+
+int mdft_server_use_graphs(mdft_server_t h, int on) {
+  return guarded([&] {
+    throw_if(!h, "null handle");
+    throw_if(h->server->engines() > 0, "server: set the graph mode before the first step");
+    h->graphs = on != 0;
+  });
+}
+
+int mdft_server_submit(mdft_server_t h, int nx, int ny, int nz, double lx, double ly, double lz, int mmax,
+                       int nsolute, const double* solute, int nsolvent, const double* solvent, double vmax,
+                       double* out_xi, long long* id) {
+  return guarded([&] {
+    throw_if(!h || !id, "server_submit: null argument");
+    throw_if(nx <= 0 || ny <= 0 || nz <= 0 || !(lx > 0) || !(ly > 0) || !(lz > 0), "server_submit: bad grid");
+    throw_if(h->capacity > 1 && (static_cast<long>(nx) * ny * nz) % 64 != 0,
+             "batched evaluation needs nx ny nz to be a multiple of 64");
+    (void)make_angular_grid(mmax, h->mrso);  // validates mmax
+    *id = h->server->submit(Server::Shape{nx, ny, nz, lx, ly, lz, mmax}, solute, nsolute, solvent, nsolvent, vmax,
+                            out_xi);
+  });
+}
+
+int mdft_server_step(mdft_server_t h, int* active, int* pending) {
+  return guarded([&] {
+    throw_if(!h, "null handle");
+    h->server->step();
+    if (active) *active = h->server->active();
+    if (pending) *pending = h->server->pending();
+  });
+}
+
+int mdft_server_poll(mdft_server_t h, long long* id, double* F, int* iterations, int* evaluations, int* status,
+                     double* residual, double* queued_s, double* total_s, int* got) {
+  return guarded([&] {
+    throw_if(!h || !got, "server_poll: null argument");
+    Server::Result r;
+    *got = h->server->poll(r) ? 1 : 0;
+    if (!*got) return;
+    if (id) *id = r.id;
+    if (F) std::memcpy(F, r.r.F, sizeof(r.r.F));
+    if (iterations) *iterations = r.r.iterations;
+    if (evaluations) *evaluations = r.r.evaluations;
+    if (status) *status = r.r.status;
+    if (residual) *residual = r.r.residual;
+    if (queued_s) *queued_s = r.queued_s;
+    if (total_s) *total_s = r.total_s;
+  });
+}
+
+int mdft_server_info(mdft_server_t h, double* bytes_in_use, int* engines, int* active, int* pending) {
+  return guarded([&] {
+    throw_if(!h, "null handle");
+    if (bytes_in_use) *bytes_in_use = h->server->bytes_in_use();
+    if (engines) *engines = h->server->engines();
+    if (active) *active = h->server->active();
+    if (pending) *pending = h->server->pending();
   });
 }
 

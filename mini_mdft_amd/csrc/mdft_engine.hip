@@ -1,0 +1,605 @@
+// Batched and served minimisation (no counterpart in the reference, whose
+// Solver runs one field once, src/mini-mdft.hpp:129-133).
+//
+// Engine: capacity slots of one shape.  A tick gives every active slot one
+// L-BFGS trial (or its start-point evaluation):
+//   1. slot_direction_kernel: for each slot, x_t = x + t d into the slot's
+//      place in the trial slab (the start point is copied there);
+//   2. the excess evaluation of the whole slab in one batched pass;
+//   3. slot_update_kernel: for each slot, g, y and the Gram entries (or, at
+//      the start point, g and its sums);
+//   4. slot_reduce_kernel: the per-slot sums, in a fixed order.
+// Steps 1-4 are one HIP graph per batch width (captured after one eager tick
+// of that width has made every lazy table), replayed every tick.  The slots'
+// scalars (L-BFGS coefficients, step, pointers) live in a device table
+// uploaded before the replay.  The per-slot decisions are LbfgsState's, the
+// same code that drives Functional::minimize; the passes use every pair column
+// up to m, unused columns reading a shared zero field with zero coefficients,
+// so each slot's arithmetic is that of a solve alone (a slot's results do not
+// depend on what else is in the batch).
+//
+// Server: requests of any shape wait in a queue; admission is first fit in
+// arrival order against an HBM budget (a solute's fields plus, for a shape
+// with no engine yet, the engine's tables and slabs); engines of different
+// shapes tick concurrently on their own streams.
+#include "mdft_engine.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "mdft_elem.hpp"
+
+namespace mdft {
+
+double now_seconds() {
+  using clock = std::chrono::steady_clock;
+  return std::chrono::duration<double>(clock::now().time_since_epoch()).count();
+}
+
+namespace {
+
+using namespace elem;
+
+constexpr int kModeIdle = 0, kModeStart = 1, kModeTrial = 2;
+// per-slot rows of the partial sums / results: the update pass's 5 m + 10
+// rows for m = kLbfgsMaxPairs at most, then ff
+constexpr int kSlotRows = 5 * kLbfgsMaxPairs + 10;
+constexpr int kSlotRes = kSlotRows + 1;
+constexpr int kSlotFf = kSlotRows;
+
+template <typename TS>
+struct SlotDesc {
+  int mode;
+  int k;  // stored pairs in use (columns k .. KM - 1 read the zero field)
+  double cg, gamma, t;
+  double cs[kLbfgsMaxPairs], cq[kLbfgsMaxPairs];
+  const double* src;  // current point (start: the initial field)
+  double* save;       // copy of the current point, or null
+  const double* V;
+  double* g;   // g at the current point (written at the start point)
+  double* gt;  // g at the trial
+  const TS* s[kLbfgsMaxPairs];
+  const TS* y[kLbfgsMaxPairs];
+  TS* sc;
+  TS* yn;
+};
+
+// grid (nb, slots): x_t of slot blockIdx.y into xslab + slot n
+template <typename TS, int KM>
+// (xslab is not restrict: a slot's current point may be its slab field, read
+// and overwritten in place element by element)
+__global__ __launch_bounds__(kFThreads) void slot_direction_kernel(const SlotDesc<TS>* __restrict__ desc,
+                                                                   double* xslab, Elem e) {
+  const SlotDesc<TS>& a = desc[blockIdx.y];
+  if (a.mode == kModeIdle) return;
+  double* xt = xslab + static_cast<long>(blockIdx.y) * e.n;
+  if (a.mode == kModeStart) {
+    MDFT_ELEMENT_LOOP(e) { xt[i] = a.src[i]; }
+    return;
+  }
+  MDFT_ELEMENT_LOOP(e) {
+    double q = a.cg * a.g[i];
+#pragma unroll
+    for (int j = 0; j < KM; ++j) q = fma(a.cq[j], static_cast<double>(a.y[j][i]), q);
+    double r = a.gamma * e.h0[io] * q;
+#pragma unroll
+    for (int j = 0; j < KM; ++j) r = fma(a.cs[j], static_cast<double>(a.s[j][i]), r);
+    const double xv = a.src[i];
+    if (a.save) a.save[i] = xv;
+    const double xn = xv + a.t * (-r);
+    xt[i] = xn;
+    a.sc[i] = static_cast<TS>(xn - xv);
+  }
+}
+
+// grid (nb, slots): the update pass of Functional's L-BFGS for slot
+// blockIdx.y (rows as in update_kernel with K = KM); at the start point, g
+// and its sums into the same rows (5K+6 g'H g, 5K+7 F_id, 5K+8 F_ext, 5K+9 max)
+template <typename TS, int KM>
+__global__ __launch_bounds__(kFThreads) void slot_update_kernel(const SlotDesc<TS>* __restrict__ desc,
+                                                                const double* __restrict__ xslab,
+                                                                const double* __restrict__ dfslab, Elem e,
+                                                                double* __restrict__ part, int nb) {
+  constexpr int NA = 5 * KM + 10;
+  constexpr int KA = KM > 0 ? KM : 1;
+  const SlotDesc<TS>& a = desc[blockIdx.y];
+  if (a.mode == kModeIdle) return;
+  const double* xt = xslab + static_cast<long>(blockIdx.y) * e.n;
+  const double* df = dfslab + static_cast<long>(blockIdx.y) * e.n;
+  part += static_cast<long>(blockIdx.y) * kSlotRows * nb;
+  double acc[NA];
+#pragma unroll
+  for (int k = 0; k < NA; ++k) acc[k] = 0.0;
+  if (a.mode == kModeStart) {
+    MDFT_ELEMENT_LOOP(e) {
+      const double xv = xt[i], v = a.V[i], wio = e.w[io], h = e.h0[io];
+      local_f(xv, v, wio, acc[5 * KM + 7], acc[5 * KM + 8]);
+      const double gv = local_grad(xv, v, df[i], wio, e);
+      a.g[i] = gv;
+      const double hg = h * gv;
+      acc[5 * KM + 6] += gv * hg;
+      acc[5 * KM + 9] = fmax(acc[5 * KM + 9], fabs(hg));
+    }
+  } else {
+    MDFT_ELEMENT_LOOP(e) {
+      const double wio = e.w[io], h = e.h0[io];
+      const double xv = xt[i], v = a.V[i];
+      local_f(xv, v, wio, acc[5 * KM + 7], acc[5 * KM + 8]);
+      const double gn = local_grad(xv, v, df[i], wio, e);
+      const TS ys = static_cast<TS>(gn - a.g[i]);
+      a.gt[i] = gn;
+      a.yn[i] = ys;
+      const double yv = static_cast<double>(ys);
+      const double scv = static_cast<double>(a.sc[i]);
+      double sj[KA], yj[KA];
+#pragma unroll
+      for (int j = 0; j < KM; ++j) {
+        sj[j] = static_cast<double>(a.s[j][i]);
+        yj[j] = static_cast<double>(a.y[j][i]);
+      }
+      const double hy = h * yv, hg = h * gn;
+#pragma unroll
+      for (int j = 0; j < KM; ++j) {
+        acc[j] += scv * yj[j];
+        acc[KM + 1 + j] += sj[j] * yv;
+        acc[2 * KM + 3 + j] += sj[j] * gn;
+        acc[3 * KM + 4 + j] += hy * yj[j];
+        acc[4 * KM + 5 + j] += yj[j] * hg;
+      }
+      acc[KM] += scv * yv;
+      acc[2 * KM + 1] += scv * scv;
+      acc[2 * KM + 2] += yv * yv;
+      acc[3 * KM + 3] += scv * gn;
+      acc[4 * KM + 4] += hy * yv;
+      acc[5 * KM + 5] += yv * hg;
+      acc[5 * KM + 6] += gn * hg;
+      acc[5 * KM + 9] = fmax(acc[5 * KM + 9], fabs(hg));
+    }
+  }
+  store_partials<NA, true>(acc, part, nb);
+}
+
+// grid (rows, slots): res[slot kSlotRes + row] = fixed-order sum (or max) of
+// the slot's row partials
+__global__ __launch_bounds__(kFThreads) void slot_reduce_kernel(const double* __restrict__ part, int nb, int max_row,
+                                                                double* __restrict__ res) {
+  const int row = blockIdx.x;
+  const bool is_max = row == max_row;
+  part += (static_cast<long>(blockIdx.y) * kSlotRows + row) * nb;
+  double v = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kFThreads) v = is_max ? fmax(v, part[i]) : v + part[i];
+  __shared__ double red[kFThreads];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int st = kFThreads / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] = is_max ? fmax(red[threadIdx.x], red[threadIdx.x + st])
+                                                    : red[threadIdx.x] + red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) res[static_cast<long>(blockIdx.y) * kSlotRes + row] = red[0];
+}
+
+__global__ __launch_bounds__(kFThreads) void fill_kernel(double* __restrict__ x, long n, double v) {
+  for (long i = static_cast<long>(blockIdx.x) * kFThreads + threadIdx.x; i < n;
+       i += static_cast<long>(gridDim.x) * kFThreads)
+    x[i] = v;
+}
+
+void fill(double* x, long n, double v, hipStream_t s) {
+  hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n, kFThreads, 65536)), dim3(kFThreads), 0, s, x, n, v);
+  MDFT_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct Engine::Slot {
+  long long id = 0;
+  bool started = false;      // start point evaluated
+  bool cur_in_slab = false;  // the current point is the slot's slab field
+  LbfgsState st;
+  LbfgsState::Trial tr;
+  bool has_trial = false;
+  double* x = nullptr;  // current point when not in the slab
+  double* V = nullptr;
+  double* g = nullptr;
+  double* gt = nullptr;
+  void* pairs = nullptr;  // 2 (m + 1) fields of TS
+  void* mem = nullptr;    // one allocation: x | V | g | gt | pairs
+  double* out = nullptr;  // where the final point goes (device), or null
+  explicit Slot(const LbfgsOptions& o) : st(o) {}
+};
+
+Engine::Engine(const FunctionalParams& p, const std::vector<double>& w, int capacity, const LbfgsOptions& o,
+               bool fp32_pairs, ExcessBatch excess)
+    : p_(p), o_(o), excess_(std::move(excess)), capacity_(capacity), fp32_(fp32_pairs) {
+  throw_if(capacity < 1, "engine: capacity must be at least 1");
+  throw_if(o.m < 1 || o.m > kLbfgsMaxPairs, "lbfgs: m must be in 1 .. 8");
+  throw_if(o.max_iter < 0 || o.max_backtrack < 1 || !(o.gtol >= 0) || !(o.c1 > 0 && o.c1 < 0.5) ||
+               !(o.eps_f >= 0),
+           "lbfgs: bad options");
+  throw_if(p.N <= 0 || p.no <= 0 || static_cast<int>(w.size()) != p.no, "engine: bad sizes");
+  throw_if(!(p.kT > 0) || !(p.dv > 0) || !(p.rho0 > 0), "engine: kT, dv and rho0 must be positive");
+  n_ = p.N * p.no;
+  const long want = (n_ + kFThreads * 16L - 1) / (kFThreads * 16L);
+  nb_ = static_cast<int>(std::min<long>(std::max<long>(want, 1), kMaxBlocks));
+  step_ = static_cast<int>((static_cast<long>(nb_) * kFThreads) % p.no);
+  std::vector<double> tab(2 * static_cast<size_t>(p.no));
+  for (int io = 0; io < p.no; ++io) {
+    tab[io] = w[io];
+    tab[p.no + io] = 1.0 / (2 * p.rho0 * p.dv * w[io] * p.kT);
+  }
+  const size_t ts = fp32_ ? sizeof(float) : sizeof(double);
+  desc_bytes_ = static_cast<size_t>(capacity) * (fp32_ ? sizeof(SlotDesc<float>) : sizeof(SlotDesc<double>));
+  try {
+    MDFT_HIP_CHECK(hipStreamCreate(&stream_));
+    MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&tab_), tab.size() * sizeof(double)));
+    MDFT_HIP_CHECK(hipMemcpy(tab_, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&xslab_), capacity * n_ * sizeof(double)));
+    MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&dfslab_), capacity * n_ * sizeof(double)));
+    MDFT_HIP_CHECK(hipMalloc(&zeros_, n_ * ts));
+    MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&part_),
+                             static_cast<size_t>(capacity) * kSlotRows * nb_ * sizeof(double)));
+    MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&res_), static_cast<size_t>(capacity) * kSlotRes * sizeof(double)));
+    MDFT_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_res_),
+                                 static_cast<size_t>(capacity) * kSlotRes * sizeof(double)));
+    MDFT_HIP_CHECK(hipMalloc(&desc_, desc_bytes_));
+    MDFT_HIP_CHECK(hipHostMalloc(&host_desc_, desc_bytes_));
+    std::memset(host_desc_, 0, desc_bytes_);
+    // finite contents everywhere the batched passes may read
+    fill(xslab_, capacity * n_, 1.0, stream_);
+    fill(dfslab_, capacity * n_, 0.0, stream_);
+    MDFT_HIP_CHECK(hipMemsetAsync(zeros_, 0, n_ * ts, stream_));
+    MDFT_HIP_CHECK(hipStreamSynchronize(stream_));
+  } catch (...) {
+    release();
+    throw;
+  }
+  shared_bytes_ = (2 * static_cast<size_t>(capacity) * n_ + 2 * p.no) * sizeof(double) + n_ * ts +
+                  static_cast<size_t>(capacity) * (kSlotRows * nb_ + kSlotRes) * sizeof(double) + desc_bytes_;
+  for (int j = 0; j < capacity; ++j) slots_.push_back(nullptr);
+}
+
+Engine::~Engine() { release(); }
+
+void Engine::release() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto& kv : graphs_by_width_) (void)hipGraphExecDestroy(kv.second);
+  graphs_by_width_.clear();
+  for (int j = 0; j < static_cast<int>(slots_.size()); ++j)
+    if (slots_[j]) release_slot(j);
+  if (tab_) (void)hipFree(tab_);
+  if (xslab_) (void)hipFree(xslab_);
+  if (dfslab_) (void)hipFree(dfslab_);
+  if (zeros_) (void)hipFree(zeros_);
+  if (part_) (void)hipFree(part_);
+  if (res_) (void)hipFree(res_);
+  if (host_res_) (void)hipHostFree(host_res_);
+  if (desc_) (void)hipFree(desc_);
+  if (host_desc_) (void)hipHostFree(host_desc_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+  tab_ = xslab_ = dfslab_ = part_ = res_ = host_res_ = nullptr;
+  zeros_ = desc_ = host_desc_ = nullptr;
+  stream_ = nullptr;
+}
+
+int Engine::active() const {
+  int a = 0;
+  for (const auto& s : slots_) a += s != nullptr;
+  return a;
+}
+
+size_t Engine::job_bytes() const {
+  const size_t ts = fp32_ ? sizeof(float) : sizeof(double);
+  return 4 * static_cast<size_t>(n_) * sizeof(double) + 2 * static_cast<size_t>(o_.m + 1) * n_ * ts;
+}
+
+int Engine::admit(long long id, double* out_xi) {
+  int j = 0;
+  while (j < capacity_ && slots_[j]) ++j;
+  throw_if(j == capacity_, "engine: no free slot");
+  auto sl = std::make_unique<Slot>(o_);
+  sl->id = id;
+  sl->out = out_xi;
+  MDFT_HIP_CHECK(hipMalloc(&sl->mem, job_bytes()));
+  double* base = static_cast<double*>(sl->mem);
+  sl->x = base;
+  sl->V = base + n_;
+  sl->g = base + 2 * n_;
+  sl->gt = base + 3 * n_;
+  sl->pairs = base + 4 * n_;
+  fill(sl->x, n_, 1.0, stream_);  // the start point xi = 1 (Solvent::init_density)
+  slots_[j] = std::move(sl);
+  return j;
+}
+
+double* Engine::vext_of(int slot) {
+  throw_if(slot < 0 || slot >= capacity_ || !slots_[slot], "engine: no such slot");
+  return slots_[slot]->V;
+}
+
+void Engine::release_slot(int j) {
+  if (slots_[j] && slots_[j]->mem) (void)hipFree(slots_[j]->mem);
+  slots_[j].reset();
+}
+
+std::vector<Engine::Done> Engine::take_done() {
+  std::vector<Done> d;
+  d.swap(done_);
+  return d;
+}
+
+void Engine::enqueue(int nbatch) {
+  const Elem e{tab_, tab_ + p_.no, p_.rho0, p_.kT, p_.dv, 2.0 * p_.prefactor / p_.N, n_, p_.no, step_};
+  const int KM = o_.m, NA = 5 * KM + 10;
+  dispatch_k(KM, [&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    if (fp32_) {
+      hipLaunchKernelGGL((slot_direction_kernel<float, K>), dim3(nb_, nbatch), dim3(kFThreads), 0, stream_,
+                         static_cast<const SlotDesc<float>*>(desc_), xslab_, e);
+    } else {
+      hipLaunchKernelGGL((slot_direction_kernel<double, K>), dim3(nb_, nbatch), dim3(kFThreads), 0, stream_,
+                         static_cast<const SlotDesc<double>*>(desc_), xslab_, e);
+    }
+  });
+  MDFT_HIP_CHECK(hipGetLastError());
+  excess_(nbatch, xslab_, dfslab_, res_ + kSlotFf, kSlotRes, stream_);
+  dispatch_k(KM, [&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    if (fp32_) {
+      hipLaunchKernelGGL((slot_update_kernel<float, K>), dim3(nb_, nbatch), dim3(kFThreads), 0, stream_,
+                         static_cast<const SlotDesc<float>*>(desc_), xslab_, dfslab_, e, part_, nb_);
+    } else {
+      hipLaunchKernelGGL((slot_update_kernel<double, K>), dim3(nb_, nbatch), dim3(kFThreads), 0, stream_,
+                         static_cast<const SlotDesc<double>*>(desc_), xslab_, dfslab_, e, part_, nb_);
+    }
+  });
+  MDFT_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(slot_reduce_kernel, dim3(NA, nbatch), dim3(kFThreads), 0, stream_, part_, nb_, 5 * KM + 9,
+                     res_);
+  MDFT_HIP_CHECK(hipGetLastError());
+}
+
+namespace {
+template <typename TS>
+void fill_desc(SlotDesc<TS>& d, int mode, const LbfgsState::Trial* tr, const std::vector<int>* order,
+               const double* src, double* save, const double* V, double* g, double* gt, TS* pairs, long n,
+               const TS* zeros, int m) {
+  d = SlotDesc<TS>();
+  d.mode = mode;
+  d.src = src;
+  d.save = save;
+  d.V = V;
+  d.g = g;
+  d.gt = gt;
+  for (int j = 0; j < m; ++j) {
+    d.s[j] = zeros;
+    d.y[j] = zeros;
+  }
+  if (!tr) return;
+  d.k = tr->k;
+  d.cg = tr->cg;
+  d.gamma = tr->gamma;
+  d.t = tr->t;
+  for (int j = 0; j < tr->k; ++j) {
+    d.cs[j] = tr->cs[j];
+    d.cq[j] = tr->cq[j];
+    d.s[j] = pairs + 2 * static_cast<long>((*order)[j]) * n;
+    d.y[j] = pairs + (2 * static_cast<long>((*order)[j]) + 1) * n;
+  }
+  d.sc = pairs + 2 * static_cast<long>(tr->cand) * n;
+  d.yn = pairs + (2 * static_cast<long>(tr->cand) + 1) * n;
+}
+}  // namespace
+
+int Engine::tick() {
+  begin_tick();
+  return end_tick();
+}
+
+void Engine::begin_tick() {
+  int hi = 0;
+  for (int j = 0; j < capacity_; ++j)
+    if (slots_[j]) hi = j + 1;
+  width_ = hi;
+  if (hi == 0) return;
+  // the slots' descriptors
+  for (int j = 0; j < hi; ++j) {
+    Slot* sl = slots_[j].get();
+    const int mode = !sl ? kModeIdle : !sl->started ? kModeStart : kModeTrial;
+    const double* src = nullptr;
+    double* save = nullptr;
+    if (mode == kModeStart) src = sl->x;
+    if (mode == kModeTrial) {
+      src = sl->cur_in_slab ? xslab_ + j * n_ : sl->x;
+      save = sl->cur_in_slab ? sl->x : nullptr;
+      sl->cur_in_slab = false;  // the slab now receives the trial; x holds the current point
+    }
+    const LbfgsState::Trial* tr = mode == kModeTrial ? &sl->tr : nullptr;
+    const std::vector<int>* order = sl ? &sl->st.order() : nullptr;
+    if (fp32_)
+      fill_desc(static_cast<SlotDesc<float>*>(host_desc_)[j], mode, tr, order, src, save, sl ? sl->V : nullptr,
+                sl ? sl->g : nullptr, sl ? sl->gt : nullptr, sl ? static_cast<float*>(sl->pairs) : nullptr, n_,
+                static_cast<const float*>(zeros_), o_.m);
+    else
+      fill_desc(static_cast<SlotDesc<double>*>(host_desc_)[j], mode, tr, order, src, save, sl ? sl->V : nullptr,
+                sl ? sl->g : nullptr, sl ? sl->gt : nullptr, sl ? static_cast<double*>(sl->pairs) : nullptr, n_,
+                static_cast<const double*>(zeros_), o_.m);
+  }
+  const size_t dsz = fp32_ ? sizeof(SlotDesc<float>) : sizeof(SlotDesc<double>);
+  MDFT_HIP_CHECK(hipMemcpyAsync(desc_, host_desc_, hi * dsz, hipMemcpyHostToDevice, stream_));
+  // the passes: eager until this width has run once, then its graph
+  auto g = graphs_by_width_.find(hi);
+  if (graphs_ && g != graphs_by_width_.end()) {
+    MDFT_HIP_CHECK(hipGraphLaunch(g->second, stream_));
+  } else if (graphs_ && eager_runs_[hi] >= 1) {
+    hipGraph_t graph = nullptr;
+    MDFT_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue(hi);
+    } catch (...) {
+      (void)hipStreamEndCapture(stream_, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    MDFT_HIP_CHECK(hipStreamEndCapture(stream_, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    MDFT_HIP_CHECK(err);
+    graphs_by_width_[hi] = exec;
+    MDFT_HIP_CHECK(hipGraphLaunch(exec, stream_));
+  } else {
+    enqueue(hi);
+    ++eager_runs_[hi];
+  }
+  MDFT_HIP_CHECK(hipMemcpyAsync(host_res_, res_, static_cast<size_t>(hi) * kSlotRes * sizeof(double),
+                                hipMemcpyDeviceToHost, stream_));
+}
+
+int Engine::end_tick() {
+  const int hi = width_;
+  width_ = 0;
+  if (hi == 0) return 0;
+  MDFT_HIP_CHECK(hipStreamSynchronize(stream_));
+
+  // the decisions
+  const int KM = o_.m;
+  int finished = 0;
+  for (int j = 0; j < hi; ++j) {
+    Slot* sl = slots_[j].get();
+    if (!sl) continue;
+    const double* h = host_res_ + static_cast<long>(j) * kSlotRes;
+    double F[4];
+    F[1] = p_.kT * p_.rho0 * p_.dv * h[5 * KM + 7];
+    F[2] = p_.rho0 * p_.dv * h[5 * KM + 8];
+    F[3] = p_.dv / p_.N * h[kSlotFf];
+    F[0] = F[1] + F[2] + F[3];
+    if (!sl->started) {
+      sl->st.start(F, h[5 * KM + 6], h[5 * KM + 9]);
+      sl->started = true;
+    } else if (sl->st.judge(F, h, KM)) {
+      std::swap(sl->g, sl->gt);
+      sl->cur_in_slab = true;
+    }
+    sl->has_trial = sl->st.next(sl->tr);
+    if (!sl->has_trial) {
+      if (sl->out) {
+        MDFT_HIP_CHECK(hipMemcpyAsync(sl->out, sl->cur_in_slab ? xslab_ + j * n_ : sl->x, n_ * sizeof(double),
+                                      hipMemcpyDeviceToDevice, stream_));
+        MDFT_HIP_CHECK(hipStreamSynchronize(stream_));
+      }
+      done_.push_back(Done{sl->id, sl->st.result()});
+      release_slot(j);
+      ++finished;
+    }
+  }
+  return finished;
+}
+
+// ---------------------------------------------------------------------------
+Server::Server(EngineFactory make, Footprint footprint, double budget_bytes, int mrso)
+    : make_(std::move(make)), footprint_(std::move(footprint)), budget_(budget_bytes), mrso_(mrso) {
+  throw_if(!(budget_bytes > 0), "server: the HBM budget must be positive");
+}
+
+long long Server::submit(const Shape& shape, const double* solute, int nsolute, const double* solvent, int nsolvent,
+                         double vmax, double* out_xi) {
+  throw_if(nsolute < 0 || nsolvent <= 0 || (nsolute > 0 && !solute) || !solvent, "server: bad sites");
+  Request r;
+  r.id = next_id_++;
+  r.shape = shape;
+  r.solute.assign(solute, solute + 6 * nsolute);
+  r.solvent.assign(solvent, solvent + 6 * nsolvent);
+  r.vmax = vmax;
+  r.out_xi = out_xi;
+  r.t_submit = now_seconds();
+  queue_.push_back(std::move(r));
+  return queue_.back().id;
+}
+
+int Server::active() const { return static_cast<int>(running_.size()); }
+
+void Server::admit() {
+  for (auto it = queue_.begin(); it != queue_.end();) {
+    const Request& rq = *it;
+    auto fp = footprint_(rq.shape);
+    auto e = engines_.find(rq.shape);
+    const bool have = e != engines_.end();
+    const double need = static_cast<double>(fp.first) + (have ? 0.0 : static_cast<double>(fp.second));
+    if (static_cast<double>(fp.first + fp.second) > budget_) {  // can never fit: reject
+      Result res{rq.id, LbfgsResult(), now_seconds() - rq.t_submit, now_seconds() - rq.t_submit, true};
+      res.r.status = 3;
+      results_.push_back(res);
+      it = queue_.erase(it);
+      continue;
+    }
+    if (have && !e->second.engine->has_free_slot()) {
+      ++it;
+      continue;
+    }
+    if (in_use_ + need > budget_) {
+      // drop idle engines of other shapes to make room
+      for (auto x = engines_.begin(); x != engines_.end() && in_use_ + need > budget_;) {
+        if (x->second.engine->active() == 0 && x != e) {
+          in_use_ -= static_cast<double>(x->second.table_bytes);
+          x = engines_.erase(x);
+        } else {
+          ++x;
+        }
+      }
+      e = engines_.find(rq.shape);
+      if (in_use_ + need > budget_) {
+        ++it;
+        continue;
+      }
+    }
+    if (e == engines_.end()) {
+      Live lv;
+      lv.engine = make_(rq.shape, lv.table_bytes);
+      in_use_ += static_cast<double>(lv.table_bytes);
+      e = engines_.emplace(rq.shape, std::move(lv)).first;
+    }
+    Engine& eng = *e->second.engine;
+    const int slot = eng.admit(rq.id, rq.out_xi);
+    in_use_ += static_cast<double>(eng.job_bytes());
+    e->second.admitted_at[rq.id] = now_seconds();
+    running_[rq.id] = rq;
+    // V_ext of the solute into the slot, on the engine's stream
+    compute_vext(rq.shape.nx, rq.shape.ny, rq.shape.nz, rq.shape.lx, rq.shape.ly, rq.shape.lz,
+                 make_angular_grid(rq.shape.mmax, mrso_), rq.solute.data(), static_cast<int>(rq.solute.size() / 6),
+                 rq.solvent.data(), static_cast<int>(rq.solvent.size() / 6), rq.vmax, eng.vext_of(slot),
+                 eng.stream());
+    it = queue_.erase(it);
+  }
+}
+
+void Server::step() {
+  admit();
+  // every engine's tick in flight on its own stream, then their decisions
+  for (auto& kv : engines_)
+    if (kv.second.engine->active() > 0) kv.second.engine->begin_tick();
+  for (auto& kv : engines_) {
+    Live& lv = kv.second;
+    lv.engine->end_tick();
+    for (auto& d : lv.engine->take_done()) {
+      const Request& rq = running_[d.id];
+      const double t = now_seconds();
+      results_.push_back(Result{d.id, d.r, lv.admitted_at[d.id] - rq.t_submit, t - rq.t_submit, false});
+      lv.admitted_at.erase(d.id);
+      running_.erase(d.id);
+      in_use_ -= static_cast<double>(lv.engine->job_bytes());
+    }
+  }
+}
+
+bool Server::poll(Result& out) {
+  if (results_.empty()) return false;
+  out = results_.front();
+  results_.pop_front();
+  return true;
+}
+
+}  // namespace mdft

@@ -26,6 +26,7 @@
 // and all new Gram entries at once.  All sums are FP64 with fixed-order
 // reductions.
 #include "mdft_functional.hpp"
+#include "mdft_elem.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -35,67 +36,7 @@
 namespace mdft {
 namespace {
 
-constexpr int kFThreads = 256;
-constexpr int kFWaves = kFThreads / 64;
-constexpr int kMaxBlocks = 2048;
-
-// Constants the element passes share.
-struct Elem {
-  const double* w;   // no
-  const double* h0;  // no
-  double rho0, kT, dv, gscale;  // gscale = 2 prefactor / N
-  long n;                       // N * no
-  int no, step;                 // step = (gridDim * threads) mod no
-};
-
-__device__ __forceinline__ double local_grad(double x, double v, double df, double wio, const Elem& e) {
-  const double x2 = fmax(x * x, 1e-300);
-  return (e.dv * wio) * (2.0 * e.rho0 * x * (e.kT * log(x2) + v) + e.gscale * df);
-}
-
-// accumulates w (x2 ln x2 - x2 + 1) and w x2 V
-__device__ __forceinline__ void local_f(double x, double v, double wio, double& fid, double& fext) {
-  const double x2 = fmax(x * x, 1e-300);
-  fid += wio * (x2 * log(x2) - x2 + 1.0);
-  fext += wio * x2 * v;
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-  return v;
-}
-
-// One partial per block and accumulator, accumulator k at part[k * nb + block];
-// with LAST_MAX the last accumulator is a maximum.
-template <int NA, bool LAST_MAX>
-__device__ __forceinline__ void store_partials(const double (&acc)[NA], double* __restrict__ part, int nb) {
-  __shared__ double red[kFWaves][NA];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < NA; ++k) {
-    const double v = (LAST_MAX && k == NA - 1) ? wave_max(acc[k]) : wave_sum(acc[k]);
-    if (lane == 0) red[wave][k] = v;
-  }
-  __syncthreads();
-  for (int k = threadIdx.x; k < NA; k += kFThreads) {
-    double v = red[0][k];
-    for (int w = 1; w < kFWaves; ++w) v = (LAST_MAX && k == NA - 1) ? fmax(v, red[w][k]) : v + red[w][k];
-    part[static_cast<long>(k) * nb + blockIdx.x] = v;
-  }
-}
-
-// the element loop of every pass: i over [0, n) with a grid stride, io = i mod no
-#define MDFT_ELEMENT_LOOP(e)                                                        \
-  const long stride_ = static_cast<long>(gridDim.x) * kFThreads;                  \
-  long i = static_cast<long>(blockIdx.x) * kFThreads + threadIdx.x;               \
-  int io = static_cast<int>(i % (e).no);                                          \
-  for (; i < (e).n; i += stride_, io = (io + (e).step >= (e).no) ? io + (e).step - (e).no : io + (e).step)
+using namespace elem;
 
 // g, F_id and F_ext sums, g'H g and max |h0 g| at x
 __global__ __launch_bounds__(kFThreads) void local_kernel(const double* __restrict__ x, const double* __restrict__ V,
@@ -284,22 +225,6 @@ constexpr double kQunit = 1.602176565e-19, kNavo = 6.02214129e23, kEps0 = 8.8541
 constexpr double kPi = 3.14159265358979323846;
 const double kQfact = kQunit * kQunit * 1.0e-3 * kNavo / (4.0 * kPi * kEps0 * 1.0e-10);
 
-template <typename F>
-void dispatch_k(int k, F&& f) {
-  switch (k) {
-    case 0: f(std::integral_constant<int, 0>{}); break;
-    case 1: f(std::integral_constant<int, 1>{}); break;
-    case 2: f(std::integral_constant<int, 2>{}); break;
-    case 3: f(std::integral_constant<int, 3>{}); break;
-    case 4: f(std::integral_constant<int, 4>{}); break;
-    case 5: f(std::integral_constant<int, 5>{}); break;
-    case 6: f(std::integral_constant<int, 6>{}); break;
-    case 7: f(std::integral_constant<int, 7>{}); break;
-    case 8: f(std::integral_constant<int, 8>{}); break;
-    default: throw Error("L-BFGS: too many stored pairs");
-  }
-}
-static_assert(kLbfgsMaxPairs == 8, "dispatch_k covers 0 .. 8 pairs");
 
 }  // namespace
 
@@ -492,9 +417,147 @@ LbfgsResult Functional::minimize(double* xi, const LbfgsOptions& o, hipStream_t 
   return fp32_pairs_ ? minimize_t<float>(xi, o, s) : minimize_t<double>(xi, o, s);
 }
 
+LbfgsState::LbfgsState(const LbfgsOptions& o)
+    : o_(o), SY_(o.m * o.m), YHY_(o.m * o.m), SG_(o.m), YHG_(o.m) {}
+
+void LbfgsState::start(const double F[4], double ghg, double gmax) {
+  std::memcpy(F_, F, sizeof(F_));
+  ghg_ = ghg;
+  gmax_ = gmax;
+  r_ = LbfgsResult();
+  r_.evaluations = 1;
+  r_.history.assign(1, F[0]);
+  order_.clear();
+  it_ = 0;
+  phase_ = kNew;
+}
+
+bool LbfgsState::next(Trial& tr) {
+  if (phase_ == kDone) return false;
+  if (phase_ == kBacktrack) {
+    cur_.t *= 0.5;
+    tr = cur_;
+    return true;
+  }
+  if (gmax_ < o_.gtol || it_ == o_.max_iter) {
+    r_.status = gmax_ < o_.gtol ? 0 : 1;
+    r_.iterations = it_;
+    r_.residual = gmax_;
+    std::memcpy(r_.F, F_, sizeof(F_));
+    phase_ = kDone;
+    return false;
+  }
+  // two-loop recursion in coefficients: q = cg g + sum cq_j y_j,
+  // r = gamma h0 q + sum cs_j s_j, d = -r
+  Trial t;
+  int k = static_cast<int>(order_.size());
+  double al[kLbfgsMaxPairs] = {};
+  for (int i = k - 1; i >= 0; --i) {
+    double sq = t.cg * SG_[i];
+    for (int j = 0; j < k; ++j) sq += t.cq[j] * sy(i, j);
+    al[i] = sq / sy(i, i);
+    t.cq[i] -= al[i];
+  }
+  t.gamma = k ? sy(k - 1, k - 1) / yhy(k - 1, k - 1) : 1.0;
+  for (int i = 0; i < k; ++i) {
+    double yq = t.cg * YHG_[i];
+    for (int j = 0; j < k; ++j) yq += t.cq[j] * yhy(i, j);
+    double yr = t.gamma * yq;
+    for (int j = 0; j < k; ++j) yr += t.cs[j] * sy(j, i);
+    t.cs[i] += al[i] - yr / sy(i, i);
+  }
+  double gq = t.cg * ghg_;
+  for (int j = 0; j < k; ++j) gq += t.cq[j] * YHG_[j];
+  double gd = t.gamma * gq;
+  for (int j = 0; j < k; ++j) gd += t.cs[j] * SG_[j];
+  gd = -gd;
+  if (!(gd < 0)) {  // not a descent direction: restart from -h0 g
+    order_.clear();
+    k = 0;
+    t = Trial();
+    gd = -ghg_;
+  }
+  t.k = k;
+  t.cand = 0;
+  while (std::find(order_.begin(), order_.end(), t.cand) != order_.end()) ++t.cand;
+  t.t = 1.0;
+  gd_ = gd;
+  tries_ = 0;
+  cur_ = t;
+  tr = t;
+  return true;
+}
+
+bool LbfgsState::judge(const double Fn[4], const double* h, int K) {
+  ++r_.evaluations;
+  ++tries_;
+  const double t = cur_.t;
+  // slope at the trial point: g_t . d = (s . g_t) / t
+  const double gtd = h[3 * K + 3] / t;
+  const double fscale = std::fabs(F_[1]) + std::fabs(F_[2]) + std::fabs(F_[3]);
+  const bool ok = std::isfinite(Fn[0]) && (Fn[0] <= F_[0] + o_.c1 * t * gd_ ||
+                                           (Fn[0] <= F_[0] + o_.eps_f * fscale && gtd <= (2 * o_.c1 - 1) * gd_));
+  if (!ok) {
+    if (tries_ >= o_.max_backtrack) {
+      r_.status = 2;
+      r_.iterations = it_;
+      r_.residual = gmax_;
+      std::memcpy(r_.F, F_, sizeof(F_));
+      phase_ = kDone;
+    } else {
+      phase_ = kBacktrack;
+    }
+    return false;
+  }
+  // the new Gram entries (pair columns beyond the trial's k are not used)
+  const int m = o_.m;
+  int k = cur_.k;
+  std::vector<double> sn_y(h, h + k), s_yn(h + K + 1, h + K + 1 + k), yn_hy(h + 3 * K + 4, h + 3 * K + 4 + k);
+  const double snyn = h[K], snsn = h[2 * K + 1], ynyn = h[2 * K + 2], sn_g = h[3 * K + 3], ynhyn = h[4 * K + 4],
+               yn_hg = h[5 * K + 5];
+  for (int j = 0; j < k; ++j) {
+    SG_[j] = h[2 * K + 3 + j];
+    YHG_[j] = h[4 * K + 5 + j];
+  }
+  ghg_ = h[5 * K + 6];
+  gmax_ = h[5 * K + 9];
+  if (snyn > 1e-12 * std::sqrt(snsn * ynyn)) {  // keep the pair
+    if (k == m) {  // drop the oldest
+      order_.erase(order_.begin());
+      for (int i = 0; i + 1 < m; ++i) {
+        for (int j = 0; j + 1 < m; ++j) {
+          sy(i, j) = sy(i + 1, j + 1);
+          yhy(i, j) = yhy(i + 1, j + 1);
+        }
+        SG_[i] = SG_[i + 1];
+        YHG_[i] = YHG_[i + 1];
+      }
+      sn_y.erase(sn_y.begin());
+      s_yn.erase(s_yn.begin());
+      yn_hy.erase(yn_hy.begin());
+      k = m - 1;
+    }
+    order_.push_back(cur_.cand);
+    for (int j = 0; j < k; ++j) {
+      sy(k, j) = sn_y[j];
+      sy(j, k) = s_yn[j];
+      yhy(k, j) = yhy(j, k) = yn_hy[j];
+    }
+    sy(k, k) = snyn;
+    yhy(k, k) = ynhyn;
+    SG_[k] = sn_g;
+    YHG_[k] = yn_hg;
+  }
+  std::memcpy(F_, Fn, sizeof(F_));
+  r_.history.push_back(F_[0]);
+  ++it_;
+  phase_ = kNew;
+  return true;
+}
+
 template <typename TS>
 LbfgsResult Functional::minimize_t(double* xi, const LbfgsOptions& o, hipStream_t s) {
-  const int m = o.m, nslot = m + 1;
+  const int nslot = o.m + 1;
   const Elem e{tab_, tab_ + p_.no, p_.rho0, p_.kT, p_.dv, 2.0 * p_.prefactor / p_.N, n_, p_.no, step_};
   double* x = xi;
   double* xt = static_cast<double*>(work_);
@@ -507,7 +570,6 @@ LbfgsResult Functional::minimize_t(double* xi, const LbfgsOptions& o, hipStream_
     Y[j] = pairs + (2 * static_cast<long>(j) + 1) * n_;
   }
 
-  LbfgsResult r;
   // the start point
   excess_(x, df_, res_ + kFfSlot, s);
   hipLaunchKernelGGL(local_kernel, dim3(nb_), dim3(kFThreads), 0, s, x, vext_, df_, g, e, part_, nb_);
@@ -516,161 +578,58 @@ LbfgsResult Functional::minimize_t(double* xi, const LbfgsOptions& o, hipStream_
   const double* h = read_results(s);
   double F[4];
   fparts(h[0], h[1], h[kFfSlot], F);
-  double GHG = h[2], gmax = h[3];
-  r.evaluations = 1;
-  r.history.push_back(F[0]);
+  LbfgsState st(o);
+  st.start(F, h[2], h[3]);
 
-  // stored pairs, oldest first, as slot indices; Gram entries by position
-  std::vector<int> order;
-  std::vector<double> SY(m * m), YHY(m * m), SG(m), YHG(m);
-  auto sy = [&](int i, int j) -> double& { return SY[i * m + j]; };
-  auto yhy = [&](int i, int j) -> double& { return YHY[i * m + j]; };
-
-  int it = 0;
-  r.status = 1;
-  for (; it < o.max_iter; ++it) {
-    if (gmax < o.gtol) {
-      r.status = 0;
-      break;
-    }
-    int k = static_cast<int>(order.size());
-    // two-loop recursion in coefficients: q = cg g + sum cq_j y_j,
-    // r = gamma h0 q + sum cs_j s_j, d = -r
-    const double cg = 1.0;
-    double cq[kLbfgsMaxPairs] = {}, cs[kLbfgsMaxPairs] = {}, al[kLbfgsMaxPairs] = {};
-    for (int i = k - 1; i >= 0; --i) {
-      double sq = cg * SG[i];
-      for (int j = 0; j < k; ++j) sq += cq[j] * sy(i, j);
-      al[i] = sq / sy(i, i);
-      cq[i] -= al[i];
-    }
-    double gamma = k ? sy(k - 1, k - 1) / yhy(k - 1, k - 1) : 1.0;
-    for (int i = 0; i < k; ++i) {
-      double yq = cg * YHG[i];
-      for (int j = 0; j < k; ++j) yq += cq[j] * yhy(i, j);
-      double yr = gamma * yq;
-      for (int j = 0; j < k; ++j) yr += cs[j] * sy(j, i);
-      cs[i] += al[i] - yr / sy(i, i);
-    }
-    double gq = cg * GHG;
-    for (int j = 0; j < k; ++j) gq += cq[j] * YHG[j];
-    double gd = gamma * gq;
-    for (int j = 0; j < k; ++j) gd += cs[j] * SG[j];
-    gd = -gd;
-    if (!(gd < 0)) {  // not a descent direction: restart from -h0 g
-      order.clear();
-      k = 0;
-      gamma = 1.0;
-      gd = -GHG;
-    }
-    int c = 0;
-    while (std::find(order.begin(), order.end(), c) != order.end()) ++c;
-
+  // each trial: x + t d, the excess evaluation there, then g, the F_id and
+  // F_ext sums and the candidate pair's Gram entries (kept if accepted)
+  LbfgsState::Trial tr;
+  while (st.next(tr)) {
+    const int K = tr.k;
     DirArgs<TS> da{};
+    UpdArgs<TS> ua{};
+    for (int j = 0; j < K; ++j) {
+      da.s[j] = ua.s[j] = S[st.order()[j]];
+      da.y[j] = ua.y[j] = Y[st.order()[j]];
+      da.cs[j] = tr.cs[j];
+      da.cq[j] = tr.cq[j];
+    }
     da.x = x;
     da.g = g;
-    UpdArgs<TS> ua{};
+    da.cg = tr.cg;
+    da.gamma = tr.gamma;
+    da.t = tr.t;
+    da.xt = xt;
+    da.sc = S[tr.cand];
     ua.xt = xt;
     ua.V = vext_;
     ua.df = df_;
     ua.g = g;
     ua.gt = gt;
-    ua.sc = S[c];
-    ua.yn = Y[c];
-    for (int j = 0; j < k; ++j) {
-      da.s[j] = ua.s[j] = S[order[j]];
-      da.y[j] = ua.y[j] = Y[order[j]];
-      da.cs[j] = cs[j];
-      da.cq[j] = cq[j];
+    ua.sc = S[tr.cand];
+    ua.yn = Y[tr.cand];
+    dispatch_k(K, [&](auto kc) {
+      hipLaunchKernelGGL((direction_kernel<TS, decltype(kc)::value>), dim3(nb_), dim3(kFThreads), 0, s, da, e);
+    });
+    MDFT_HIP_CHECK(hipGetLastError());
+    excess_(xt, df_, res_ + kFfSlot, s);
+    dispatch_k(K, [&](auto kc) {
+      hipLaunchKernelGGL((update_kernel<TS, decltype(kc)::value>), dim3(nb_), dim3(kFThreads), 0, s, ua, e, part_,
+                         nb_);
+    });
+    MDFT_HIP_CHECK(hipGetLastError());
+    reduce_rows(part_, nb_, 0, 5 * K + 10, 5 * K + 9, res_, s);
+    h = read_results(s);
+    double Fn[4];
+    fparts(h[5 * K + 7], h[5 * K + 8], h[kFfSlot], Fn);
+    if (st.judge(Fn, h, K)) {
+      std::swap(x, xt);
+      std::swap(g, gt);
     }
-    da.cg = cg;
-    da.gamma = gamma;
-    da.xt = xt;
-    da.sc = S[c];
-    const int K = k;
-    // each trial: x + t d, the excess evaluation there, then g, the F_id and
-    // F_ext sums and the candidate pair's Gram entries (kept if accepted)
-    const double fscale = std::fabs(F[1]) + std::fabs(F[2]) + std::fabs(F[3]);
-    double t = 1.0, Fn[4] = {0, 0, 0, 0};
-    bool accepted = false;
-    for (int b = 0; b < o.max_backtrack; ++b) {
-      da.t = t;
-      dispatch_k(K, [&](auto kc) {
-        hipLaunchKernelGGL((direction_kernel<TS, decltype(kc)::value>), dim3(nb_), dim3(kFThreads), 0, s, da, e);
-      });
-      MDFT_HIP_CHECK(hipGetLastError());
-      excess_(xt, df_, res_ + kFfSlot, s);
-      dispatch_k(K, [&](auto kc) {
-        hipLaunchKernelGGL((update_kernel<TS, decltype(kc)::value>), dim3(nb_), dim3(kFThreads), 0, s, ua, e, part_,
-                           nb_);
-      });
-      MDFT_HIP_CHECK(hipGetLastError());
-      reduce_rows(part_, nb_, 0, 5 * K + 10, 5 * K + 9, res_, s);
-      h = read_results(s);
-      fparts(h[5 * K + 7], h[5 * K + 8], h[kFfSlot], Fn);
-      ++r.evaluations;
-      // slope at the trial point: g_t . d = (sc . g_t) / t
-      const double gtd = h[3 * K + 3] / t;
-      if (std::isfinite(Fn[0]) && (Fn[0] <= F[0] + o.c1 * t * gd ||
-                                   (Fn[0] <= F[0] + o.eps_f * fscale && gtd <= (2 * o.c1 - 1) * gd))) {
-        accepted = true;
-        break;
-      }
-      t *= 0.5;
-    }
-    if (!accepted) {
-      r.status = 2;
-      break;
-    }
-
-    std::vector<double> sn_y(h, h + K), s_yn(h + K + 1, h + 2 * K + 1), yn_hy(h + 3 * K + 4, h + 4 * K + 4);
-    const double snyn = h[K], snsn = h[2 * K + 1], ynyn = h[2 * K + 2], sn_g = h[3 * K + 3], ynhyn = h[4 * K + 4],
-                 yn_hg = h[5 * K + 5];
-    for (int j = 0; j < K; ++j) {
-      SG[j] = h[2 * K + 3 + j];
-      YHG[j] = h[4 * K + 5 + j];
-    }
-    GHG = h[5 * K + 6];
-    gmax = h[5 * K + 9];
-    if (snyn > 1e-12 * std::sqrt(snsn * ynyn)) {  // keep the pair
-      if (k == m) {  // drop the oldest
-        order.erase(order.begin());
-        for (int i = 0; i + 1 < m; ++i) {
-          for (int j = 0; j + 1 < m; ++j) {
-            sy(i, j) = sy(i + 1, j + 1);
-            yhy(i, j) = yhy(i + 1, j + 1);
-          }
-          SG[i] = SG[i + 1];
-          YHG[i] = YHG[i + 1];
-        }
-        sn_y.erase(sn_y.begin());
-        s_yn.erase(s_yn.begin());
-        yn_hy.erase(yn_hy.begin());
-        k = m - 1;
-      }
-      order.push_back(c);
-      for (int j = 0; j < k; ++j) {
-        sy(k, j) = sn_y[j];
-        sy(j, k) = s_yn[j];
-        yhy(k, j) = yhy(j, k) = yn_hy[j];
-      }
-      sy(k, k) = snyn;
-      yhy(k, k) = ynhyn;
-      SG[k] = sn_g;
-      YHG[k] = yn_hg;
-    }
-    std::swap(x, xt);
-    std::swap(g, gt);
-    std::memcpy(F, Fn, sizeof(F));
-    r.history.push_back(F[0]);
   }
-  if (it == o.max_iter && r.status == 1 && gmax < o.gtol) r.status = 0;
-  r.iterations = it;
-  r.residual = gmax;
-  std::memcpy(r.F, F, sizeof(F));
   if (x != xi) MDFT_HIP_CHECK(hipMemcpyAsync(xi, x, n_ * sizeof(double), hipMemcpyDeviceToDevice, s));
   MDFT_HIP_CHECK(hipStreamSynchronize(s));
-  return r;
+  return st.result();
 }
 
 template LbfgsResult Functional::minimize_t<double>(double*, const LbfgsOptions&, hipStream_t);

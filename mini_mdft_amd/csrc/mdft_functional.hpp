@@ -45,6 +45,45 @@ struct LbfgsResult {
   std::vector<double> history;  // F of the start point, then after every iteration
 };
 
+// Host side of one L-BFGS solve in vector-free form: the stored pairs'
+// slots and Gram entries, the two-loop recursion in coefficients, and the line
+// search decisions.  A driver launches the passes: Functional::minimize for
+// one field, the batch engine for many.  Pair slots are 0 .. m (m + 1 slots:
+// the candidate pair of a trial never overwrites a stored one).
+class LbfgsState {
+ public:
+  struct Trial {
+    int k = 0;     // stored pairs the direction uses: slots order()[0 .. k)
+    int cand = 0;  // slot receiving the candidate pair (s = x_t - x, y = g_t - g)
+    double cg = 1.0, gamma = 1.0, t = 1.0;
+    double cs[kLbfgsMaxPairs] = {}, cq[kLbfgsMaxPairs] = {};
+  };
+  explicit LbfgsState(const LbfgsOptions& o);
+  // results at the start point: F (4 terms), g'H g and max |h0 g|
+  void start(const double F[4], double ghg, double gmax);
+  // plans the next trial; false once the solve has ended (result() is final)
+  bool next(Trial& tr);
+  // the trial's F (4 terms) and update-pass results h (layout of a pass
+  // with K pair columns, K >= the trial's k); true when the trial is
+  // accepted (the driver then takes x_t and g_t as the current point)
+  bool judge(const double Fn[4], const double* h, int K);
+  const std::vector<int>& order() const { return order_; }
+  const LbfgsResult& result() const { return r_; }
+
+ private:
+  double& sy(int i, int j) { return SY_[i * o_.m + j]; }
+  double& yhy(int i, int j) { return YHY_[i * o_.m + j]; }
+  LbfgsOptions o_;
+  LbfgsResult r_;
+  enum Phase { kNew, kBacktrack, kDone } phase_ = kNew;
+  double F_[4] = {0, 0, 0, 0};
+  double ghg_ = 0.0, gmax_ = 0.0, gd_ = 0.0;
+  int it_ = 0, tries_ = 0;
+  Trial cur_;
+  std::vector<int> order_;
+  std::vector<double> SY_, YHY_, SG_, YHG_;
+};
+
 class Functional {
  public:
   // the excess (c-proj) evaluation: fills df (device, N x no doubles) for xi

@@ -92,27 +92,28 @@ def model_c_symmetric(nmax: int, mrso: int, nq: int, seed: int, dq: float = DQ_T
                       amplitude: float = 1.0) -> LucTable:
     """A model c(q) for minimisations: exchange symmetric and stabilising.
 
-    Only the tuples with m = n and mu = nu are nonzero, with equal values at
-    khi and -khi; for this family the excess operator (proj2angl . conv .
-    angl2proj) is self-adjoint in the w-weighted orientation inner product, so
-    F_exc is a quadratic form whose gradient is 2 prefactor w vexc d(Delta rho)
-    (tests/test_functional.py checks <a, K b>_w = <b, K a>_w).  Each value is
-    -a exp(-(q r0)^2 / 4) with a = amplitude U[0.2, 1] / (1 + m) and r0 ~
-    U[1.5, 3] A (the stream of (seed, 2 tuple + {0, 1})): negative like the
-    core of a physical c, which keeps F bounded below.  The physical direct
-    correlation function has the same exchange symmetry; the independent
-    draws of synthetic_c do not."""
+    Only the tuples (m, m, 0, 0, 0) are nonzero: the correlation of the
+    molecular axes along q, one channel per m <= nmax.  For this family the
+    excess operator K = proj2angl . conv . angl2proj is self-adjoint in the
+    w-weighted orientation inner product and negative semidefinite, so F_exc
+    = (dv / N) prefactor rho0^2 <u, K u>_w (prefactor < 0) is a nonnegative
+    quadratic form and F is bounded below; the gradient 2 prefactor w vexc
+    d(Delta rho) is exact (tests/test_functional.py checks the symmetry and
+    the spectrum).  Each value is -a exp(-(q r0)^2 / 4) with a = amplitude
+    U[0.2, 1] / (1 + m) and r0 ~ U[2, 3.5] A (the stream of (seed, 2 tuple +
+    {0, 1})): negative like the core of a physical c, decayed by the Nyquist
+    wave numbers of the grids used (where even grids pair a mode with itself
+    and break the symmetry).  The independent draws of synthetic_c, and
+    channels with mu != 0 or khi != 0, make K indefinite."""
     tup = luc_tuples(nmax, mrso)
     u = uniform(seed, 2 * tup.shape[0]).reshape(-1, 2)
     q = np.arange(nq, dtype=np.float64) * dq
     c = np.zeros((tup.shape[0], nq), dtype=np.complex128)
-    index = {tuple(int(v) for v in t): i for i, t in enumerate(tup)}
     for i, (m, n, mu, nu, khi) in enumerate(tup):
-        if m == n and mu == nu and khi >= 0:
+        if m == n and mu == 0 and nu == 0 and khi == 0:
             a = amplitude * (0.2 + 0.8 * u[i, 0]) / (1.0 + m)
-            r0 = 1.5 + 1.5 * u[i, 1]
+            r0 = 2.0 + 1.5 * u[i, 1]
             c[i] = -a * np.exp(-(q * r0) ** 2 / 4.0)
-            c[index[(int(m), int(n), int(mu), int(nu), -int(khi))]] = c[i]
     return LucTable(tup, c, dq)
 
 

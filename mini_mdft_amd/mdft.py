@@ -440,3 +440,87 @@ class Functional:
                                         hist.ctypes.data_as(_lib._pd), stream_handle(stream)))
         return dict(F=tuple(F), iterations=it.value, evaluations=nev.value, status=st.value,
                     residual=res.value, history=hist[:it.value + 1].copy())
+
+
+class Server:
+    """Batched and served minimisation (no reference counterpart): solutes of
+    any shape under one HBM budget, same-shape solutes advanced together in
+    engines of `capacity` slots whose ticks evaluate all slots in one batched
+    pass of every stage, replayed as one HIP graph.  `table` is the c(q)
+    (mini_mdft_amd.luc.LucTable) for every mmax that will be requested."""
+
+    STATUS = {0: "converged", 1: "max_iter", 2: "line search", 3: "rejected (HBM budget)"}
+
+    def __init__(self, table, mrso: int, n0: float, temperature: float, precision: int = MIXED_X6,
+                 capacity: int = 4, m: int = 5, max_iter: int = 400, gtol: float = 1e-5,
+                 budget_bytes: float = 200e9):
+        tup = np.ascontiguousarray(table.tuples, dtype=np.int32)
+        c = np.ascontiguousarray(table.c, dtype=np.complex128)
+        h = ctypes.c_void_p()
+        check(lib().mdft_server_create(tup.shape[0], tup.ctypes.data_as(_lib._pi), c.ctypes.data_as(_lib._pd),
+                                       c.shape[1], ctypes.c_double(table.dq), mrso, ctypes.c_double(n0),
+                                       ctypes.c_double(temperature), precision, capacity, m, max_iter,
+                                       ctypes.c_double(gtol), ctypes.c_double(budget_bytes), ctypes.byref(h)))
+        self._h, self._destroy = h, lib().mdft_server_destroy
+        self.mrso, self._outs = mrso, {}
+
+    def __del__(self):
+        h, destroy = getattr(self, "_h", None), getattr(self, "_destroy", None)
+        if h and destroy is not None:
+            destroy(h)
+
+    def use_graphs(self, on: bool) -> None:
+        check(lib().mdft_server_use_graphs(self._h, 1 if on else 0))
+
+    def submit(self, n_nodes, length, mmax: int, solute, solvent, vmax: float, out=None) -> int:
+        """Queues one solute; `out` (a float64 CUDA tensor of N x no) receives
+        the final xi.  Returns the request id."""
+        solute = np.ascontiguousarray(solute, dtype=np.float64).reshape(-1, 6)
+        solvent = np.ascontiguousarray(solvent, dtype=np.float64).reshape(-1, 6)
+        rid = ctypes.c_longlong()
+        check(lib().mdft_server_submit(self._h, *(int(v) for v in n_nodes), *(ctypes.c_double(v) for v in length),
+                                       mmax, solute.shape[0], solute.ctypes.data_as(_lib._pd), solvent.shape[0],
+                                       solvent.ctypes.data_as(_lib._pd), ctypes.c_double(vmax),
+                                       ptr(out) if out is not None else None, ctypes.byref(rid)))
+        if out is not None:
+            self._outs[rid.value] = out  # kept alive until the solve ends
+        return rid.value
+
+    def step(self):
+        """Admissions, then one tick of every engine; returns (active, pending)."""
+        a, p = ctypes.c_int(), ctypes.c_int()
+        check(lib().mdft_server_step(self._h, ctypes.byref(a), ctypes.byref(p)))
+        return a.value, p.value
+
+    def poll(self):
+        """One finished solve as a dict, or None."""
+        rid, it, nev, st, got = (ctypes.c_longlong(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int(),
+                                 ctypes.c_int())
+        F = (ctypes.c_double * 4)()
+        res, tq, tt = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        check(lib().mdft_server_poll(self._h, ctypes.byref(rid), F, ctypes.byref(it), ctypes.byref(nev),
+                                     ctypes.byref(st), ctypes.byref(res), ctypes.byref(tq), ctypes.byref(tt),
+                                     ctypes.byref(got)))
+        if not got.value:
+            return None
+        self._outs.pop(rid.value, None)
+        return dict(id=rid.value, F=tuple(F), iterations=it.value, evaluations=nev.value, status=st.value,
+                    residual=res.value, queued_s=tq.value, total_s=tt.value)
+
+    def info(self):
+        b, e, a, p = ctypes.c_double(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().mdft_server_info(self._h, ctypes.byref(b), ctypes.byref(e), ctypes.byref(a), ctypes.byref(p)))
+        return dict(bytes_in_use=b.value, engines=e.value, active=a.value, pending=p.value)
+
+    def run(self, max_steps: int = 100000):
+        """Steps until nothing is active or pending; returns the results by id."""
+        out = {}
+        for _ in range(max_steps):
+            a, p = self.step()
+            while (r := self.poll()) is not None:
+                out[r["id"]] = r
+            if a == 0 and p == 0:
+                break
+        while (r := self.poll()) is not None:
+            out[r["id"]] = r
+        return out

@@ -148,6 +148,31 @@ def test_model_c_makes_the_excess_operator_self_adjoint(oracle, sym):
         assert asym >= 1e-3  # the independent draws of synthetic_c are not symmetric
 
 
+@pytest.mark.parametrize("sym", [True, False])
+def test_model_c_makes_the_excess_operator_negative_semidefinite(oracle, sym):
+    """The dense w-weighted matrix of K on an odd 3^3 grid (mmax 2): no
+    positive eigenvalue for the model c, positive ones for synthetic_c."""
+    n, L, mmax = (3, 3, 3), (4.0,) * 3, 2
+    s = oracle.angular_sizes(mmax, MRSO)
+    w = oracle.angular_tables(mmax, MRSO)["w"]
+    nq = luc.nq_for(luc.qmax_of_grid(n, L), luc.DQ_TIP3P)
+    tab = luc.model_c_symmetric(mmax, MRSO, nq, seed=5) if sym else luc.synthetic_c(mmax, MRSO, nq, seed=5)
+    conv = oracle.Conv(n, L, mmax, MRSO, tab.tuples, tab.c, tab.dq)
+    dim = 27 * s["no"]
+    A = np.zeros((dim, dim))
+    for j in range(dim):
+        e = np.zeros(dim)
+        e[j] = 1.0
+        p = oracle.angl2proj(e.reshape(27, s["ntheta"], s["nphi"], s["npsi"]), n, mmax, MRSO)
+        A[:, j] = oracle.proj2angl(conv.execute(p), mmax, MRSO).ravel()
+    A *= np.tile(w, 27)[:, None]
+    ev = np.linalg.eigvalsh(0.5 * (A + A.T))
+    if sym:
+        assert ev.max() <= 1e-12 * abs(ev).max() and ev.min() < 0
+    else:
+        assert ev.max() > 1e-3 * abs(ev).max()
+
+
 def test_functional_gradient_finite_difference(oracle):
     s, w, conv, V = _setup(oracle)
     rng = np.random.default_rng(2)

@@ -1,0 +1,146 @@
+"""Batched and served minimisation: each solute's result is bitwise the one it
+gets alone (alone in an engine, in a mix of four with staggered arrivals, and
+through the plain Functional.minimize), graph replay changes nothing, and
+admission keeps the device footprint under the budget."""
+import numpy as np
+import pytest
+
+from mini_mdft_amd import FP32, FP64, MIXED_X6, luc, solute
+from mini_mdft_amd import mdft as M
+
+N3, L3, MMAX, MRSO = (8, 8, 8), (10.0,) * 3, 3, 2
+TEMP, N0 = 298.15, 0.0332891
+KT = M.kT(TEMP)
+WATER = solute.SOLVENT_SITES["tip3p"]
+VMAX = 30 * KT
+
+
+def _table(nmax=3, n=N3, L=L3):
+    return luc.model_c_symmetric(nmax, MRSO, luc.nq_for(luc.qmax_of_grid(n, L), luc.DQ_TIP3P) + 4, seed=5)
+
+
+def _solutes(k, L=L3):
+    return [solute.synthetic_solute(3, 10 + i, L, radius=2.0, charge_scale=0.1) for i in range(k)]
+
+
+def test_server_argument_checks():
+    tab = _table()
+    with pytest.raises(M.MDFTError, match="FP32"):
+        M.Server(tab, MRSO, N0, TEMP, FP32)
+    with pytest.raises(M.MDFTError, match="m must be"):
+        M.Server(tab, MRSO, N0, TEMP, FP64, m=9)
+    with pytest.raises(M.MDFTError, match="capacity"):
+        M.Server(tab, MRSO, N0, TEMP, FP64, capacity=0)
+    srv = M.Server(tab, MRSO, N0, TEMP, FP64, capacity=2)
+    with pytest.raises(M.MDFTError, match="multiple of 64"):
+        srv.submit((9, 9, 9), L3, MMAX, _solutes(1)[0], WATER, VMAX)
+    assert srv.info() == dict(bytes_in_use=0.0, engines=0, active=0, pending=0)
+
+
+def _run(srv, subs, stagger=0):
+    """Submits (solute, out) pairs, the later ones after `stagger` steps."""
+    ids = []
+    for i, (sol, out) in enumerate(subs):
+        if i == 2 and stagger:
+            for _ in range(stagger):
+                srv.step()
+        ids.append(srv.submit(N3, L3, MMAX, sol, WATER, VMAX, out=out))
+    res = srv.run()
+    return [res[i] for i in ids]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [FP64, MIXED_X6])
+def test_alone_equals_in_the_mix_bitwise(prec):
+    import torch
+
+    tab = _table()
+    sols = _solutes(4)
+    no = M.AngularGrid(MMAX, MRSO).m_no
+    new = lambda: torch.zeros((512, no), dtype=torch.float64, device="cuda")  # noqa: E731
+    # the first solute alone in an engine of four slots
+    alone_out = new()
+    alone = _run(M.Server(tab, MRSO, N0, TEMP, prec, capacity=4), [(sols[0], alone_out)])[0]
+    # all four, two of them arriving three ticks later
+    outs = [new() for _ in sols]
+    mix = _run(M.Server(tab, MRSO, N0, TEMP, prec, capacity=4), list(zip(sols, outs)), stagger=3)
+    assert alone["status"] == 0 and all(r["status"] == 0 for r in mix)
+    assert mix[0]["F"] == alone["F"]
+    assert mix[0]["iterations"] == alone["iterations"] and mix[0]["evaluations"] == alone["evaluations"]
+    assert torch.equal(outs[0], alone_out)
+    # the other solutes differ (the comparison can fail)
+    assert mix[1]["F"] != mix[0]["F"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [FP64, MIXED_X6])
+def test_server_matches_functional_minimize(prec):
+    """A solute through the server and through the plain Functional.minimize
+    on the same inputs: the same decisions and the same numbers."""
+    import torch
+
+    tab = _table()
+    sol = _solutes(1)[0]
+    sg, ag = M.SpatialGrid(N3, L3), M.AngularGrid(MMAX, MRSO)
+    out = torch.zeros((sg.N, ag.m_no), dtype=torch.float64, device="cuda")
+    r_srv = _run(M.Server(tab, MRSO, N0, TEMP, prec, capacity=2), [(sol, out)])[0]
+    conv = M.Convolution.from_table(tab, sg, ag, M.OrientationProjectionMap(ag), FP64 if prec == FP64 else FP32)
+    solver = M.Solver(conv, M.rho0_of(N0, MRSO), M.prefactor_of(TEMP, MRSO, N0), prec)
+    V = M.vext(sg, ag, sol, WATER, VMAX)
+    xi = torch.ones_like(out)
+    r = M.Functional(solver, V, KT).minimize(xi, m=5, max_iter=400, gtol=1e-5)
+    assert r["status"] == r_srv["status"] == 0
+    assert r["iterations"] == r_srv["iterations"] and r["evaluations"] == r_srv["evaluations"]
+    assert r["F"] == r_srv["F"]
+    assert torch.equal(xi, out)
+
+
+@pytest.mark.gpu
+def test_graph_replay_changes_nothing():
+    import torch
+
+    tab = _table()
+    sols = _solutes(3)
+    no = M.AngularGrid(MMAX, MRSO).m_no
+    res = {}
+    for graphs in (False, True):
+        srv = M.Server(tab, MRSO, N0, TEMP, MIXED_X6, capacity=4)
+        srv.use_graphs(graphs)
+        outs = [torch.zeros((512, no), dtype=torch.float64, device="cuda") for _ in sols]
+        res[graphs] = (_run(srv, list(zip(sols, outs)), stagger=2), outs)
+    (a, oa), (b, ob) = res[False], res[True]
+    for ra, rb, xa, xb in zip(a, b, oa, ob):
+        assert ra["F"] == rb["F"] and ra["iterations"] == rb["iterations"]
+        assert torch.equal(xa, xb)
+
+
+@pytest.mark.gpu
+def test_mixed_shapes_under_a_budget():
+    """Two shapes and a budget for about two solutes at a time: everything
+    finishes, the footprint never exceeds the budget, and a request larger
+    than the budget is rejected (status 3)."""
+    tab = _table(nmax=3, n=(16, 16, 16), L=(12.0,) * 3)
+    shapes = [((8, 8, 8), (10.0,) * 3, 3), ((16, 8, 4), (12.0, 8.0, 6.0), 2)]
+    probe = M.Server(tab, MRSO, N0, TEMP, MIXED_X6, capacity=2, budget_bytes=1e12)
+    probe.submit(*shapes[0], _solutes(1)[0], WATER, VMAX)
+    probe.step()
+    one = probe.info()["bytes_in_use"]
+    budget = 2.2 * one
+    srv = M.Server(tab, MRSO, N0, TEMP, MIXED_X6, capacity=2, budget_bytes=budget)
+    ids = []
+    for i in range(6):
+        n, L, mm = shapes[i % 2]
+        ids.append(srv.submit(n, L, mm, solute.synthetic_solute(3, 30 + i, L, radius=2.0, charge_scale=0.1),
+                              WATER, VMAX))
+    big = srv.submit((64, 64, 64), (30.0,) * 3, 5, _solutes(1)[0], WATER, VMAX)
+    done, peak = {}, 0.0
+    for _ in range(100000):
+        a, p = srv.step()
+        peak = max(peak, srv.info()["bytes_in_use"])
+        while (r := srv.poll()) is not None:
+            done[r["id"]] = r
+        if a == 0 and p == 0:
+            break
+    assert done[big]["status"] == 3
+    assert all(done[i]["status"] == 0 for i in ids)
+    assert 0 < peak <= budget
