@@ -1,5 +1,7 @@
-// Batched unnormalised 3D c2c FFT for gfx950: one launch per axis, each a
-// mixed-radix Stockham transform of a tile of lines staged in LDS.
+// Batched unnormalised 3D c2c FFT for gfx950: mixed-radix Stockham
+// transforms of tiles staged in LDS, two launches per 3D transform when a
+// (y, z) plane fits in LDS (one pass for z and y together, one for x),
+// otherwise one launch per axis.
 //
 // Replaces the kokkos-fft plans of common/MDFT_Convolution.hpp:198-208 and
 // their execution at :328-329 (forward) and :546-547 (backward), both with
@@ -132,6 +134,109 @@ __device__ __forceinline__ void stockham_stage_generic(const cplx<T>* src, cplx<
     dst[w * ld + (j / Ns) * Ns * R + k + s * Ns] = acc;
   }
 }
+
+// The same Stockham stage over lines of a 2D LDS tile with a line stride ls
+// and an element stride es: element j of line w sits at w ls + j es.  With
+// W_FAST, consecutive threads take consecutive lines (the column pass of a
+// plane: lines are neighbouring columns, so the accesses of a wave are
+// consecutive).
+template <typename T, int R, bool W_FAST>
+__device__ __forceinline__ void stockham_stage_2d(const cplx<T>* src, cplx<T>* dst, const cplx<T>* tw, int n, int ls,
+                                                  int es, int Ns, int nlines, int sign, int tid, int nthreads) {
+  const int nb = n / R;
+  const int tstep = n / (Ns * R);
+  for (int beta = tid; beta < nlines * nb; beta += nthreads) {
+    int w, j;
+    if (W_FAST) {
+      j = beta / nlines;
+      w = beta - j * nlines;
+    } else {
+      w = beta / nb;
+      j = beta - w * nb;
+    }
+    const int k = j % Ns;
+    const cplx<T>* in = src + w * ls + j * es;
+    cplx<T> v[R];
+    v[0] = in[0];
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = cmul<T>(in[r * nb * es], twiddle<T>(tw, r * k * tstep, sign));
+    dft_fixed<T, R>(v, sign, tw, n);
+    cplx<T>* out = dst + w * ls + ((j / Ns) * Ns * R + k) * es;
+#pragma unroll
+    for (int r = 0; r < R; ++r) out[r * Ns * es] = v[r];
+  }
+}
+
+// the stages of one axis over a 2D LDS tile (radices 2, 3, 4, 5), ping-pong
+// between a and b; returns the buffer holding the result
+template <typename T, bool W_FAST>
+__device__ cplx<T>* stockham_2d(cplx<T>* a, cplx<T>* b, const cplx<T>* tw, const FftStages& st, int ls, int es,
+                                int nlines, int sign, int tid, int nthreads) {
+  int Ns = 1;
+  for (int s = 0; s < st.nstages; ++s) {
+    const int R = st.radix[s];
+    switch (R) {
+      case 4: stockham_stage_2d<T, 4, W_FAST>(a, b, tw, st.n, ls, es, Ns, nlines, sign, tid, nthreads); break;
+      case 2: stockham_stage_2d<T, 2, W_FAST>(a, b, tw, st.n, ls, es, Ns, nlines, sign, tid, nthreads); break;
+      case 3: stockham_stage_2d<T, 3, W_FAST>(a, b, tw, st.n, ls, es, Ns, nlines, sign, tid, nthreads); break;
+      default: stockham_stage_2d<T, 5, W_FAST>(a, b, tw, st.n, ls, es, Ns, nlines, sign, tid, nthreads); break;
+    }
+    __syncthreads();
+    cplx<T>* t = a;
+    a = b;
+    b = t;
+    Ns *= R;
+  }
+  return a;
+}
+
+constexpr int kPlaneThreads = 512;
+
+// z and y transforms of one (batch, x) plane of ny x nz contiguous values in
+// one pass: the plane is staged in LDS (rows padded to nz + 1), transformed
+// along its rows, then along its columns, and written back.  Radices 2-5 only.
+template <typename T>
+__global__ __launch_bounds__(kPlaneThreads) void fft_plane_kernel(cplx<T>* __restrict__ data, FftStages sz,
+                                                                  FftStages sy, const cplx<T>* __restrict__ twz_g,
+                                                                  const cplx<T>* __restrict__ twy_g, int sign) {
+  extern __shared__ unsigned char lds_raw[];
+  const int nz = sz.n, ny = sy.n, ld = nz + 1;
+  cplx<T>* bufA = reinterpret_cast<cplx<T>*>(lds_raw);
+  cplx<T>* bufB = bufA + ny * ld;
+  cplx<T>* twz = bufB + ny * ld;
+  cplx<T>* twy = twz + nz;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < nz; i += kPlaneThreads) twz[i] = twz_g[i];
+  for (int i = tid; i < ny; i += kPlaneThreads) twy[i] = twy_g[i];
+  cplx<T>* plane = data + static_cast<long>(blockIdx.x) * ny * nz;
+  for (int t = tid; t < ny * nz; t += kPlaneThreads) {
+    const int y = t / nz, z = t - y * nz;
+    bufA[y * ld + z] = plane[t];
+  }
+  __syncthreads();
+  cplx<T>* a = stockham_2d<T, false>(bufA, bufB, twz, sz, ld, 1, ny, sign, tid, kPlaneThreads);
+  cplx<T>* b = a == bufA ? bufB : bufA;
+  a = stockham_2d<T, true>(a, b, twy, sy, 1, ld, nz, sign, tid, kPlaneThreads);
+  for (int t = tid; t < ny * nz; t += kPlaneThreads) {
+    const int y = t / nz, z = t - y * nz;
+    plane[t] = a[y * ld + z];
+  }
+}
+
+bool small_radices(const FftStages& st) {
+  for (int s = 0; s < st.nstages; ++s)
+    if (st.radix[s] > 5) return false;
+  return true;
+}
+
+// LDS of the plane kernel: two padded planes and both twiddle tables
+template <typename T>
+size_t plane_lds(int ny, int nz) {
+  return (2 * static_cast<size_t>(ny) * (nz + 1) + ny + nz) * sizeof(cplx<T>);
+}
+// the fused z/y pass is used when two planes fit in half of the 160 KB LDS
+// (at least two blocks of 8 waves per CU)
+constexpr size_t kPlaneLdsMax = 80 * 1024;
 
 // One block transforms a tile of W lines of length n.
 //   STRIDED = false: line l occupies data[l*n .. l*n+n); the tile is lines
@@ -297,9 +402,18 @@ template <typename T>
 void Fft3dPlan::execute(cplx<T>* data, long batch, int sign, hipStream_t stream) const {
   throw_if(sign != -1 && sign != 1, "FFT sign must be -1 or +1");
   const long nxl = nx, nyl = ny, nzl = nz;
-  launch_axis<T>(data, pz, sign, batch * nxl * nyl, 1, false, stream);   // z: contiguous lines
-  launch_axis<T>(data, py, sign, batch * nxl, nzl, true, stream);        // y: stride nz
-  launch_axis<T>(data, px, sign, batch, nyl * nzl, true, stream);        // x: stride ny*nz
+  const long planes = batch * nxl;
+  if (ny > 1 && nz > 1 && small_radices(py.st) && small_radices(pz.st) && plane_lds<T>(ny, nz) <= kPlaneLdsMax &&
+      planes <= 0x7fffffffL) {
+    // z and y in one pass over each contiguous (y, z) plane
+    hipLaunchKernelGGL((fft_plane_kernel<T>), dim3(static_cast<unsigned>(planes)), dim3(kPlaneThreads),
+                       plane_lds<T>(ny, nz), stream, data, pz.st, py.st, pz.twiddles<T>(), py.twiddles<T>(), sign);
+    MDFT_HIP_CHECK(hipGetLastError());
+  } else {
+    launch_axis<T>(data, pz, sign, batch * nxl * nyl, 1, false, stream);  // z: contiguous lines
+    launch_axis<T>(data, py, sign, batch * nxl, nzl, true, stream);       // y: stride nz
+  }
+  launch_axis<T>(data, px, sign, batch, nyl * nzl, true, stream);  // x: stride ny*nz
 }
 
 template void Fft3dPlan::execute<double>(cplx<double>*, long, int, hipStream_t) const;
