@@ -334,7 +334,7 @@ def main():
         df = torch.empty_like(xi)
         ff = solver.energy_cproj(xi, df)
         dt, stages = timed(solver, xi, df, ffd, args.steps, args.warmup)
-        results[form] = dict(ff=ff, df=df, sec=dt, stages=stages)
+        results[form] = dict(ff=ff, df=df, sec=dt, stages=stages, fft=conv.fft_config())
         print(f"[bench] {form}: {args.steps / dt:.1f} evals/s, F={ff:.10g}, stages(ms)="
               f"{ {k: round(v, 4) for k, v in stages.items()} }", file=sys.stderr, flush=True)
         del solver, conv
@@ -424,6 +424,7 @@ def main():
                    "grad_rel_l2_vs_fp64": grad_rel3, "meets_dF_bar": abs(dF3) <= 0.01},
         "stages_ms_GBs": stages,
         "stages_fp64_ms": {k: round(v, 5) for k, v in f64["stages"].items()},
+        "fft_autotuned": {"mixed_x6": x6["fft"], "fp64": f64["fft"]},
         "p99_solve": p99,
         "served_batch": served,
         "served_stream": streamed,

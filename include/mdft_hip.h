@@ -126,6 +126,18 @@ int mdft_conv_destroy(mdft_conv_t h);
  * dtype MDFT_DOUBLE or MDFT_FLOAT. */
 int mdft_fft3d(void* data, int batch, int nx, int ny, int nz, int sign, int dtype, void* stream);
 
+/* The same transform in a given configuration (no counterpart in the
+ * reference): plane != 0 runs z and y as one pass over each (y, z) plane
+ * staged in LDS (plane_threads 256 or 512; radices 2-5; the plane must fit in
+ * 160 KB of LDS), else one pass per axis with wz / wy / wx lines per tile
+ * (0: the default).  Every configuration runs the same butterflies on the
+ * same lines: results are bitwise those of mdft_fft3d.  A Convolution tunes
+ * its configuration on first use by timing these candidates;
+ * mdft_conv_fft_config reports it (tuned = 0 before the first execute). */
+int mdft_fft3d_config(void* data, int batch, int nx, int ny, int nz, int sign, int dtype, int plane,
+                      int plane_threads, int wz, int wy, int wx, void* stream);
+int mdft_conv_fft_config(mdft_conv_t h, int* tuned, int* plane, int* plane_threads, int* wz, int* wy, int* wx);
+
 /* ---- Solvent free functions ---------------------------------------------- */
 
 /* common/MDFT_Solvent.hpp:575-596 (get_delta_rho): delta_rho = rho0 (xi^2 - 1)

@@ -50,6 +50,8 @@ _SIGS = {
     "mdft_conv_execute": (_i, [_vp, _vp, _vp]),
     "mdft_conv_destroy": (_i, [_vp]),
     "mdft_fft3d": (_i, [_vp, _i, _i, _i, _i, _i, _i, _vp]),
+    "mdft_fft3d_config": (_i, [_vp, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _vp]),
+    "mdft_conv_fft_config": (_i, [_vp, _pi, _pi, _pi, _pi, _pi, _pi]),
     "mdft_delta_rho": (_i, [_vp, _vp, _sz, _d, _i, _vp]),
     "mdft_delta_f": (_i, [_vp, _vp, _vp, _vp, _sz, _i, _d, _d, _pd, _i, _vp]),
     "mdft_solver_create": (_i, [_vp, _d, _d, _i, ctypes.POINTER(_vp)]),

@@ -333,6 +333,44 @@ int mdft_fft3d(void* data, int batch, int nx, int ny, int nz, int sign, int dtyp
   });
 }
 
+int mdft_fft3d_config(void* data, int batch, int nx, int ny, int nz, int sign, int dtype, int plane,
+                      int plane_threads, int wz, int wy, int wx, void* stream) {
+  return guarded([&] {
+    throw_if(!data || batch < 0 || nx <= 0 || ny <= 0 || nz <= 0, "fft3d: bad arguments");
+    check_dtype(dtype);
+    Fft3dPlan plan(nx, ny, nz);
+    FftConfig c;
+    c.plane = plane != 0;
+    c.plane_threads = plane_threads;
+    c.wz = wz;
+    c.wy = wy;
+    c.wx = wx;
+    hipStream_t s = as_stream(stream);
+    if (dtype == MDFT_DOUBLE)
+      plan.execute_config<double>(c, static_cast<cplx<double>*>(data), batch, sign, s);
+    else
+      plan.execute_config<float>(c, static_cast<cplx<float>*>(data), batch, sign, s);
+    MDFT_HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int mdft_conv_fft_config(mdft_conv_t h, int* tuned, int* plane, int* plane_threads, int* wz, int* wy, int* wx) {
+  return guarded([&] {
+    throw_if(!h, "null handle");
+    const Fft3dPlan& f = h->dev->fft;
+    const bool f64 = h->precision == MDFT_FP64;
+    const bool t = f64 ? f.tuned<double>() : f.tuned<float>();
+    FftConfig c = f64 ? f.default_config<double>() : f.default_config<float>();
+    if (t) c = f64 ? f.autotune<double>(0, nullptr) : f.autotune<float>(0, nullptr);  // the cached choice
+    if (tuned) *tuned = t ? 1 : 0;
+    if (plane) *plane = c.plane ? 1 : 0;
+    if (plane_threads) *plane_threads = c.plane_threads;
+    if (wz) *wz = c.wz;
+    if (wy) *wy = c.wy;
+    if (wx) *wx = c.wx;
+  });
+}
+
 // ---------------------------------------------------------------------------
 int mdft_delta_rho(const void* xi, void* delta_rho, size_t n, double rho0, int dtype, void* stream) {
   return guarded([&] {

@@ -902,6 +902,7 @@ void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch)
 template <typename T>
 void ConvDevice::execute(cplx<T>* p, double chop_eps, hipStream_t s, StageTimer* timer, int nbatch) const {
   const long batch = static_cast<long>(tab.ang.np) * nbatch;
+  if (!fft.tuned<T>()) fft.autotune<T>(batch, s);  // once per element type, before any timing mark
   if (timer) timer->mark(s, kStageConvStart);
   fft.execute<T>(p, batch, -1, s);
   if (timer) timer->mark(s, kStageFftFwd);

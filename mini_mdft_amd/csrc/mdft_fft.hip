@@ -14,7 +14,10 @@
 // host table.
 #include "mdft_fft.hpp"
 
+#include <algorithm>
 #include <cmath>
+#include <type_traits>
+#include <utility>
 
 namespace mdft {
 
@@ -206,18 +209,19 @@ __global__ __launch_bounds__(kPlaneThreads) void fft_plane_kernel(cplx<T>* __res
   cplx<T>* twz = bufB + ny * ld;
   cplx<T>* twy = twz + nz;
   const int tid = threadIdx.x;
-  for (int i = tid; i < nz; i += kPlaneThreads) twz[i] = twz_g[i];
-  for (int i = tid; i < ny; i += kPlaneThreads) twy[i] = twy_g[i];
+  const int nth = blockDim.x;
+  for (int i = tid; i < nz; i += nth) twz[i] = twz_g[i];
+  for (int i = tid; i < ny; i += nth) twy[i] = twy_g[i];
   cplx<T>* plane = data + static_cast<long>(blockIdx.x) * ny * nz;
-  for (int t = tid; t < ny * nz; t += kPlaneThreads) {
+  for (int t = tid; t < ny * nz; t += nth) {
     const int y = t / nz, z = t - y * nz;
     bufA[y * ld + z] = plane[t];
   }
   __syncthreads();
-  cplx<T>* a = stockham_2d<T, false>(bufA, bufB, twz, sz, ld, 1, ny, sign, tid, kPlaneThreads);
+  cplx<T>* a = stockham_2d<T, false>(bufA, bufB, twz, sz, ld, 1, ny, sign, tid, nth);
   cplx<T>* b = a == bufA ? bufB : bufA;
-  a = stockham_2d<T, true>(a, b, twy, sy, 1, ld, nz, sign, tid, kPlaneThreads);
-  for (int t = tid; t < ny * nz; t += kPlaneThreads) {
+  a = stockham_2d<T, true>(a, b, twy, sy, 1, ld, nz, sign, tid, nth);
+  for (int t = tid; t < ny * nz; t += nth) {
     const int y = t / nz, z = t - y * nz;
     plane[t] = a[y * ld + z];
   }
@@ -234,9 +238,11 @@ template <typename T>
 size_t plane_lds(int ny, int nz) {
   return (2 * static_cast<size_t>(ny) * (nz + 1) + ny + nz) * sizeof(cplx<T>);
 }
-// the fused z/y pass is used when two planes fit in half of the 160 KB LDS
-// (at least two blocks of 8 waves per CU)
+// the fused z/y pass is used untuned when two planes fit in half of the
+// 160 KB LDS (at least two blocks of 8 waves per CU); the tuner also tries it
+// up to the whole LDS
 constexpr size_t kPlaneLdsMax = 80 * 1024;
+constexpr size_t kLdsMax = 160 * 1024;
 
 // One block transforms a tile of W lines of length n.
 //   STRIDED = false: line l occupies data[l*n .. l*n+n); the tile is lines
@@ -329,13 +335,19 @@ int choose_tile_lines(int n) {
 }
 
 template <typename T>
+size_t axis_lds(int n, int W) {
+  return (2 * static_cast<size_t>(W) * (n + 1) + n) * sizeof(cplx<T>);
+}
+
+// W = 0: the default tile (1024 / n lines)
+template <typename T>
 void launch_axis(cplx<T>* data, const FftPlan1D& p, int sign, long nlines_or_outer, long S, bool strided,
-                 hipStream_t stream) {
+                 hipStream_t stream, int W = 0) {
   if (p.st.n == 1 || nlines_or_outer == 0) return;
-  int W = choose_tile_lines(p.st.n);
+  if (W <= 0) W = choose_tile_lines(p.st.n);
   if (strided && S < W) W = static_cast<int>(S);
-  const size_t lds = (2 * static_cast<size_t>(W) * (p.st.n + 1) + p.st.n) * sizeof(cplx<T>);
-  throw_if(lds > 160 * 1024, "FFT tile does not fit in LDS");
+  const size_t lds = axis_lds<T>(p.st.n, W);
+  throw_if(lds > kLdsMax, "FFT tile does not fit in LDS");
   const cplx<T>* tw = p.twiddles<T>();
   if (!strided) {
     const long nblk = (nlines_or_outer + W - 1) / W;
@@ -398,25 +410,144 @@ const cplx<float>* FftPlan1D::twiddles<float>() const {
 
 Fft3dPlan::Fft3dPlan(int nx_, int ny_, int nz_) : nx(nx_), ny(ny_), nz(nz_), px(nx_), py(ny_), pz(nz_) {}
 
+bool Fft3dPlan::plane_ok(size_t elem, int threads) const {
+  (void)threads;
+  return ny > 1 && nz > 1 && small_radices(py.st) && small_radices(pz.st) &&
+         (2 * static_cast<size_t>(ny) * (nz + 1) + ny + nz) * elem <= kLdsMax;
+}
+
 template <typename T>
-void Fft3dPlan::execute(cplx<T>* data, long batch, int sign, hipStream_t stream) const {
+void Fft3dPlan::execute_config(const FftConfig& c, cplx<T>* data, long batch, int sign, hipStream_t stream) const {
   throw_if(sign != -1 && sign != 1, "FFT sign must be -1 or +1");
   const long nxl = nx, nyl = ny, nzl = nz;
   const long planes = batch * nxl;
-  if (ny > 1 && nz > 1 && small_radices(py.st) && small_radices(pz.st) && plane_lds<T>(ny, nz) <= kPlaneLdsMax &&
-      planes <= 0x7fffffffL) {
+  if (c.plane) {
+    throw_if(!plane_ok(sizeof(cplx<T>), c.plane_threads) || planes > 0x7fffffffL ||
+                 (c.plane_threads != 256 && c.plane_threads != 512),
+             "FFT: the fused plane pass does not apply to this shape");
     // z and y in one pass over each contiguous (y, z) plane
-    hipLaunchKernelGGL((fft_plane_kernel<T>), dim3(static_cast<unsigned>(planes)), dim3(kPlaneThreads),
+    hipLaunchKernelGGL((fft_plane_kernel<T>), dim3(static_cast<unsigned>(planes)), dim3(c.plane_threads),
                        plane_lds<T>(ny, nz), stream, data, pz.st, py.st, pz.twiddles<T>(), py.twiddles<T>(), sign);
     MDFT_HIP_CHECK(hipGetLastError());
   } else {
-    launch_axis<T>(data, pz, sign, batch * nxl * nyl, 1, false, stream);  // z: contiguous lines
-    launch_axis<T>(data, py, sign, batch * nxl, nzl, true, stream);       // y: stride nz
+    launch_axis<T>(data, pz, sign, batch * nxl * nyl, 1, false, stream, c.wz);  // z: contiguous lines
+    launch_axis<T>(data, py, sign, batch * nxl, nzl, true, stream, c.wy);       // y: stride nz
   }
-  launch_axis<T>(data, px, sign, batch, nyl * nzl, true, stream);  // x: stride ny*nz
+  launch_axis<T>(data, px, sign, batch, nyl * nzl, true, stream, c.wx);  // x: stride ny*nz
+}
+
+template <typename T>
+FftConfig Fft3dPlan::default_config() const {
+  FftConfig c;
+  c.plane = plane_ok(sizeof(cplx<T>), 512) && plane_lds<T>(ny, nz) <= kPlaneLdsMax;
+  c.plane_threads = 512;
+  return c;
+}
+
+template <typename T>
+void Fft3dPlan::execute(cplx<T>* data, long batch, int sign, hipStream_t stream) const {
+  const int k = std::is_same<T, double>::value ? 1 : 0;
+  execute_config<T>(tuned_[k] ? cfg_[k] : default_config<T>(), data, batch, sign, stream);
+}
+
+// Times the candidate configurations on a scratch array of `batch`
+// transforms and keeps the fastest: the tile width of each per-axis pass
+// (256, 512, 1024, 2048 values per tile) and, where a (y, z) plane fits in
+// LDS, the fused z/y pass with 256 or 512 threads.  Every candidate runs the
+// same Stockham stages on the same lines, so the choice changes timing only.
+template <typename T>
+FftConfig Fft3dPlan::autotune(long batch, hipStream_t s) const {
+  const int k = std::is_same<T, double>::value ? 1 : 0;
+  if (tuned_[k]) return cfg_[k];
+  const long N = static_cast<long>(nx) * ny * nz;
+  batch = std::max<long>(1, batch);
+  cplx<T>* buf = nullptr;
+  MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&buf), batch * N * sizeof(cplx<T>)));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  FftConfig best = default_config<T>();
+  try {
+    MDFT_HIP_CHECK(hipMemsetAsync(buf, 0, batch * N * sizeof(cplx<T>), s));
+    MDFT_HIP_CHECK(hipEventCreate(&e0));
+    MDFT_HIP_CHECK(hipEventCreate(&e1));
+    auto time_ms = [&](auto&& run) {
+      run();
+      run();
+      float best_ms = 1e30f;
+      for (int r = 0; r < 3; ++r) {
+        MDFT_HIP_CHECK(hipEventRecord(e0, s));
+        run();
+        MDFT_HIP_CHECK(hipEventRecord(e1, s));
+        MDFT_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        MDFT_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        best_ms = std::min(best_ms, ms);
+      }
+      return best_ms;
+    };
+    const long nxl = nx, nyl = ny, nzl = nz;
+    // per-axis tile widths
+    auto best_w = [&](const FftPlan1D& p, long lines, long S, bool strided) {
+      int bw = 0;
+      float bt = 1e30f;
+      for (int vals : {256, 512, 1024, 2048}) {
+        int W = std::max(1, std::min(64, vals / std::max(1, p.st.n)));
+        if (strided && S < W) W = static_cast<int>(S);
+        if (axis_lds<T>(p.st.n, W) > kLdsMax) continue;
+        const float t = time_ms([&] { launch_axis<T>(buf, p, -1, lines, S, strided, s, W); });
+        if (t < bt) {
+          bt = t;
+          bw = W;
+        }
+      }
+      return std::make_pair(bw, bt);
+    };
+    const auto z = best_w(pz, batch * nxl * nyl, 1, false);
+    const auto y = best_w(py, batch * nxl, nzl, true);
+    const auto x = best_w(px, batch, nyl * nzl, true);
+    FftConfig axes;
+    axes.plane = false;
+    axes.wz = z.first;
+    axes.wy = y.first;
+    axes.wx = x.first;
+    best = axes;
+    float best_t = z.second + y.second + x.second;
+    if (plane_ok(sizeof(cplx<T>), 512) && batch * nxl <= 0x7fffffffL) {
+      for (int th : {256, 512}) {
+        FftConfig c = axes;
+        c.plane = true;
+        c.plane_threads = th;
+        const float t = time_ms([&] {
+          hipLaunchKernelGGL((fft_plane_kernel<T>), dim3(static_cast<unsigned>(batch * nxl)), dim3(th),
+                             plane_lds<T>(ny, nz), s, buf, pz.st, py.st, pz.twiddles<T>(), py.twiddles<T>(), -1);
+        }) + x.second;
+        if (t < best_t) {
+          best_t = t;
+          best = c;
+        }
+      }
+    }
+  } catch (...) {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(buf);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  MDFT_HIP_CHECK(hipStreamSynchronize(s));
+  MDFT_HIP_CHECK(hipFree(buf));
+  cfg_[k] = best;
+  tuned_[k] = true;
+  return best;
 }
 
 template void Fft3dPlan::execute<double>(cplx<double>*, long, int, hipStream_t) const;
 template void Fft3dPlan::execute<float>(cplx<float>*, long, int, hipStream_t) const;
+template void Fft3dPlan::execute_config<double>(const FftConfig&, cplx<double>*, long, int, hipStream_t) const;
+template void Fft3dPlan::execute_config<float>(const FftConfig&, cplx<float>*, long, int, hipStream_t) const;
+template FftConfig Fft3dPlan::autotune<double>(long, hipStream_t) const;
+template FftConfig Fft3dPlan::autotune<float>(long, hipStream_t) const;
+template FftConfig Fft3dPlan::default_config<double>() const;
+template FftConfig Fft3dPlan::default_config<float>() const;
 
 }  // namespace mdft

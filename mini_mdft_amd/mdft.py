@@ -243,6 +243,12 @@ class Convolution:
         check(lib().mdft_conv_table(self._h, out.ctypes.data_as(_lib._pd)))
         return out
 
+    def fft_config(self) -> dict:
+        """The FFT configuration this convolution tuned on its first execute."""
+        v = [ctypes.c_int() for _ in range(6)]
+        check(lib().mdft_conv_fft_config(self._h, *(ctypes.byref(x) for x in v)))
+        return dict(zip(("tuned", "plane", "plane_threads", "wz", "wy", "wx"), (x.value for x in v)))
+
     def execute(self, deltarho_p, stream=None):
         """In place on (np, nx, ny, nz) complex (:324-548)."""
         _require(deltarho_p, self.complex_dtype, self.sg.N * self.ag.m_np, "deltarho_p")
@@ -357,6 +363,20 @@ def fft3d(data, sign: int, stream=None):
     batch = data.numel() // (nx * ny * nz)
     dt = DOUBLE if data.dtype == torch.complex128 else FLOAT
     check(lib().mdft_fft3d(ptr(data), batch, nx, ny, nz, sign, dt, stream_handle(stream)))
+
+
+def fft3d_config(data, sign: int, plane: bool, plane_threads: int = 512, wz: int = 0, wy: int = 0, wx: int = 0,
+                 stream=None):
+    """fft3d in a given configuration (fused z/y plane pass, or per-axis tiles
+    of wz/wy/wx lines); bitwise the same result as fft3d."""
+    torch = _torch()
+    if data.dtype not in (torch.complex128, torch.complex64) or not data.is_cuda or not data.is_contiguous():
+        raise MDFTError("fft3d needs a contiguous complex CUDA tensor")
+    nx, ny, nz = data.shape[-3:]
+    batch = data.numel() // (nx * ny * nz)
+    dt = DOUBLE if data.dtype == torch.complex128 else FLOAT
+    check(lib().mdft_fft3d_config(ptr(data), batch, nx, ny, nz, sign, dt, 1 if plane else 0, plane_threads, wz, wy,
+                                  wx, stream_handle(stream)))
 
 
 def fill_uniform(t, seed: int, lo: float, hi: float, offset: int = 0, stream=None):
