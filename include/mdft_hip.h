@@ -247,7 +247,8 @@ int mdft_functional_eval(mdft_functional_t h, const double* xi, double* g, doubl
  * NULL, else max_iter + 1 doubles receiving F at the start and after every
  * iteration): F[4] at the returned point, the iterations made, the
  * functional evaluations, status (0 converged, 1 max_iter reached, 2 no
- * acceptable point within 30 trials) and max |h0 g|.  The handle keeps its
+ * acceptable point within 30 trials, 4 F or g not finite at the start point:
+ * NaN or Inf in xi or V_ext) and max |h0 g|.  The handle keeps its
  * work space (3 N no doubles and 2 (m + 1) N no pair values) between calls. */
 int mdft_lbfgs_minimize(mdft_functional_t h, double* xi, int m, int max_iter, double gtol, double* F,
                         int* iterations, int* evaluations, int* status, double* residual, double* history,
@@ -275,7 +276,8 @@ int mdft_lbfgs_minimize(mdft_functional_t h, double* xi, int m, int max_iter, do
  *     different shapes run concurrently on their own streams); synchronises;
  *   poll: one finished solute per call (*got = 0 when none): F[4], the
  *     iterations and evaluations, status (0 converged, 1 max_iter, 2 no
- *     acceptable step, 3 rejected by the budget), max |h0 g|, and the seconds
+ *     acceptable step, 3 rejected by the budget, 4 F or g not finite at the
+ *     start point), max |h0 g|, and the seconds
  *     from submit to admission and to the end;
  *   use_graphs: graph replay on (default) or off, before the first step. */
 typedef struct mdft_server_s* mdft_server_t;

@@ -508,6 +508,9 @@ Server::Server(EngineFactory make, Footprint footprint, double budget_bytes, int
 long long Server::submit(const Shape& shape, const double* solute, int nsolute, const double* solvent, int nsolvent,
                          double vmax, double* out_xi) {
   throw_if(nsolute < 0 || nsolvent <= 0 || (nsolute > 0 && !solute) || !solvent, "server: bad sites");
+  for (int i = 0; i < 6 * nsolute; ++i) throw_if(!std::isfinite(solute[i]), "server: solute sites must be finite");
+  for (int i = 0; i < 6 * nsolvent; ++i) throw_if(!std::isfinite(solvent[i]), "server: solvent sites must be finite");
+  throw_if(!(vmax > 0), "server: vmax must be positive (inf: no cap)");
   Request r;
   r.id = next_id_++;
   r.shape = shape;

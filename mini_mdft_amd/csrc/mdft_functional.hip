@@ -439,6 +439,15 @@ bool LbfgsState::next(Trial& tr) {
     tr = cur_;
     return true;
   }
+  if (it_ == 0 && !(std::isfinite(F_[0]) && std::isfinite(gmax_) && std::isfinite(ghg_))) {
+    // a start point where F or g is not finite (NaN or Inf in xi or V_ext)
+    r_.status = 4;
+    r_.iterations = 0;
+    r_.residual = gmax_;
+    std::memcpy(r_.F, F_, sizeof(F_));
+    phase_ = kDone;
+    return false;
+  }
   if (gmax_ < o_.gtol || it_ == o_.max_iter) {
     r_.status = gmax_ < o_.gtol ? 0 : 1;
     r_.iterations = it_;

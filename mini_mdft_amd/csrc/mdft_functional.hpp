@@ -40,7 +40,9 @@ constexpr int kLbfgsMaxPairs = 8;
 struct LbfgsResult {
   double F[4] = {0, 0, 0, 0};  // F, F_id, F_ext, F_exc at the returned point
   int iterations = 0, evaluations = 0;
-  int status = 1;  // 0 converged, 1 max_iter reached, 2 no Armijo point found
+  // 0 converged, 1 max_iter reached, 2 no acceptable step, 4 F or g not
+  // finite at the start point
+  int status = 1;
   double residual = 0.0;  // max |h0 g| at the returned point
   std::vector<double> history;  // F of the start point, then after every iteration
 };

@@ -421,7 +421,7 @@ class Functional:
     common/MDFT_System.hpp:188-189).  The excess term is the solver's c-proj
     evaluation in its precision form (MDFT_FP64 or a mixed form)."""
 
-    STATUS = {0: "converged", 1: "max_iter", 2: "line search"}
+    STATUS = {0: "converged", 1: "max_iter", 2: "line search", 4: "non-finite start"}
 
     def __init__(self, solver: Solver, vext_tensor, kT_: float):
         torch = _torch()
@@ -469,7 +469,7 @@ class Server:
     pass of every stage, replayed as one HIP graph.  `table` is the c(q)
     (mini_mdft_amd.luc.LucTable) for every mmax that will be requested."""
 
-    STATUS = {0: "converged", 1: "max_iter", 2: "line search", 3: "rejected (HBM budget)"}
+    STATUS = {0: "converged", 1: "max_iter", 2: "line search", 3: "rejected (HBM budget)", 4: "non-finite start"}
 
     def __init__(self, table, mrso: int, n0: float, temperature: float, precision: int = MIXED_X6,
                  capacity: int = 4, m: int = 5, max_iter: int = 400, gtol: float = 1e-5,

@@ -393,7 +393,7 @@ def lbfgs(fg, x0: np.ndarray, h0: np.ndarray, m: int = 5, max_iter: int = 200, g
     not given.  Converged when max |h0 g| < gtol.  Returns (x, F, g,
     iterations, evaluations, F history, status) with status 0 = converged,
     1 = max_iter iterations made, 2 = no acceptable point (x is the last
-    accepted point)."""
+    accepted point), 4 = F or g not finite at x0."""
     def call(x):
         r = fg(x)
         return (r[0], r[1], abs(r[0])) if len(r) == 2 else r
@@ -402,6 +402,8 @@ def lbfgs(fg, x0: np.ndarray, h0: np.ndarray, m: int = 5, max_iter: int = 200, g
     F, g, scale = call(x)
     S, Y = [], []
     hist, nev = [F], 1
+    if not (np.isfinite(F) and np.all(np.isfinite(g))):
+        return x, F, g, 0, nev, hist, 4
     for it in range(max_iter):
         if np.abs(h0 * g).max() < gtol:
             return x, F, g, it, nev, hist, 0

@@ -407,3 +407,36 @@ def test_gpu_functional_errors(oracle):
             fun.minimize(xi, m=m)
     with pytest.raises(M.MDFTError, match="bad sites"):
         M.vext(sg, ag, np.zeros((1, 6)), np.zeros((0, 6)), 1.0)
+
+
+def test_lbfgs_nonfinite_start_ends_at_once(oracle):
+    s, w, conv, V = _setup(oracle)
+    h0 = np.broadcast_to(1.0 / (2 * RHO0 * DV * w * KT), (NPTS, s["no"])).ravel()
+    def fg(x):
+        (F, *parts), g = oracle.functional(conv, x.reshape(NPTS, -1), V, w, RHO0, PREF, KT, DV)
+        return F, g.ravel(), sum(abs(p) for p in parts)
+    x0 = np.ones(NPTS * s["no"])
+    x0[123] = np.nan
+    res = oracle.lbfgs(fg, x0, h0)
+    assert res[6] == 4 and res[3] == 0 and res[4] == 1
+
+
+@pytest.mark.gpu
+def test_gpu_lbfgs_nonfinite_start_ends_at_once(oracle):
+    """A NaN injected into xi: the solve ends after the start evaluation with
+    status 4 (no line search on NaN); the server refuses such inputs up front."""
+    P = _gpu_problem(oracle)
+    torch = P["torch"]
+    xi = torch.ones((P["N"], P["ag"].m_no), dtype=torch.float64, device="cuda")
+    xi[5, 7] = float("nan")
+    r = P["fun"].minimize(xi, m=5, max_iter=50, gtol=1e-6)
+    assert r["status"] == 4 and r["iterations"] == 0 and r["evaluations"] == 1
+    # the server refuses non-finite sites and a NaN cap at submission
+    tab = luc.model_c_symmetric(MMAX, MRSO, NQ + 4, seed=5)
+    srv = M.Server(tab, MRSO, N0, TEMP, M.FP64, capacity=1)
+    bad = solute.synthetic_solute(2, 3, L3, radius=2.0, charge_scale=0.1)
+    bad[1, 0] = float("nan")
+    with pytest.raises(M.MDFTError, match="finite"):
+        srv.submit(N3, L3, MMAX, bad, solute.SOLVENT_SITES["tip3p"], 30 * KT)
+    with pytest.raises(M.MDFTError, match="vmax"):
+        srv.submit(N3, L3, MMAX, bad[:1], solute.SOLVENT_SITES["tip3p"], float("nan"))
