@@ -436,6 +436,61 @@ void test_functional(const std::string& path, int np_luc) {
   expect_throw([&] { fun.minimize(xd, 9, 5, 1e-5); }, "m must be in 1 .. 8");
 }
 
+
+// The server through the facade: the first solute alone and in a batch of
+// three give the same F, iteration count and final xi bit for bit.
+void test_server(const std::string& path, int np_luc) {
+  const int n = 8;
+  const double L = 10.0;
+  MDFT::SpatialGrid<execution_space, double> grid({n, n, n}, {L, L, L});
+  MDFT::AngularGrid<execution_space, double> ag(3, 2);
+  const double T = 298.15, n0 = 0.0332891, kT = 1.38064852e-23 * 6.02214129e23 * T * 1e-3;
+  const double qmax = std::sqrt(3.0) * M_PI * n / L;
+  LucFile f = read_luc(path, np_luc, qmax);
+  // the exchange-symmetric definite part of the file's table: (m, m, 0, 0, 0) with a negative value
+  std::vector<std::complex<double>> c(f.c.size(), 0.0);
+  for (int i = 0; i < np_luc; ++i) {
+    const int* t = &f.tuples[i * 5];
+    if (t[0] == t[1] && t[2] == 0 && t[3] == 0 && t[4] == 0)
+      for (int q = 0; q < f.nq; ++q) c[i * f.nq + q] = -std::abs(f.c[i * f.nq + q].real()) / (1.0 + t[0]);
+  }
+  const std::vector<MDFT::Site> water{{-0.834, 3.15061, 0.636386, 0, 0, 0},
+                                      {0.417, 0, 0, 0.756950, 0, 0.585882},
+                                      {0.417, 0, 0, -0.756950, 0, 0.585882}};
+  const std::vector<std::vector<MDFT::Site>> sol{{{0.05, 3.0, 0.4, 5.1, 4.9, 5.2}, {-0.05, 2.8, 0.3, 4.0, 5.5, 5.0}},
+                                                 {{0.08, 3.3, 0.5, 5.0, 5.0, 5.0}},
+                                                 {{-0.06, 2.6, 0.2, 4.5, 5.5, 4.8}}};
+  const size_t total = static_cast<size_t>(n) * n * n * ag.m_no;
+  MDFT::View<double, 4> alone("alone", n, n, n, ag.m_no);
+  std::vector<MDFT::View<double, 4>> mix;
+  for (int k = 0; k < 3; ++k) mix.emplace_back("mix", n, n, n, ag.m_no);
+  MDFT::SolveResult ra{};
+  {
+    MDFT::Server srv(f.tuples, c, f.nq, f.dq, 2, n0, T, MDFT::Precision::FP64, 3);
+    srv.submit(grid, 3, sol[0], water, 30 * kT, alone.data());
+    ra = srv.run().at(0);
+  }
+  std::vector<MDFT::SolveResult> rm;
+  {
+    MDFT::Server srv(f.tuples, c, f.nq, f.dq, 2, n0, T, MDFT::Precision::FP64, 3);
+    for (int k = 0; k < 3; ++k) srv.submit(grid, 3, sol[k], water, 30 * kT, mix[k].data());
+    rm = srv.run();
+  }
+  g_where = "Server";
+  EXPECT_TRUE(rm.size() == 3);
+  const MDFT::SolveResult* first = nullptr;
+  for (const auto& r : rm)
+    if (r.id == 1) first = &r;
+  EXPECT_TRUE(first != nullptr);
+  std::printf("    alone: status %d, %d iterations, F %.12g; in the batch: status %d, %d iterations, F %.12g\n",
+              ra.status, ra.iterations, ra.F[0], first->status, first->iterations, first->F[0]);
+  EXPECT_TRUE(ra.status == 0 && first->status == 0);
+  EXPECT_TRUE(ra.iterations == first->iterations && ra.F[0] == first->F[0]);
+  auto ha = MDFT::create_mirror_view_and_copy(alone);
+  auto hm = MDFT::create_mirror_view_and_copy(mix[0]);
+  EXPECT_TRUE(ha.size() == total && std::memcmp(ha.data(), hm.data(), total * sizeof(double)) == 0);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -472,6 +527,7 @@ int main(int argc, char** argv) {
     run("get_delta_rho/get_delta_f<double>", [] { test_solvent_functions<double>(); });
     run("ExcessFunctional energy_cproj_mrso 8^3", [&] { test_excess_functional(path, np_luc); });
     run("Functional: V_ext, eval, minimize 8^3", [&] { test_functional(path, np_luc); });
+    run("Server: alone == in a batch 8^3", [&] { test_server(path, np_luc); });
   }
   std::printf("[==========] %d tests, %d failed\n", g_run, g_failed);
   return g_failed == 0 ? 0 : 1;
