@@ -200,10 +200,13 @@ __device__ __forceinline__ double2 phase_pow(double q0, double q1, int mup) {
 // d^l_{mup,chi}(theta_q) = Re(R^l_{mup,chi}(q) e^{+i mup phi_q}) of every pair,
 // R from the FP64 recursion, rounded to T.  Row `pair` of dtab holds the nR
 // compact entries, or with padw > 0 the padded layout of the compile-time OZ
-// kernel: (mmax+1)^2 rows (l, mup) of padw entries, chi + l in the first
-// 2l+1 and zeros after.
+// kernels: rows (l, mup) of padw entries, chi + l in the first 2l+1 and zeros
+// after; all (mmax+1)^2 rows, or with half only the (mmax+1)(mmax+2)/2 rows
+// of mup >= 0 (the rows of -mup follow from
+// d^l_{-mup,-chi} = (-1)^{mup-chi} d^l_{mup,chi}, which the values of the
+// FP64 recursion satisfy exactly).
 template <typename T>
-__global__ __launch_bounds__(kWave) void oz_dtable_kernel(T* __restrict__ dtab, OzArgs<T> a, int padw) {
+__global__ __launch_bounds__(kWave) void oz_dtable_kernel(T* __restrict__ dtab, OzArgs<T> a, int padw, int half) {
   extern __shared__ unsigned char lds_raw[];
   double2* Rd = reinterpret_cast<double2*>(lds_raw);
   const int lane = threadIdx.x;
@@ -224,12 +227,17 @@ __global__ __launch_bounds__(kWave) void oz_dtable_kernel(T* __restrict__ dtab, 
       dtab[pair * nR + it] = static_cast<T>(v.x * e.x - v.y * e.y);
     }
   } else {
-    const int nrow = (a.mmax + 1) * (a.mmax + 1);
+    const int nrow = half ? (a.mmax + 1) * (a.mmax + 2) / 2 : (a.mmax + 1) * (a.mmax + 1);
     for (int pos = lane; pos < nrow * padw; pos += kWave) {
       const int row = pos / padw, col = pos - row * padw;
-      int l = 0;
-      while ((l + 1) * (l + 1) <= row) ++l;
-      const int mup = row - l * l - l;
+      int l = 0, mup;
+      if (half) {
+        while ((l + 1) * (l + 2) / 2 <= row) ++l;
+        mup = row - l * (l + 1) / 2;
+      } else {
+        while ((l + 1) * (l + 1) <= row) ++l;
+        mup = row - l * l - l;
+      }
       double v = 0.0;
       if (col < 2 * l + 1) {
         const double2 e = phase_pow(q0, q1, mup);
@@ -500,6 +508,225 @@ __global__ __launch_bounds__(kWave) void oz_pair_ct_kernel(cplx<T>* __restrict__
       p[static_cast<long>(ip) * a.pstride + sq] = o0;
       p[static_cast<long>(ip) * a.pstride + smq] = singular ? cconj<T>(o1) : o1;
     }
+  }
+}
+
+// The compile-time OZ kernel with PB pairs per block and one thread per
+// (pair, projection): the 2 NP outputs of two pairs fill 176 of the 192
+// lanes of three waves at mmax = 5 (88 of 128 with one pair per wave and two
+// outputs per lane), and each pair keeps its own LDS region (`region`
+// bytes).  Blocks take groups of PB consecutive pairs in memory order,
+// XCD-mapped like single pairs (a.chunk counts groups).  The arithmetic of
+// every output is that of oz_pair_ct_kernel.
+template <int NP>
+constexpr int oz_pairs_per_block() {
+  return 2 * NP <= 256 ? 2 : 1;
+}
+
+template <typename T, int MMAX, int MRSO, int PB>
+__global__ __launch_bounds__(256) void oz_pairs_ct_kernel(cplx<T>* __restrict__ p, const T* __restrict__ dtab,
+                                                          OzArgs<T> a, int region) {
+  using S = OzShape<MMAX, MRSO>;
+  constexpr int W = S::W, NP = S::NP;
+  extern __shared__ unsigned char lds_raw[];
+  __shared__ cplx<T> phs[PB][W];  // e^{-i mup phi_q} of each pair
+  const int tid = threadIdx.x;
+  const int pp = tid / NP;  // pair of this thread within the block
+  const int ip = tid - pp * NP;
+  const int ppc = pp < PB ? pp : 0;
+  int job;
+  const long grp = oz_block_pair(a, job);
+  const long pair = grp * PB + pp;
+  const bool active = pp < PB && pair < a.npairs;  // uniform over the threads of one pair
+  cplx<T>* G = reinterpret_cast<cplx<T>*>(lds_raw + static_cast<long>(ppc) * region);
+  cplx<T>* Gm = G + S::NG * W;
+  cplx<T>* Dq = Gm + S::NG * W;
+  cplx<T>* Dmq = Dq + NP;
+  cplx<T>* ceff = Dmq + NP;
+  T* dP0 = reinterpret_cast<T*>(ceff + a.np_new);  // leading zero row, then ND rows
+  T* dP = dP0 + W;
+  cplx<T>* ph = phs[ppc];
+  p += static_cast<long>(job) * a.N;
+
+  int ix = 0, iy = 0, iz = 0, ixm = 0, iym = 0, izm = 0;
+  long sq = 0, smq = 0;
+  double q0 = 0, q1 = 0, q2 = 0;
+  T g0x = 0, g0y = 0, g1x = 0, g1y = 0;
+  int info = 0, ias = 0;
+  if (active) {
+    const int pk = a.pairs[pair];
+    ix = pk & 1023;
+    iy = (pk >> 10) & 1023;
+    iz = (pk >> 20) & 1023;
+    ixm = ix == 0 ? 0 : a.nx - ix;
+    iym = iy == 0 ? 0 : a.ny - iy;
+    izm = iz == 0 ? 0 : a.nz - iz;
+    sq = (static_cast<long>(ix) * a.ny + iy) * a.nz + iz;
+    smq = (static_cast<long>(ixm) * a.ny + iym) * a.nz + izm;
+    q0 = a.kxyz[ix];
+    q1 = a.kxyz[a.nx + iy];
+    q2 = a.kxyz[a.nx + a.ny + iz];
+    // gathers (:420-425) and this thread's metadata first
+    const cplx<T> u = p[static_cast<long>(ip) * a.pstride + sq], v = p[static_cast<long>(ip) * a.pstride + smq];
+    g0x = u.x;
+    g0y = u.y;
+    g1x = v.x;
+    g1y = v.y;
+    info = a.ipinfo[ip];
+    ias = a.ia_start[ip];
+    // the pair's half d table (rows (l, mup >= 0) of W entries), loaded by its
+    // NP threads and written with the mirrored rows (l, -mup):
+    // d^l_{-mup,-chi} = (-1)^{mup-chi} d^l_{mup,chi}; padding stays zero
+    constexpr int NH = (MMAX + 1) * (MMAX + 2) / 2, NE = NH * W, NU = (NE + NP - 1) / NP;
+    const T* src = dtab + pair * NE;
+    T dreg[NU];
+#pragma unroll
+    for (int k = 0; k < NU; ++k) dreg[k] = src[min(ip + k * NP, NE - 1)];
+#pragma unroll
+    for (int k = 0; k < NU; ++k) {
+      const int e = ip + k * NP;
+      if (e < NE) {
+        const int h = e / W, c = e - h * W;
+        int l = 0;
+#pragma unroll
+        for (int t = 1; t <= MMAX; ++t) l += h >= t * (t + 1) / 2 ? 1 : 0;
+        const int mup = h - l * (l + 1) / 2;
+        const T v = dreg[k];
+        dP[(l * l + l + mup) * W + c] = v;
+        if (c <= 2 * l) {
+          dP[(l * l + l - mup) * W + (2 * l - c)] = ((mup + c + l) & 1) ? -v : v;
+        } else {
+          dP[(l * l + l - mup) * W + c] = T(0);
+        }
+      }
+    }
+    if (ip < W) dP0[ip] = T(0);
+    // zero G and Gm (the padding past 2m+1 must stay zero)
+    for (int i = ip; i < 2 * S::NG * W; i += NP) G[i] = mkc<T>(0, 0);
+    if (ip < W) {
+      const double2 e = phase_pow(q0, q1, ip - MMAX);
+      ph[ip] = mkc<T>(static_cast<T>(e.x), static_cast<T>(-e.y));
+    }
+    // c(|q|) rows iq, iq+1 lerped into LDS, kCBatch rows per thread in flight
+    const double qn = sqrt(q0 * q0 + q1 * q1 + q2 * q2);
+    const double eff = qn / a.dq;
+    int iq = static_cast<int>(eff);
+    const double alpha = eff - static_cast<double>(iq);
+    if (iq > a.nq - 2) iq = a.nq - 2;  // unreachable for |q| <= qmax (nq has 1.5 rows of margin)
+    const T al = static_cast<T>(alpha), bl = static_cast<T>(1.0 - alpha);
+    const cplx<T>* c0 = a.ctab + static_cast<long>(iq) * a.np_new;
+    const cplx<T>* c1 = c0 + a.np_new;
+    for (int ib = ip; ib < a.np_new; ib += kCBatch * NP) {
+      T lx[kCBatch], ly[kCBatch], hx[kCBatch], hy[kCBatch];
+#pragma unroll
+      for (int k = 0; k < kCBatch; ++k) {
+        const int ia = min(ib + k * NP, a.np_new - 1);
+        const cplx<T> lo = c0[ia], hi = c1[ia];
+        lx[k] = lo.x;
+        ly[k] = lo.y;
+        hx[k] = hi.x;
+        hy[k] = hi.y;
+      }
+#pragma unroll
+      for (int k = 0; k < kCBatch; ++k) {
+        const int ia = ib + k * NP;
+        if (ia < a.np_new) ceff[ia] = mkc<T>(al * hx[k] + bl * lx[k], al * hy[k] + bl * ly[k]);
+      }
+    }
+  }
+  __syncthreads();
+
+  const int m = info & 0xff, mupm = (info >> 8) & 0xff, mu2 = info >> 16;
+  const int chi = mupm - MMAX;  // the same field read as chi in the q frame
+  const int row = grow_rt<MMAX, MRSO>(m) + mu2;
+  // phased gathers g' = g e^{-i mup phi} into G[(m, mu2)][mup + m]
+  if (active) {
+    const cplx<T> e = ph[mupm];
+    const int at = row * W + mupm - MMAX + m;
+    G[at] = cmul<T>(mkc<T>(g0x, g0y), e);
+    Gm[at] = cmul<T>(mkc<T>(g1x, g1y), e);
+  }
+  __syncthreads();
+
+  // rotation to the q frame (:427-443)
+  if (active) {
+    const cplx<T>* g0 = G + row * W;
+    const cplx<T>* g1 = Gm + row * W;
+    const T* d0 = dP + m * m * W + (chi + m);
+    const T* d1 = dP + m * m * W + (m - chi);
+    T ar = 0, ai = 0, br = 0, bi = 0;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const cplx<T> x = g0[j], y = g1[j];
+      const T u = d0[j * W], v = d1[j * W];
+      ar = fma(x.x, u, ar);
+      ai = fma(x.y, u, ai);
+      br = fma(y.x, v, br);
+      bi = fma(y.y, v, bi);
+    }
+    Dq[ip] = mkc<T>(ar, ai);
+    Dmq[ip] = (m & 1) ? mkc<T>(-br, -bi) : mkc<T>(br, bi);
+  }
+  __syncthreads();
+
+  // OZ product (:468-496) and chop (:514-517)
+  if (active) {
+    const int achi = chi < 0 ? -chi : chi;
+    const cplx<T>* ce = ceff + (ias - tofs_rt<MMAX, MRSO>(achi));
+    cplx<T> s0 = mkc<T>(0, 0), s1 = mkc<T>(0, 0);
+#pragma unroll
+    for (int n = 0; n <= MMAX; ++n) {
+      if (n >= achi) {
+        const int h = n / MRSO, kn = n / MRSO + 1;
+        const cplx<T>* cn = ce + S::tofs(n);
+        const int srcb = S::mbase(n) + (chi + n) * kn;
+#pragma unroll
+        for (int k = 0; k <= 2 * h; ++k) {
+          const int nu2 = k - h;
+          const cplx<T> c = cn[k];
+          if (nu2 < 0) {
+            const int src = srcb - nu2;
+            s0 = cfma<T>(s0, c, Dq[src]);
+            s1 = cfma<T>(s1, c, Dmq[src]);
+          } else {
+            const int src = srcb + nu2;
+            s0 = cfma_conjb<T>(s0, c, Dmq[src]);
+            s1 = cfma_conjb<T>(s1, c, Dq[src]);
+          }
+        }
+      }
+    }
+    const int at = row * W + chi + m;
+    G[at] = chop<T>(s0, a.chop_eps);
+    Gm[at] = chop<T>(s1, a.chop_eps);
+  }
+  __syncthreads();
+
+  // inverse rotation and stores (:518-541)
+  if (active) {
+    const bool self = (ixm == ix && iym == iy && izm == iz);
+    const bool singular = self && (ix == a.nx / 2 || iy == a.ny / 2 || iz == a.nz / 2);
+    const int mup = mupm - MMAX;
+    const cplx<T>* g0 = G + row * W;
+    const cplx<T>* g1 = Gm + row * W;
+    const T* dr = dP + (m * m + mup + m) * W;
+    const T* drm = dr + 2 * m;  // d_{mup,-chi} = drm[-j]; reads past the row start meet gamma's zeros
+    T ar = 0, ai = 0, br = 0, bi = 0;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const cplx<T> x = g0[j], y = g1[j];
+      const T u = dr[j], v = drm[-j];
+      ar = fma(x.x, u, ar);
+      ai = fma(x.y, u, ai);
+      br = fma(y.x, v, br);
+      bi = fma(y.y, v, bi);
+    }
+    const cplx<T> e = ph[mupm];  // e^{-i mup phi}; the output takes its conjugate
+    const cplx<T> o0 = cmul<T>(mkc<T>(ar, ai), cconj<T>(e));
+    cplx<T> o1 = cmul<T>(mkc<T>(br, bi), cconj<T>(e));
+    if (m & 1) o1 = mkc<T>(-o1.x, -o1.y);
+    p[static_cast<long>(ip) * a.pstride + sq] = o0;
+    p[static_cast<long>(ip) * a.pstride + smq] = singular ? cconj<T>(o1) : o1;
   }
 }
 
@@ -858,11 +1085,15 @@ const T* ConvDevice::dtable(hipStream_t s) const {
   void*& slot = std::is_same<T, double>::value ? dtab64 : dtab32;
   if (!slot) {
     const size_t nR = static_cast<size_t>(tab.mmax + 1) * (4 * (tab.mmax + 1) * (tab.mmax + 1) - 1) / 3;
-    const size_t row = padw ? static_cast<size_t>(tab.mmax + 1) * (tab.mmax + 1) * padw : nR;
+    // the FP32 compile-time kernel reads half tables (rows mup >= 0)
+    const int half = padw && std::is_same<T, float>::value ? 1 : 0;
+    const size_t rows = half ? static_cast<size_t>(tab.mmax + 1) * (tab.mmax + 2) / 2
+                             : static_cast<size_t>(tab.mmax + 1) * (tab.mmax + 1);
+    const size_t row = padw ? rows * padw : nR;
     MDFT_HIP_CHECK(hipMalloc(&slot, std::max<size_t>(1, npairs * row * sizeof(T))));
     OzArgs<T> a = oz_args<T>(*this, 0.0);
     hipLaunchKernelGGL((oz_dtable_kernel<T>), dim3(static_cast<unsigned>(npairs)), dim3(kWave),
-                       nR * sizeof(double2), s, static_cast<T*>(slot), a, padw);
+                       nR * sizeof(double2), s, static_cast<T*>(slot), a, padw, half);
     MDFT_HIP_CHECK(hipGetLastError());
   }
   return static_cast<const T*>(slot);
@@ -879,7 +1110,9 @@ void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch)
   a.pstride = a.N * nbatch;
   const long nblk = a.chunk * kXcds * nbatch;
   throw_if(nblk > 0x7fffffffL, "too many q pairs");
-  if (padw) {
+  if (padw && std::is_same<T, double>::value) {
+    // FP64: one pair per 64-lane block (the two-pair block holds 61 KB of LDS
+    // in FP64 and measured slower)
     dispatch_oz(tab.mmax, tab.mrso, [&](auto mm, auto mr) {
       constexpr int MMAX = decltype(mm)::value, MRSO = decltype(mr)::value;
       using Sh = OzShape<MMAX, MRSO>;
@@ -888,6 +1121,27 @@ void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch)
       throw_if(lds > 64 * 1024, "angular grid too large for the OZ kernel");
       hipLaunchKernelGGL((oz_pair_ct_kernel<T, MMAX, MRSO>), dim3(static_cast<unsigned>(nblk)), dim3(kWave), lds, s,
                          p, dt, a);
+    });
+    MDFT_HIP_CHECK(hipGetLastError());
+    return;
+  }
+  if (padw) {
+    dispatch_oz(tab.mmax, tab.mrso, [&](auto mm, auto mr) {
+      constexpr int MMAX = decltype(mm)::value, MRSO = decltype(mr)::value;
+      using Sh = OzShape<MMAX, MRSO>;
+      constexpr int PB = oz_pairs_per_block<Sh::NP>();
+      const size_t pair_lds = (2 * static_cast<size_t>(Sh::NG) * Sh::W + 2 * Sh::NP + tab.np_new) * sizeof(cplx<T>) +
+                              static_cast<size_t>(Sh::W) * (Sh::ND + 1) * sizeof(T);
+      const size_t region = (pair_lds + 15) / 16 * 16;
+      throw_if(PB * region > 64 * 1024, "angular grid too large for the OZ kernel");
+      OzArgs<T> ag = a;
+      const long ngroups = (npairs + PB - 1) / PB;
+      ag.chunk = (ngroups + kXcds - 1) / kXcds;
+      const long nb = ag.chunk * kXcds * nbatch;
+      throw_if(nb > 0x7fffffffL, "too many q pairs");
+      const int threads = (PB * Sh::NP + kWave - 1) / kWave * kWave;
+      hipLaunchKernelGGL((oz_pairs_ct_kernel<T, MMAX, MRSO, PB>), dim3(static_cast<unsigned>(nb)), dim3(threads),
+                         PB * region, s, p, dt, ag, static_cast<int>(region));
     });
     MDFT_HIP_CHECK(hipGetLastError());
     return;
