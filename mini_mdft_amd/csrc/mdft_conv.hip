@@ -512,22 +512,88 @@ __global__ __launch_bounds__(kWave) void oz_pair_ct_kernel(cplx<T>* __restrict__
 }
 
 // The compile-time OZ kernel with PB pairs per block and one thread per
-// (pair, projection): the 2 NP outputs of two pairs fill 176 of the 192
-// lanes of three waves at mmax = 5 (88 of 128 with one pair per wave and two
-// outputs per lane), and each pair keeps its own LDS region (`region`
-// bytes).  Blocks take groups of PB consecutive pairs in memory order,
-// XCD-mapped like single pairs (a.chunk counts groups).  The arithmetic of
-// every output is that of oz_pair_ct_kernel.
+// Per-pair set-up values of the FP32 compile-time OZ kernel, computed once
+// per Convolution by oz_meta_kernel with the expressions of the per-pair
+// kernels: the packed (ix, iy, iz), the c(q) row iq and the interpolation
+// weights alpha and 1 - alpha rounded to T; and, in a table of its own, the
+// phases e^{-i mup phi_q} rounded to T for mup = 0..mmax (those of -mup are
+// their conjugates: phase_pow(-mup) is the conjugate of phase_pow(mup), and
+// rounding commutes with negation).
+template <typename T>
+struct alignas(16) OzPairMeta {
+  int pk, iq;
+  T al, bl;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void oz_meta_kernel(OzPairMeta<T>* __restrict__ meta, cplx<T>* __restrict__ phase,
+                                                      OzArgs<T> a) {
+  const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= a.npairs) return;
+  const int pk = a.pairs[i];
+  const int ix = pk & 1023, iy = (pk >> 10) & 1023, iz = (pk >> 20) & 1023;
+  const double q0 = a.kxyz[ix], q1 = a.kxyz[a.nx + iy], q2 = a.kxyz[a.nx + a.ny + iz];
+  const double qn = sqrt(q0 * q0 + q1 * q1 + q2 * q2);
+  const double eff = qn / a.dq;
+  int iq = static_cast<int>(eff);
+  const double alpha = eff - static_cast<double>(iq);
+  if (iq > a.nq - 2) iq = a.nq - 2;  // unreachable for |q| <= qmax (nq has 1.5 rows of margin)
+  OzPairMeta<T> m;
+  m.pk = pk;
+  m.iq = iq;
+  m.al = static_cast<T>(alpha);
+  m.bl = static_cast<T>(1.0 - alpha);
+  meta[i] = m;
+  for (int mup = 0; mup <= a.mmax; ++mup) {
+    const double2 e = phase_pow(q0, q1, mup);
+    phase[i * (a.mmax + 1) + mup] = mkc<T>(static_cast<T>(e.x), static_cast<T>(-e.y));
+  }
+}
+
+// OZ terms of all projections of a compile-time grid (np_new): output
+// (m, chi, mu2) sums the (2 (n / mrso) + 1) terms of every n >= |chi|.
+template <int MMAX, int MRSO>
+constexpr int oz_terms() {
+  using S = OzShape<MMAX, MRSO>;
+  int t = 0;
+  for (int m = 0; m <= MMAX; ++m)
+    for (int chi = -m; chi <= m; ++chi) t += (m / MRSO + 1) * (S::tofs(MMAX + 1) - S::tofs(chi < 0 ? -chi : chi));
+  return t;
+}
+
+// pairs per OZ block: two when their outputs fit 256 lanes
 template <int NP>
 constexpr int oz_pairs_per_block() {
   return 2 * NP <= 256 ? 2 : 1;
 }
 
+// element idx of a table at a 32-bit byte offset from a block-uniform base
+// (the load then takes a scalar base and a 32-bit vector offset)
+template <typename V>
+__device__ __forceinline__ V ld_off(const V* base, unsigned idx) {
+  return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + idx * static_cast<unsigned>(sizeof(V)));
+}
+
+// The OZ pass for compile-time grids in FP32: PB (q, -q) pairs per block, NP
+// threads per pair, one (pair, projection) output per thread (176 of 192
+// lanes at mmax = 5), each pair with its own LDS region (`region` bytes).
+// Blocks take groups of PB consecutive pairs in memory order, XCD-mapped like
+// oz_block_pair (a.chunk counts groups).  Every input of the block is
+// requested at once, right after the pair's set-up record: the q and -q
+// gathers of every projection, the pair's half d table (rows mup >= 0), the
+// two c(q) rows (np_new entries each, a compile-time count) and the phases.
+// The mirrored d rows (d^l_{-mup,-chi} = (-1)^{mup-chi} d^l_{mup,chi}), the
+// phases and the lerped c row then go to LDS, and the arithmetic of every
+// output is that of oz_pair_ct_kernel.
 template <typename T, int MMAX, int MRSO, int PB>
 __global__ __launch_bounds__(256) void oz_pairs_ct_kernel(cplx<T>* __restrict__ p, const T* __restrict__ dtab,
-                                                          OzArgs<T> a, int region) {
+                                                          const OzPairMeta<T>* __restrict__ meta,
+                                                          const cplx<T>* __restrict__ phase, OzArgs<T> a,
+                                                          int region) {
   using S = OzShape<MMAX, MRSO>;
-  constexpr int W = S::W, NP = S::NP;
+  constexpr int W = S::W, NP = S::NP, NC = oz_terms<MMAX, MRSO>();
+  constexpr int NH = (MMAX + 1) * (MMAX + 2) / 2, NE = NH * W;
+  constexpr int NU = (NE + NP - 1) / NP, NCU = (NC + NP - 1) / NP;
   extern __shared__ unsigned char lds_raw[];
   __shared__ cplx<T> phs[PB][W];  // e^{-i mup phi_q} of each pair
   const int tid = threadIdx.x;
@@ -543,95 +609,86 @@ __global__ __launch_bounds__(256) void oz_pairs_ct_kernel(cplx<T>* __restrict__ 
   cplx<T>* Dq = Gm + S::NG * W;
   cplx<T>* Dmq = Dq + NP;
   cplx<T>* ceff = Dmq + NP;
-  T* dP0 = reinterpret_cast<T*>(ceff + a.np_new);  // leading zero row, then ND rows
+  T* dP0 = reinterpret_cast<T*>(ceff + NC);  // leading zero row, then ND rows
   T* dP = dP0 + W;
   cplx<T>* ph = phs[ppc];
   p += static_cast<long>(job) * a.N;
 
-  int ix = 0, iy = 0, iz = 0, ixm = 0, iym = 0, izm = 0;
-  long sq = 0, smq = 0;
-  double q0 = 0, q1 = 0, q2 = 0;
+  int info = 0, ias = 0, pk = 0;
   T g0x = 0, g0y = 0, g1x = 0, g1y = 0;
-  int info = 0, ias = 0;
+  long sq = 0, smq = 0;
+  bool singular = false;
   if (active) {
-    const int pk = a.pairs[pair];
-    ix = pk & 1023;
-    iy = (pk >> 10) & 1023;
-    iz = (pk >> 20) & 1023;
-    ixm = ix == 0 ? 0 : a.nx - ix;
-    iym = iy == 0 ? 0 : a.ny - iy;
-    izm = iz == 0 ? 0 : a.nz - iz;
+    // inputs that need only the pair index: the half d table and the phases
+    const T* dgrp = dtab + grp * (PB * NE);  // block-uniform
+    T dreg[NU];
+#pragma unroll
+    for (int k = 0; k < NU; ++k) {
+      const int e = ip + k * NP;
+      dreg[k] = ld_off(dgrp, static_cast<unsigned>(pp * NE + (k + 1 < NU ? e : min(e, NE - 1))));
+    }
+    cplx<T> phv = mkc<T>(0, 0);
+    const int amup = ip < MMAX ? MMAX - ip : ip - MMAX;
+    if (ip < W) phv = phase[pair * (MMAX + 1) + amup];
+    const OzPairMeta<T> mt = meta[pair];
+    pk = mt.pk;
+    const int ix = pk & 1023, iy = (pk >> 10) & 1023, iz = (pk >> 20) & 1023;
+    const int ixm = ix == 0 ? 0 : a.nx - ix, iym = iy == 0 ? 0 : a.ny - iy, izm = iz == 0 ? 0 : a.nz - iz;
     sq = (static_cast<long>(ix) * a.ny + iy) * a.nz + iz;
     smq = (static_cast<long>(ixm) * a.ny + iym) * a.nz + izm;
-    q0 = a.kxyz[ix];
-    q1 = a.kxyz[a.nx + iy];
-    q2 = a.kxyz[a.nx + a.ny + iz];
-    // gathers (:420-425) and this thread's metadata first
+    singular = (ixm == ix && iym == iy && izm == iz) && (ix == a.nx / 2 || iy == a.ny / 2 || iz == a.nz / 2);
+    // gathers (:420-425), the two c rows and this thread's metadata
     const cplx<T> u = p[static_cast<long>(ip) * a.pstride + sq], v = p[static_cast<long>(ip) * a.pstride + smq];
+    cplx<T> lo[NCU], hi[NCU];
+    const unsigned crow = static_cast<unsigned>(mt.iq) * NC;
+#pragma unroll
+    for (int k = 0; k < NCU; ++k) {
+      const int ia = ip + k * NP;
+      const unsigned at = crow + static_cast<unsigned>(k + 1 < NCU ? ia : min(ia, NC - 1));
+      lo[k] = ld_off(a.ctab, at);
+      hi[k] = ld_off(a.ctab, at + NC);
+    }
+    info = a.ipinfo[ip];
+    ias = a.ia_start[ip];
     g0x = u.x;
     g0y = u.y;
     g1x = v.x;
     g1y = v.y;
-    info = a.ipinfo[ip];
-    ias = a.ia_start[ip];
-    // the pair's half d table (rows (l, mup >= 0) of W entries), loaded by its
-    // NP threads and written with the mirrored rows (l, -mup):
-    // d^l_{-mup,-chi} = (-1)^{mup-chi} d^l_{mup,chi}; padding stays zero
-    constexpr int NH = (MMAX + 1) * (MMAX + 2) / 2, NE = NH * W, NU = (NE + NP - 1) / NP;
-    const T* src = dtab + pair * NE;
-    T dreg[NU];
-#pragma unroll
-    for (int k = 0; k < NU; ++k) dreg[k] = src[min(ip + k * NP, NE - 1)];
+    // d rows (l, mu >= 0) and their mirrors (l, -mu); padding stays zero
+    constexpr bool ALIGNED = NP % W == 0;  // entry ip + k NP is row ip / W + k NP / W, column ip % W
+    const int h0 = ip / W, c0 = ip - h0 * W;
 #pragma unroll
     for (int k = 0; k < NU; ++k) {
       const int e = ip + k * NP;
       if (e < NE) {
-        const int h = e / W, c = e - h * W;
+        const int h = ALIGNED ? h0 + k * (NP / W) : e / W;
+        const int c = ALIGNED ? c0 : e - h * W;
         int l = 0;
 #pragma unroll
         for (int t = 1; t <= MMAX; ++t) l += h >= t * (t + 1) / 2 ? 1 : 0;
-        const int mup = h - l * (l + 1) / 2;
-        const T v = dreg[k];
-        dP[(l * l + l + mup) * W + c] = v;
+        const int mu = h - l * (l + 1) / 2;
+        const T dv = dreg[k];
+        const int base = (l * l + l) * W;
+        dP[base + mu * W + c] = dv;
         if (c <= 2 * l) {
-          dP[(l * l + l - mup) * W + (2 * l - c)] = ((mup + c + l) & 1) ? -v : v;
+          dP[base - mu * W + (2 * l - c)] = ((mu + c + l) & 1) ? -dv : dv;
         } else {
-          dP[(l * l + l - mup) * W + c] = T(0);
+          dP[base - mu * W + c] = T(0);
         }
       }
     }
-    if (ip < W) dP0[ip] = T(0);
+    if (ip < W) {
+      dP0[ip] = T(0);
+      ph[ip] = ip < MMAX ? cconj<T>(phv) : phv;
+    }
     // zero G and Gm (the padding past 2m+1 must stay zero)
     for (int i = ip; i < 2 * S::NG * W; i += NP) G[i] = mkc<T>(0, 0);
-    if (ip < W) {
-      const double2 e = phase_pow(q0, q1, ip - MMAX);
-      ph[ip] = mkc<T>(static_cast<T>(e.x), static_cast<T>(-e.y));
-    }
-    // c(|q|) rows iq, iq+1 lerped into LDS, kCBatch rows per thread in flight
-    const double qn = sqrt(q0 * q0 + q1 * q1 + q2 * q2);
-    const double eff = qn / a.dq;
-    int iq = static_cast<int>(eff);
-    const double alpha = eff - static_cast<double>(iq);
-    if (iq > a.nq - 2) iq = a.nq - 2;  // unreachable for |q| <= qmax (nq has 1.5 rows of margin)
-    const T al = static_cast<T>(alpha), bl = static_cast<T>(1.0 - alpha);
-    const cplx<T>* c0 = a.ctab + static_cast<long>(iq) * a.np_new;
-    const cplx<T>* c1 = c0 + a.np_new;
-    for (int ib = ip; ib < a.np_new; ib += kCBatch * NP) {
-      T lx[kCBatch], ly[kCBatch], hx[kCBatch], hy[kCBatch];
+    // c(|q|) rows iq, iq+1 lerped into LDS
+    const T al = mt.al, bl = mt.bl;
 #pragma unroll
-      for (int k = 0; k < kCBatch; ++k) {
-        const int ia = min(ib + k * NP, a.np_new - 1);
-        const cplx<T> lo = c0[ia], hi = c1[ia];
-        lx[k] = lo.x;
-        ly[k] = lo.y;
-        hx[k] = hi.x;
-        hy[k] = hi.y;
-      }
-#pragma unroll
-      for (int k = 0; k < kCBatch; ++k) {
-        const int ia = ib + k * NP;
-        if (ia < a.np_new) ceff[ia] = mkc<T>(al * hx[k] + bl * lx[k], al * hy[k] + bl * ly[k]);
-      }
+    for (int k = 0; k < NCU; ++k) {
+      const int ia = ip + k * NP;
+      if (ia < NC) ceff[ia] = mkc<T>(al * hi[k].x + bl * lo[k].x, al * hi[k].y + bl * lo[k].y);
     }
   }
   __syncthreads();
@@ -704,8 +761,6 @@ __global__ __launch_bounds__(256) void oz_pairs_ct_kernel(cplx<T>* __restrict__ 
 
   // inverse rotation and stores (:518-541)
   if (active) {
-    const bool self = (ixm == ix && iym == iy && izm == iz);
-    const bool singular = self && (ix == a.nx / 2 || iy == a.ny / 2 || iz == a.nz / 2);
     const int mup = mupm - MMAX;
     const cplx<T>* g0 = G + row * W;
     const cplx<T>* g1 = Gm + row * W;
@@ -1050,7 +1105,7 @@ ConvDevice::ConvDevice(int nx, int ny, int nz, double lx, double ly, double lz, 
 ConvDevice::~ConvDevice() {
   for (void* q : {static_cast<void*>(kxyz), static_cast<void*>(pairs), static_cast<void*>(ia_start),
                   static_cast<void*>(ipinfo), static_cast<void*>(ctab64), static_cast<void*>(ctab32),
-                  static_cast<void*>(rcoef), dtab64, dtab32})
+                  static_cast<void*>(rcoef), dtab64, dtab32, pmeta32, pphase32})
     if (q) (void)hipFree(q);
 }
 
@@ -1099,6 +1154,20 @@ const T* ConvDevice::dtable(hipStream_t s) const {
   return static_cast<const T*>(slot);
 }
 
+const void* ConvDevice::pair_meta(hipStream_t s, const void** phases) const {
+  if (!pmeta32) {
+    MDFT_HIP_CHECK(hipMalloc(&pmeta32, std::max<size_t>(1, npairs * sizeof(OzPairMeta<float>))));
+    MDFT_HIP_CHECK(hipMalloc(&pphase32, std::max<size_t>(1, npairs * (tab.mmax + 1) * sizeof(float2))));
+    OzArgs<float> a = oz_args<float>(*this, 0.0);
+    const unsigned nb = static_cast<unsigned>((npairs + 255) / 256);
+    hipLaunchKernelGGL((oz_meta_kernel<float>), dim3(nb), dim3(256), 0, s, static_cast<OzPairMeta<float>*>(pmeta32),
+                       static_cast<cplx<float>*>(pphase32), a);
+    MDFT_HIP_CHECK(hipGetLastError());
+  }
+  *phases = pphase32;
+  return pmeta32;
+}
+
 template <typename T>
 void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch) const {
   if (npairs == 0) return;
@@ -1125,26 +1194,33 @@ void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch)
     MDFT_HIP_CHECK(hipGetLastError());
     return;
   }
-  if (padw) {
-    dispatch_oz(tab.mmax, tab.mrso, [&](auto mm, auto mr) {
-      constexpr int MMAX = decltype(mm)::value, MRSO = decltype(mr)::value;
-      using Sh = OzShape<MMAX, MRSO>;
-      constexpr int PB = oz_pairs_per_block<Sh::NP>();
-      const size_t pair_lds = (2 * static_cast<size_t>(Sh::NG) * Sh::W + 2 * Sh::NP + tab.np_new) * sizeof(cplx<T>) +
-                              static_cast<size_t>(Sh::W) * (Sh::ND + 1) * sizeof(T);
-      const size_t region = (pair_lds + 15) / 16 * 16;
-      throw_if(PB * region > 64 * 1024, "angular grid too large for the OZ kernel");
-      OzArgs<T> ag = a;
-      const long ngroups = (npairs + PB - 1) / PB;
-      ag.chunk = (ngroups + kXcds - 1) / kXcds;
-      const long nb = ag.chunk * kXcds * nbatch;
-      throw_if(nb > 0x7fffffffL, "too many q pairs");
-      const int threads = (PB * Sh::NP + kWave - 1) / kWave * kWave;
-      hipLaunchKernelGGL((oz_pairs_ct_kernel<T, MMAX, MRSO, PB>), dim3(static_cast<unsigned>(nb)), dim3(threads),
-                         PB * region, s, p, dt, ag, static_cast<int>(region));
-    });
-    MDFT_HIP_CHECK(hipGetLastError());
-    return;
+  if constexpr (std::is_same<T, float>::value) {
+    if (padw) {
+      const cplx<T>* ph = nullptr;
+      const OzPairMeta<T>* mt = static_cast<const OzPairMeta<T>*>(pair_meta(s, reinterpret_cast<const void**>(&ph)));
+      dispatch_oz(tab.mmax, tab.mrso, [&](auto mm, auto mr) {
+        constexpr int MMAX = decltype(mm)::value, MRSO = decltype(mr)::value;
+        using Sh = OzShape<MMAX, MRSO>;
+        constexpr int PB = oz_pairs_per_block<Sh::NP>();
+        constexpr int NC = oz_terms<MMAX, MRSO>();
+        throw_if(tab.np_new != NC, "internal error: OZ term count");
+        throw_if(static_cast<double>(tab.nq) * NC * sizeof(cplx<T>) >= 4294967296.0, "c(q) table too large");
+        const size_t pair_lds = (2 * static_cast<size_t>(Sh::NG) * Sh::W + 2 * Sh::NP + NC) * sizeof(cplx<T>) +
+                                static_cast<size_t>(Sh::W) * (Sh::ND + 1) * sizeof(T);
+        const size_t region = (pair_lds + 15) / 16 * 16;
+        throw_if(PB * region > 64 * 1024, "angular grid too large for the OZ kernel");
+        OzArgs<T> ag = a;
+        const long ngroups = (npairs + PB - 1) / PB;
+        ag.chunk = (ngroups + kXcds - 1) / kXcds;
+        const long nb = ag.chunk * kXcds * nbatch;
+        throw_if(nb > 0x7fffffffL, "too many q pairs");
+        const int threads = (PB * Sh::NP + kWave - 1) / kWave * kWave;
+        hipLaunchKernelGGL((oz_pairs_ct_kernel<T, MMAX, MRSO, PB>), dim3(static_cast<unsigned>(nb)), dim3(threads),
+                           PB * region, s, p, dt, mt, ph, ag, static_cast<int>(region));
+      });
+      MDFT_HIP_CHECK(hipGetLastError());
+      return;
+    }
   }
   const size_t lds = oz_lds_bytes(sizeof(T));
   throw_if(lds > 64 * 1024, "angular grid too large for the OZ kernel");

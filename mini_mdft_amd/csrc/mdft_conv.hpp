@@ -30,6 +30,12 @@ struct ConvDevice {
   const T* dtable(hipStream_t s) const;
   mutable void* dtab64 = nullptr;
   mutable void* dtab32 = nullptr;
+  // per-pair set-up values of the FP32 compile-time OZ kernel (packed index,
+  // c(q) row, interpolation weights; and in *phases the phases of mup =
+  // 0..mmax), built on first use on stream s
+  const void* pair_meta(hipStream_t s, const void** phases) const;
+  mutable void* pmeta32 = nullptr;
+  mutable void* pphase32 = nullptr;
   // row width of the padded d layout when the compile-time OZ kernel serves
   // this (mmax, mrso) (2 mmax + 1), else 0 (compact rows, generic kernel)
   int padw = 0;
