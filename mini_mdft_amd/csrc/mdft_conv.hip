@@ -567,6 +567,40 @@ constexpr int oz_pairs_per_block() {
   return 2 * NP <= 256 ? 2 : 1;
 }
 
+// Where the half d table's entries go in LDS: entry e = ip + k NP of a
+// pair's record (row h = (l, mu >= 0) with h = l (l + 1) / 2 + mu, column c)
+// is read by thread ip into register k and written to dP[(l^2 + l + mu) W + c]
+// (bits 0-11) and to its mirror dP[(l^2 + l - mu) W + 2l - c] with the sign
+// (-1)^{mu + c - l} (bits 12-23, kDmapNeg), or, for a padding column c > 2l,
+// a zero at dP[(l^2 + l - mu) W + c] (kDmapZero).  Built once per
+// Convolution, as code[k NP + ip].
+constexpr int kDmapNeg = 1 << 24, kDmapZero = 1 << 25, kDmapValid = 1 << 26;
+
+std::vector<int> oz_dmap(int mmax, int np) {
+  const int W = 2 * mmax + 1, NE = (mmax + 1) * (mmax + 2) / 2 * W, nu = (NE + np - 1) / np;
+  std::vector<int> code(static_cast<size_t>(nu) * np, 0);
+  for (int k = 0; k < nu; ++k)
+    for (int ip = 0; ip < np; ++ip) {
+      const int e = ip + k * np;
+      if (e >= NE) continue;
+      const int h = e / W, c = e % W;
+      int l = 0;
+      while ((l + 1) * (l + 2) / 2 <= h) ++l;
+      const int mu = h - l * (l + 1) / 2;
+      const int base = (l * l + l) * W;
+      int v = (base + mu * W + c) | kDmapValid;
+      if (c <= 2 * l) {
+        v |= (base - mu * W + 2 * l - c) << 12;
+        if ((mu + c + l) & 1) v |= kDmapNeg;
+      } else {
+        v |= (base - mu * W + c) << 12;
+        v |= kDmapZero;
+      }
+      code[static_cast<size_t>(k) * np + ip] = v;
+    }
+  return code;
+}
+
 // element idx of a table at a 32-bit byte offset from a block-uniform base
 // (the load then takes a scalar base and a 32-bit vector offset)
 template <typename V>
@@ -588,8 +622,8 @@ __device__ __forceinline__ V ld_off(const V* base, unsigned idx) {
 template <typename T, int MMAX, int MRSO, int PB>
 __global__ __launch_bounds__(256) void oz_pairs_ct_kernel(cplx<T>* __restrict__ p, const T* __restrict__ dtab,
                                                           const OzPairMeta<T>* __restrict__ meta,
-                                                          const cplx<T>* __restrict__ phase, OzArgs<T> a,
-                                                          int region) {
+                                                          const cplx<T>* __restrict__ phase,
+                                                          const int* __restrict__ dmap, OzArgs<T> a, int region) {
   using S = OzShape<MMAX, MRSO>;
   constexpr int W = S::W, NP = S::NP, NC = oz_terms<MMAX, MRSO>();
   constexpr int NH = (MMAX + 1) * (MMAX + 2) / 2, NE = NH * W;
@@ -622,10 +656,12 @@ __global__ __launch_bounds__(256) void oz_pairs_ct_kernel(cplx<T>* __restrict__ 
     // inputs that need only the pair index: the half d table and the phases
     const T* dgrp = dtab + grp * (PB * NE);  // block-uniform
     T dreg[NU];
+    int dcode[NU];
 #pragma unroll
     for (int k = 0; k < NU; ++k) {
       const int e = ip + k * NP;
       dreg[k] = ld_off(dgrp, static_cast<unsigned>(pp * NE + (k + 1 < NU ? e : min(e, NE - 1))));
+      dcode[k] = dmap[k * NP + ip];
     }
     cplx<T> phv = mkc<T>(0, 0);
     const int amup = ip < MMAX ? MMAX - ip : ip - MMAX;
@@ -654,27 +690,15 @@ __global__ __launch_bounds__(256) void oz_pairs_ct_kernel(cplx<T>* __restrict__ 
     g0y = u.y;
     g1x = v.x;
     g1y = v.y;
-    // d rows (l, mu >= 0) and their mirrors (l, -mu); padding stays zero
-    constexpr bool ALIGNED = NP % W == 0;  // entry ip + k NP is row ip / W + k NP / W, column ip % W
-    const int h0 = ip / W, c0 = ip - h0 * W;
+    // d rows (l, mu >= 0) and their mirrors (l, -mu), at the LDS places
+    // the thread's map entries give; padding stays zero
 #pragma unroll
     for (int k = 0; k < NU; ++k) {
-      const int e = ip + k * NP;
-      if (e < NE) {
-        const int h = ALIGNED ? h0 + k * (NP / W) : e / W;
-        const int c = ALIGNED ? c0 : e - h * W;
-        int l = 0;
-#pragma unroll
-        for (int t = 1; t <= MMAX; ++t) l += h >= t * (t + 1) / 2 ? 1 : 0;
-        const int mu = h - l * (l + 1) / 2;
+      const int code = dcode[k];
+      if (code & kDmapValid) {
         const T dv = dreg[k];
-        const int base = (l * l + l) * W;
-        dP[base + mu * W + c] = dv;
-        if (c <= 2 * l) {
-          dP[base - mu * W + (2 * l - c)] = ((mu + c + l) & 1) ? -dv : dv;
-        } else {
-          dP[base - mu * W + c] = T(0);
-        }
+        dP[code & 0xfff] = dv;
+        dP[(code >> 12) & 0xfff] = (code & kDmapZero) ? T(0) : ((code & kDmapNeg) ? -dv : dv);
       }
     }
     if (ip < W) {
@@ -1105,7 +1129,7 @@ ConvDevice::ConvDevice(int nx, int ny, int nz, double lx, double ly, double lz, 
 ConvDevice::~ConvDevice() {
   for (void* q : {static_cast<void*>(kxyz), static_cast<void*>(pairs), static_cast<void*>(ia_start),
                   static_cast<void*>(ipinfo), static_cast<void*>(ctab64), static_cast<void*>(ctab32),
-                  static_cast<void*>(rcoef), dtab64, dtab32, pmeta32, pphase32})
+                  static_cast<void*>(rcoef), dtab64, dtab32, pmeta32, pphase32, static_cast<void*>(dmap)})
     if (q) (void)hipFree(q);
 }
 
@@ -1158,6 +1182,7 @@ const void* ConvDevice::pair_meta(hipStream_t s, const void** phases) const {
   if (!pmeta32) {
     MDFT_HIP_CHECK(hipMalloc(&pmeta32, std::max<size_t>(1, npairs * sizeof(OzPairMeta<float>))));
     MDFT_HIP_CHECK(hipMalloc(&pphase32, std::max<size_t>(1, npairs * (tab.mmax + 1) * sizeof(float2))));
+    dmap = upload(oz_dmap(tab.mmax, tab.ang.np));
     OzArgs<float> a = oz_args<float>(*this, 0.0);
     const unsigned nb = static_cast<unsigned>((npairs + 255) / 256);
     hipLaunchKernelGGL((oz_meta_kernel<float>), dim3(nb), dim3(256), 0, s, static_cast<OzPairMeta<float>*>(pmeta32),
@@ -1216,7 +1241,7 @@ void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch)
         throw_if(nb > 0x7fffffffL, "too many q pairs");
         const int threads = (PB * Sh::NP + kWave - 1) / kWave * kWave;
         hipLaunchKernelGGL((oz_pairs_ct_kernel<T, MMAX, MRSO, PB>), dim3(static_cast<unsigned>(nb)), dim3(threads),
-                           PB * region, s, p, dt, mt, ph, ag, static_cast<int>(region));
+                           PB * region, s, p, dt, mt, ph, dmap, ag, static_cast<int>(region));
       });
       MDFT_HIP_CHECK(hipGetLastError());
       return;

@@ -33,6 +33,14 @@ __device__ __forceinline__ cplx<T> twiddle(const cplx<T>* tw, int t, int sign) {
   return w;
 }
 
+// x * w with one fused and one rounded product, written out so that every
+// kernel (run-time or compile-time stages, any tiling) rounds the same way
+// whatever the compiler would contract
+template <typename T>
+__device__ __forceinline__ cplx<T> tw_mul(cplx<T> x, cplx<T> w) {
+  return mkc<T>(fma(x.x, w.x, -(x.y * w.y)), fma(x.x, w.y, x.y * w.x));
+}
+
 // in-place DFT of R values; sign -1 forward, +1 backward
 template <typename T>
 __device__ __forceinline__ void dft_small(cplx<T>* v, int R, int sign, const cplx<T>* tw, int n) {
@@ -50,6 +58,7 @@ __device__ __forceinline__ void dft_small(cplx<T>* v, int R, int sign, const cpl
     v[1] = cadd<T>(d02, wd);
     v[3] = csub<T>(d02, wd);
   } else if (R == 3) {
+#pragma clang fp contract(off)
     const T h = static_cast<T>(0.86602540378443864676);  // sqrt(3)/2
     cplx<T> a = v[0], t = cadd<T>(v[1], v[2]), u = csub<T>(v[1], v[2]);
     cplx<T> m = mkc<T>(a.x - static_cast<T>(0.5) * t.x, a.y - static_cast<T>(0.5) * t.y);
@@ -106,7 +115,7 @@ __device__ __forceinline__ void stockham_stage(const cplx<T>* src, cplx<T>* dst,
     cplx<T> v[R];
     v[0] = in[0];
 #pragma unroll
-    for (int r = 1; r < R; ++r) v[r] = cmul<T>(in[r * nb], twiddle<T>(tw, r * k * tstep, sign));
+    for (int r = 1; r < R; ++r) v[r] = tw_mul<T>(in[r * nb], twiddle<T>(tw, r * k * tstep, sign));
     dft_fixed<T, R>(v, sign, tw, n);
     cplx<T>* out = dst + w * ld + (j / Ns) * Ns * R + k;
 #pragma unroll
@@ -131,7 +140,7 @@ __device__ __forceinline__ void stockham_stage_generic(const cplx<T>* src, cplx<
     const cplx<T>* in = src + w * ld + j;
     cplx<T> acc = in[0];
     for (int r = 1; r < R; ++r) {
-      const cplx<T> x = cmul<T>(in[r * nb], twiddle<T>(tw, r * k * tstep, sign));
+      const cplx<T> x = tw_mul<T>(in[r * nb], twiddle<T>(tw, r * k * tstep, sign));
       acc = cfma<T>(acc, x, twiddle<T>(tw, ((r * s) % R) * step, sign));
     }
     dst[w * ld + (j / Ns) * Ns * R + k + s * Ns] = acc;
@@ -162,7 +171,7 @@ __device__ __forceinline__ void stockham_stage_2d(const cplx<T>* src, cplx<T>* d
     cplx<T> v[R];
     v[0] = in[0];
 #pragma unroll
-    for (int r = 1; r < R; ++r) v[r] = cmul<T>(in[r * nb * es], twiddle<T>(tw, r * k * tstep, sign));
+    for (int r = 1; r < R; ++r) v[r] = tw_mul<T>(in[r * nb * es], twiddle<T>(tw, r * k * tstep, sign));
     dft_fixed<T, R>(v, sign, tw, n);
     cplx<T>* out = dst + w * ls + ((j / Ns) * Ns * R + k) * es;
 #pragma unroll
@@ -226,6 +235,12 @@ __global__ __launch_bounds__(kPlaneThreads) void fft_plane_kernel(cplx<T>* __res
     plane[t] = a[y * ld + z];
   }
 }
+
+// one fused z/y pass over `planes` planes: the compile-time kernel for square
+// power-of-two planes up to 64 x 64, else fft_plane_kernel
+template <typename T>
+void launch_plane(cplx<T>* data, const FftPlan1D& pz, const FftPlan1D& py, long planes, int threads, int sign,
+                  hipStream_t s);
 
 bool small_radices(const FftStages& st) {
   for (int s = 0; s < st.nstages; ++s)
@@ -327,6 +342,175 @@ __global__ __launch_bounds__(kFftThreads) void fft_axis_kernel(cplx<T>* __restri
   }
 }
 
+// Compile-time kernels for power-of-two lengths 16..128 (radix 4 stages, then
+// one radix 2 when log2 n is odd: the schedule of FftPlan1D).  Each stage is
+// stockham_stage_2d with the length, strides, line count and thread count
+// known at compile time, so the index arithmetic reduces to shifts and masks
+// and the butterfly loop unrolls; the arithmetic of every butterfly (LDS
+// twiddles, run-time sign, explicitly rounded products, operation order) is
+// that of the run-time stages, so results are bitwise the same.
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+constexpr bool ct_length(int n) { return n == 16 || n == 32 || n == 64 || n == 128; }
+template <int N>
+constexpr int ct_nstages() {
+  return ilog2(N) / 2 + ilog2(N) % 2;
+}
+template <int N>
+constexpr int ct_radix(int s) {
+  return s < ilog2(N) / 2 ? 4 : 2;
+}
+
+template <typename T, int R, int N, int NS, int LS, int ES, int NLINES, int NTH, bool W_FAST>
+__device__ __forceinline__ void ct_stage_2d(const cplx<T>* src, cplx<T>* dst, const cplx<T>* tw, int sign, int tid) {
+  constexpr int NB = N / R, TSTEP = N / (NS * R), TOTAL = NLINES * NB;
+  constexpr int ITERS = (TOTAL + NTH - 1) / NTH;
+#pragma unroll
+  for (int i = 0; i < ITERS; ++i) {
+    const int beta = tid + i * NTH;
+    if (TOTAL % NTH == 0 || beta < TOTAL) {
+      int w, j;
+      if constexpr (W_FAST) {
+        j = beta / NLINES;
+        w = beta - j * NLINES;
+      } else {
+        w = beta / NB;
+        j = beta - w * NB;
+      }
+      const int k = j % NS;
+      const cplx<T>* in = src + w * LS + j * ES;
+      cplx<T> v[R];
+      v[0] = in[0];
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[r] = tw_mul<T>(in[r * NB * ES], twiddle<T>(tw, r * k * TSTEP, sign));
+      dft_fixed<T, R>(v, sign, tw, N);
+      cplx<T>* out = dst + w * LS + ((j / NS) * NS * R + k) * ES;
+#pragma unroll
+      for (int r = 0; r < R; ++r) out[r * NS * ES] = v[r];
+    }
+  }
+}
+
+// stages S.. of one axis over an LDS tile, ping-pong between a and b; the
+// result ends in a when the remaining stage count is even, else in b
+template <typename T, int N, int LS, int ES, int NLINES, int NTH, bool W_FAST, int S = 0, int NS = 1>
+__device__ __forceinline__ void ct_stockham_2d(cplx<T>* a, cplx<T>* b, const cplx<T>* tw, int sign, int tid) {
+  if constexpr (S < ct_nstages<N>()) {
+    constexpr int R = ct_radix<N>(S);
+    ct_stage_2d<T, R, N, NS, LS, ES, NLINES, NTH, W_FAST>(a, b, tw, sign, tid);
+    __syncthreads();
+    ct_stockham_2d<T, N, LS, ES, NLINES, NTH, W_FAST, S + 1, NS * R>(b, a, tw, sign, tid);
+  }
+}
+
+// fft_plane_kernel for an N x N plane with NTH threads
+template <typename T, int N, int NTH>
+__global__ __launch_bounds__(NTH) void fft_plane_ct_kernel(cplx<T>* __restrict__ data, const cplx<T>* __restrict__ twz_g,
+                                                           const cplx<T>* __restrict__ twy_g, int sign) {
+  extern __shared__ unsigned char lds_raw[];
+  constexpr int LD = N + 1, NN = N * N, ITERS = (NN + NTH - 1) / NTH;
+  cplx<T>* bufA = reinterpret_cast<cplx<T>*>(lds_raw);
+  cplx<T>* bufB = bufA + N * LD;
+  cplx<T>* twz = bufB + N * LD;
+  cplx<T>* twy = twz + N;
+  const int tid = threadIdx.x;
+  if (tid < N) {
+    twz[tid] = twz_g[tid];
+    twy[tid] = twy_g[tid];
+  }
+  cplx<T>* plane = data + static_cast<long>(blockIdx.x) * NN;
+#pragma unroll
+  for (int i = 0; i < ITERS; ++i) {
+    const int t = tid + i * NTH;
+    if (NN % NTH == 0 || t < NN) bufA[(t / N) * LD + t % N] = plane[t];
+  }
+  __syncthreads();
+  // z along the rows, then y along the columns
+  ct_stockham_2d<T, N, LD, 1, N, NTH, false>(bufA, bufB, twz, sign, tid);
+  constexpr bool z_in_a = ct_nstages<N>() % 2 == 0;
+  cplx<T>* a = z_in_a ? bufA : bufB;
+  cplx<T>* b = z_in_a ? bufB : bufA;
+  ct_stockham_2d<T, N, 1, LD, N, NTH, true>(a, b, twy, sign, tid);
+  const cplx<T>* res = bufA;  // two axes of the same stage count end where they began
+#pragma unroll
+  for (int i = 0; i < ITERS; ++i) {
+    const int t = tid + i * NTH;
+    if (NN % NTH == 0 || t < NN) plane[t] = res[(t / N) * LD + t % N];
+  }
+}
+
+// fft_axis_kernel for length N and full tiles of W lines (W N in 256..2048)
+template <typename T, bool STRIDED, int N, int W>
+__global__ __launch_bounds__(kFftThreads) void fft_axis_ct_kernel(cplx<T>* __restrict__ data,
+                                                                  const cplx<T>* __restrict__ tw_g, int sign, long S) {
+  extern __shared__ unsigned char lds_raw[];
+  constexpr int LD = N + 1, NTH = kFftThreads, ITERS = W * N / NTH;
+  static_assert(W * N % NTH == 0, "tile of whole thread rounds");
+  cplx<T>* bufA = reinterpret_cast<cplx<T>*>(lds_raw);
+  cplx<T>* bufB = bufA + W * LD;
+  cplx<T>* tw = bufB + W * LD;
+  const int tid = threadIdx.x;
+  if (tid < N) tw[tid] = tw_g[tid];
+  long base;
+  if constexpr (!STRIDED) {
+    base = static_cast<long>(blockIdx.x) * (W * N);
+#pragma unroll
+    for (int i = 0; i < ITERS; ++i) {
+      const int t = tid + i * NTH;
+      bufA[(t / N) * LD + t % N] = data[base + t];
+    }
+  } else {
+    const long nchunk = S / W;
+    const long o = blockIdx.x / nchunk;
+    base = o * N * S + (blockIdx.x - o * nchunk) * W;
+#pragma unroll
+    for (int i = 0; i < ITERS; ++i) {
+      const int t = tid + i * NTH;
+      const int e = t / W, w = t % W;
+      bufA[w * LD + e] = data[base + static_cast<long>(e) * S + w];
+    }
+  }
+  __syncthreads();
+  ct_stockham_2d<T, N, LD, 1, W, NTH, false>(bufA, bufB, tw, sign, tid);
+  const cplx<T>* res = ct_nstages<N>() % 2 == 0 ? bufA : bufB;
+  if constexpr (!STRIDED) {
+#pragma unroll
+    for (int i = 0; i < ITERS; ++i) {
+      const int t = tid + i * NTH;
+      data[base + t] = res[(t / N) * LD + t % N];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < ITERS; ++i) {
+      const int t = tid + i * NTH;
+      const int e = t / W, w = t % W;
+      data[base + static_cast<long>(e) * S + w] = res[w * LD + e];
+    }
+  }
+}
+
+// the compile-time kernel of (n, W), if any: f(integral_constant N, W)
+template <typename F>
+bool dispatch_axis_ct(int n, int W, F&& f) {
+  auto by_w = [&](auto nn) {
+    constexpr int N = decltype(nn)::value;
+    switch (W * N) {
+      case 256: f(nn, std::integral_constant<int, 256 / N>{}); return true;
+      case 512: f(nn, std::integral_constant<int, 512 / N>{}); return true;
+      case 1024: f(nn, std::integral_constant<int, 1024 / N>{}); return true;
+      case 2048: f(nn, std::integral_constant<int, 2048 / N>{}); return true;
+      default: return false;
+    }
+  };
+  if (W <= 0 || W > 64) return false;
+  switch (n) {
+    case 16: return by_w(std::integral_constant<int, 16>{});
+    case 32: return by_w(std::integral_constant<int, 32>{});
+    case 64: return by_w(std::integral_constant<int, 64>{});
+    case 128: return by_w(std::integral_constant<int, 128>{});
+    default: return false;
+  }
+}
+
 int choose_tile_lines(int n) {
   int W = 1024 / n;
   if (W < 1) W = 1;
@@ -339,6 +523,33 @@ size_t axis_lds(int n, int W) {
   return (2 * static_cast<size_t>(W) * (n + 1) + n) * sizeof(cplx<T>);
 }
 
+template <typename T>
+void launch_plane(cplx<T>* data, const FftPlan1D& pz, const FftPlan1D& py, long planes, int threads, int sign,
+                  hipStream_t s) {
+  const int n = pz.st.n;
+  const size_t lds = plane_lds<T>(py.st.n, n);
+  const dim3 grid(static_cast<unsigned>(planes));
+  if (py.st.n == n && (n == 16 || n == 32 || n == 64)) {
+    auto go = [&](auto nn) {
+      constexpr int N = decltype(nn)::value;
+      if (threads == 256) {
+        hipLaunchKernelGGL((fft_plane_ct_kernel<T, N, 256>), grid, dim3(256), lds, s, data, pz.twiddles<T>(),
+                           py.twiddles<T>(), sign);
+      } else {
+        hipLaunchKernelGGL((fft_plane_ct_kernel<T, N, 512>), grid, dim3(512), lds, s, data, pz.twiddles<T>(),
+                           py.twiddles<T>(), sign);
+      }
+    };
+    if (n == 16) go(std::integral_constant<int, 16>{});
+    if (n == 32) go(std::integral_constant<int, 32>{});
+    if (n == 64) go(std::integral_constant<int, 64>{});
+  } else {
+    hipLaunchKernelGGL((fft_plane_kernel<T>), grid, dim3(threads), lds, s, data, pz.st, py.st, pz.twiddles<T>(),
+                       py.twiddles<T>(), sign);
+  }
+  MDFT_HIP_CHECK(hipGetLastError());
+}
+
 // W = 0: the default tile (1024 / n lines)
 template <typename T>
 void launch_axis(cplx<T>* data, const FftPlan1D& p, int sign, long nlines_or_outer, long S, bool strided,
@@ -349,6 +560,25 @@ void launch_axis(cplx<T>* data, const FftPlan1D& p, int sign, long nlines_or_out
   const size_t lds = axis_lds<T>(p.st.n, W);
   throw_if(lds > kLdsMax, "FFT tile does not fit in LDS");
   const cplx<T>* tw = p.twiddles<T>();
+  const bool full = strided ? S % W == 0 : nlines_or_outer % W == 0;
+  if (full && ct_length(p.st.n)) {
+    const long nblk = strided ? nlines_or_outer * (S / W) : nlines_or_outer / W;
+    throw_if(nblk > 0x7fffffffL, "too many FFT tiles");
+    const bool done = dispatch_axis_ct(p.st.n, W, [&](auto nn, auto ww) {
+      constexpr int N = decltype(nn)::value, WW = decltype(ww)::value;
+      if (strided) {
+        hipLaunchKernelGGL((fft_axis_ct_kernel<T, true, N, WW>), dim3(static_cast<unsigned>(nblk)), dim3(kFftThreads),
+                           lds, stream, data, tw, sign, S);
+      } else {
+        hipLaunchKernelGGL((fft_axis_ct_kernel<T, false, N, WW>), dim3(static_cast<unsigned>(nblk)),
+                           dim3(kFftThreads), lds, stream, data, tw, sign, S);
+      }
+    });
+    if (done) {
+      MDFT_HIP_CHECK(hipGetLastError());
+      return;
+    }
+  }
   if (!strided) {
     const long nblk = (nlines_or_outer + W - 1) / W;
     throw_if(nblk > 0x7fffffffL, "too many FFT tiles");
@@ -426,9 +656,7 @@ void Fft3dPlan::execute_config(const FftConfig& c, cplx<T>* data, long batch, in
                  (c.plane_threads != 256 && c.plane_threads != 512),
              "FFT: the fused plane pass does not apply to this shape");
     // z and y in one pass over each contiguous (y, z) plane
-    hipLaunchKernelGGL((fft_plane_kernel<T>), dim3(static_cast<unsigned>(planes)), dim3(c.plane_threads),
-                       plane_lds<T>(ny, nz), stream, data, pz.st, py.st, pz.twiddles<T>(), py.twiddles<T>(), sign);
-    MDFT_HIP_CHECK(hipGetLastError());
+    launch_plane<T>(data, pz, py, planes, c.plane_threads, sign, stream);
   } else {
     launch_axis<T>(data, pz, sign, batch * nxl * nyl, 1, false, stream, c.wz);  // z: contiguous lines
     launch_axis<T>(data, py, sign, batch * nxl, nzl, true, stream, c.wy);       // y: stride nz
@@ -517,8 +745,7 @@ FftConfig Fft3dPlan::autotune(long batch, hipStream_t s) const {
         c.plane = true;
         c.plane_threads = th;
         const float t = time_ms([&] {
-          hipLaunchKernelGGL((fft_plane_kernel<T>), dim3(static_cast<unsigned>(batch * nxl)), dim3(th),
-                             plane_lds<T>(ny, nz), s, buf, pz.st, py.st, pz.twiddles<T>(), py.twiddles<T>(), -1);
+          launch_plane<T>(buf, pz, py, batch * nxl, th, -1, s);
         }) + x.second;
         if (t < best_t) {
           best_t = t;
