@@ -47,7 +47,7 @@ N_SIDE, BOX, MMAX, MRSO = 64, 30.0, 5, 2
 TEMP, N0 = 298.15, 0.0332891
 C_SEED, XI_SEED, XI_AMP = 2024, 12345, 0.1
 OZ_FLOP_PER_PAIR = 52344  # rotations, OZ product, phases (both members of the pair)
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "oz_pair_64c_mmax5_traffic.md")
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "oz_pairs_64c_mmax5_traffic.md")
 # the solve leg: 20 sites within 5 A of the centre, charge scale 0.1, V_ext
 # capped at 100 kT, L-BFGS with 3 pairs to max |h0 g| < 1e-5 (3 pairs take
 # as many iterations as 5 here, 80-91, and move 4 fewer pair fields per pass)
