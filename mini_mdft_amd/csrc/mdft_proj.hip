@@ -421,17 +421,30 @@ __global__ __launch_bounds__(kThreads) void delta_f_kernel(const T* __restrict__
 }
 
 // block j: out[j out_stride] = scale * sum of partials[j count .. (j + 1) count)
-__global__ __launch_bounds__(kThreads) void reduce_partials_kernel(const double* __restrict__ partials, int count,
-                                                                   double scale, double* __restrict__ out,
-                                                                   long out_stride) {
+// in a fixed order: thread t sums t, t + T, ... in four interleaved chains
+// (all four loads of a round in flight together), then the block combines
+// the threads' sums as a tree
+constexpr int kRedThreads = 1024;
+__global__ __launch_bounds__(kRedThreads) void reduce_partials_kernel(const double* __restrict__ partials, int count,
+                                                                      double scale, double* __restrict__ out,
+                                                                      long out_stride) {
   partials += static_cast<long>(blockIdx.x) * count;
   out += blockIdx.x * out_stride;
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < count; i += kThreads) acc += partials[i];
-  __shared__ double red[kThreads];
-  red[threadIdx.x] = acc;
+  constexpr int T = kRedThreads;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int i = threadIdx.x;
+  for (; i + 3 * T < count; i += 4 * T) {
+    const double v0 = partials[i], v1 = partials[i + T], v2 = partials[i + 2 * T], v3 = partials[i + 3 * T];
+    a0 += v0;
+    a1 += v1;
+    a2 += v2;
+    a3 += v3;
+  }
+  for (; i < count; i += T) a0 += partials[i];
+  __shared__ double red[T];
+  red[threadIdx.x] = (a0 + a1) + (a2 + a3);
   __syncthreads();
-  for (int s = kThreads / 2; s > 0; s >>= 1) {
+  for (int s = T / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
@@ -653,7 +666,7 @@ int delta_f(const T* xi, const T* vexc, const double* w, T* df, long nxyz, int n
 
 void reduce_partials(const double* partials, int count, double scale, double* out, hipStream_t s, int nbatch,
                      long out_stride) {
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3(nbatch), dim3(kThreads), 0, s, partials, count, scale, out,
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(nbatch), dim3(kRedThreads), 0, s, partials, count, scale, out,
                      out_stride);
   MDFT_HIP_CHECK(hipGetLastError());
 }
