@@ -237,7 +237,7 @@ __global__ __launch_bounds__(kPlaneThreads) void fft_plane_kernel(cplx<T>* __res
 }
 
 // one fused z/y pass over `planes` planes: the compile-time kernel for square
-// power-of-two planes up to 64 x 64, else fft_plane_kernel
+// planes of the compile-time lengths up to 96 x 96, else fft_plane_kernel
 template <typename T>
 void launch_plane(cplx<T>* data, const FftPlan1D& pz, const FftPlan1D& py, long planes, int threads, int sign,
                   hipStream_t s);
@@ -342,22 +342,47 @@ __global__ __launch_bounds__(kFftThreads) void fft_axis_kernel(cplx<T>* __restri
   }
 }
 
-// Compile-time kernels for power-of-two lengths 16..128 (radix 4 stages, then
-// one radix 2 when log2 n is odd: the schedule of FftPlan1D).  Each stage is
-// stockham_stage_2d with the length, strides, line count and thread count
-// known at compile time, so the index arithmetic reduces to shifts and masks
-// and the butterfly loop unrolls; the arithmetic of every butterfly (LDS
-// twiddles, run-time sign, explicitly rounded products, operation order) is
-// that of the run-time stages, so results are bitwise the same.
-constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
-constexpr bool ct_length(int n) { return n == 16 || n == 32 || n == 64 || n == 128; }
+// Compile-time kernels for the lengths the grids of the served mixes use
+// (16, 32, 48, 64, 80, 96, 128), with the radix schedule of FftPlan1D (4s
+// first, then 2, 3, 5).  Each stage is stockham_stage_2d with the length,
+// strides, line count and thread count known at compile time, so the index
+// arithmetic reduces to shifts, masks and constant divisions and the
+// butterfly loop unrolls; the arithmetic of every butterfly (LDS twiddles,
+// run-time sign, explicitly rounded products, operation order) is that of the
+// run-time stages, so results are bitwise the same.
+constexpr bool ct_length(int n) {
+  return n == 16 || n == 32 || n == 48 || n == 64 || n == 80 || n == 96 || n == 128;
+}
 template <int N>
 constexpr int ct_nstages() {
-  return ilog2(N) / 2 + ilog2(N) % 2;
+  constexpr int ps[4] = {4, 2, 3, 5};
+  int r = N, c = 0;
+  for (int p : ps)
+    while (r % p == 0) {
+      r /= p;
+      ++c;
+    }
+  return c;
 }
 template <int N>
 constexpr int ct_radix(int s) {
-  return s < ilog2(N) / 2 ? 4 : 2;
+  constexpr int ps[4] = {4, 2, 3, 5};
+  int r = N, c = 0;
+  for (int p : ps)
+    while (r % p == 0) {
+      if (c == s) return p;
+      r /= p;
+      ++c;
+    }
+  return 0;
+}
+// the tile widths the FFT tuner tries for length N (values per tile 256,
+// 512, 1024, 2048; the default is the third)
+template <int N>
+constexpr int tuner_w(int i) {
+  constexpr int vals[4] = {256, 512, 1024, 2048};
+  const int w = vals[i] / N;
+  return w < 1 ? 1 : (w > 64 ? 64 : w);
 }
 
 template <typename T, int R, int N, int NS, int LS, int ES, int NLINES, int NTH, bool W_FAST>
@@ -438,13 +463,12 @@ __global__ __launch_bounds__(NTH) void fft_plane_ct_kernel(cplx<T>* __restrict__
   }
 }
 
-// fft_axis_kernel for length N and full tiles of W lines (W N in 256..2048)
+// fft_axis_kernel for length N and full tiles of W lines (W one of the tuner's widths)
 template <typename T, bool STRIDED, int N, int W>
 __global__ __launch_bounds__(kFftThreads) void fft_axis_ct_kernel(cplx<T>* __restrict__ data,
                                                                   const cplx<T>* __restrict__ tw_g, int sign, long S) {
   extern __shared__ unsigned char lds_raw[];
-  constexpr int LD = N + 1, NTH = kFftThreads, ITERS = W * N / NTH;
-  static_assert(W * N % NTH == 0, "tile of whole thread rounds");
+  constexpr int LD = N + 1, NTH = kFftThreads, WN = W * N, ITERS = (WN + NTH - 1) / NTH;
   cplx<T>* bufA = reinterpret_cast<cplx<T>*>(lds_raw);
   cplx<T>* bufB = bufA + W * LD;
   cplx<T>* tw = bufB + W * LD;
@@ -456,7 +480,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_axis_ct_kernel(cplx<T>* __res
 #pragma unroll
     for (int i = 0; i < ITERS; ++i) {
       const int t = tid + i * NTH;
-      bufA[(t / N) * LD + t % N] = data[base + t];
+      if (WN % NTH == 0 || t < WN) bufA[(t / N) * LD + t % N] = data[base + t];
     }
   } else {
     const long nchunk = S / W;
@@ -466,7 +490,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_axis_ct_kernel(cplx<T>* __res
     for (int i = 0; i < ITERS; ++i) {
       const int t = tid + i * NTH;
       const int e = t / W, w = t % W;
-      bufA[w * LD + e] = data[base + static_cast<long>(e) * S + w];
+      if (WN % NTH == 0 || t < WN) bufA[w * LD + e] = data[base + static_cast<long>(e) * S + w];
     }
   }
   __syncthreads();
@@ -476,14 +500,14 @@ __global__ __launch_bounds__(kFftThreads) void fft_axis_ct_kernel(cplx<T>* __res
 #pragma unroll
     for (int i = 0; i < ITERS; ++i) {
       const int t = tid + i * NTH;
-      data[base + t] = res[(t / N) * LD + t % N];
+      if (WN % NTH == 0 || t < WN) data[base + t] = res[(t / N) * LD + t % N];
     }
   } else {
 #pragma unroll
     for (int i = 0; i < ITERS; ++i) {
       const int t = tid + i * NTH;
       const int e = t / W, w = t % W;
-      data[base + static_cast<long>(e) * S + w] = res[w * LD + e];
+      if (WN % NTH == 0 || t < WN) data[base + static_cast<long>(e) * S + w] = res[w * LD + e];
     }
   }
 }
@@ -493,19 +517,20 @@ template <typename F>
 bool dispatch_axis_ct(int n, int W, F&& f) {
   auto by_w = [&](auto nn) {
     constexpr int N = decltype(nn)::value;
-    switch (W * N) {
-      case 256: f(nn, std::integral_constant<int, 256 / N>{}); return true;
-      case 512: f(nn, std::integral_constant<int, 512 / N>{}); return true;
-      case 1024: f(nn, std::integral_constant<int, 1024 / N>{}); return true;
-      case 2048: f(nn, std::integral_constant<int, 2048 / N>{}); return true;
-      default: return false;
-    }
+    if (W == tuner_w<N>(0)) f(nn, std::integral_constant<int, tuner_w<N>(0)>{});
+    else if (W == tuner_w<N>(1)) f(nn, std::integral_constant<int, tuner_w<N>(1)>{});
+    else if (W == tuner_w<N>(2)) f(nn, std::integral_constant<int, tuner_w<N>(2)>{});
+    else if (W == tuner_w<N>(3)) f(nn, std::integral_constant<int, tuner_w<N>(3)>{});
+    else return false;
+    return true;
   };
-  if (W <= 0 || W > 64) return false;
   switch (n) {
     case 16: return by_w(std::integral_constant<int, 16>{});
     case 32: return by_w(std::integral_constant<int, 32>{});
+    case 48: return by_w(std::integral_constant<int, 48>{});
     case 64: return by_w(std::integral_constant<int, 64>{});
+    case 80: return by_w(std::integral_constant<int, 80>{});
+    case 96: return by_w(std::integral_constant<int, 96>{});
     case 128: return by_w(std::integral_constant<int, 128>{});
     default: return false;
   }
@@ -529,7 +554,7 @@ void launch_plane(cplx<T>* data, const FftPlan1D& pz, const FftPlan1D& py, long 
   const int n = pz.st.n;
   const size_t lds = plane_lds<T>(py.st.n, n);
   const dim3 grid(static_cast<unsigned>(planes));
-  if (py.st.n == n && (n == 16 || n == 32 || n == 64)) {
+  if (py.st.n == n && ct_length(n) && n <= 96) {
     auto go = [&](auto nn) {
       constexpr int N = decltype(nn)::value;
       if (threads == 256) {
@@ -540,9 +565,14 @@ void launch_plane(cplx<T>* data, const FftPlan1D& pz, const FftPlan1D& py, long 
                            py.twiddles<T>(), sign);
       }
     };
-    if (n == 16) go(std::integral_constant<int, 16>{});
-    if (n == 32) go(std::integral_constant<int, 32>{});
-    if (n == 64) go(std::integral_constant<int, 64>{});
+    switch (n) {
+      case 16: go(std::integral_constant<int, 16>{}); break;
+      case 32: go(std::integral_constant<int, 32>{}); break;
+      case 48: go(std::integral_constant<int, 48>{}); break;
+      case 64: go(std::integral_constant<int, 64>{}); break;
+      case 80: go(std::integral_constant<int, 80>{}); break;
+      default: go(std::integral_constant<int, 96>{}); break;
+    }
   } else {
     hipLaunchKernelGGL((fft_plane_kernel<T>), grid, dim3(threads), lds, s, data, pz.st, py.st, pz.twiddles<T>(),
                        py.twiddles<T>(), sign);

@@ -42,7 +42,8 @@ def _problem(oracle, n, L, mmax, seed=1, mrso=MRSO):
 
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("shape", [(2, 5, 5, 5), (3, 4, 6, 5), (2, 8, 8, 8), (1, 16, 12, 10), (2, 32, 32, 32),
-                                   (1, 64, 64, 64), (1, 96, 48, 80), (2, 7, 11, 13), (1, 1, 1, 1)])
+                                   (1, 64, 64, 64), (1, 96, 48, 80), (2, 7, 11, 13), (1, 1, 1, 1),
+                                   (1, 128, 16, 16), (2, 16, 32, 128)])
 @pytest.mark.parametrize("prec", [FP64, FP32])
 def test_fft3d(shape, prec):
     r = np.random.default_rng(0)
@@ -264,7 +265,8 @@ def test_argument_errors():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("shape", [(2, 8, 12, 10), (1, 16, 16, 16), (2, 64, 64, 64), (3, 7, 9, 5)])
+@pytest.mark.parametrize("shape", [(2, 8, 12, 10), (1, 16, 16, 16), (2, 64, 64, 64), (3, 7, 9, 5), (1, 128, 32, 16),
+                                   (2, 32, 32, 32)])
 @pytest.mark.parametrize("dt", ["c64", "c128"])
 def test_fft_configurations_are_bitwise_equal(shape, dt):
     """Every configuration the FFT tuner can pick (fused z/y plane pass with 256
