@@ -249,9 +249,10 @@ __global__ __launch_bounds__(kWave) void oz_dtable_kernel(T* __restrict__ dtab, 
   }
 }
 
-// The OZ pass for angular grids with compile-time (mmax, mrso): one pair per
-// 64-lane block as in oz_pair_kernel, with every LDS access a lane base plus a
-// compile-time offset and every loop unrolled.  Layouts (W = 2 mmax + 1):
+// The OZ pass for angular grids with compile-time (mmax, mrso), FP64 (FP32
+// runs oz_pairs_ct_kernel): one pair per 64-lane block as in oz_pair_kernel,
+// with every LDS access a lane base plus a compile-time offset and every loop
+// unrolled.  Layouts (W = 2 mmax + 1):
 //   G, Gm:  rows (m, mu2) of W entries, entry mup + m: the phased gathers,
 //           later gamma with entry chi + m (zero padding past 2m+1);
 //   Dq, Dmq: the rotated vectors in projection order;
@@ -1204,20 +1205,22 @@ void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch)
   a.pstride = a.N * nbatch;
   const long nblk = a.chunk * kXcds * nbatch;
   throw_if(nblk > 0x7fffffffL, "too many q pairs");
-  if (padw && std::is_same<T, double>::value) {
-    // FP64: one pair per 64-lane block (the two-pair block holds 61 KB of LDS
-    // in FP64 and measured slower)
-    dispatch_oz(tab.mmax, tab.mrso, [&](auto mm, auto mr) {
-      constexpr int MMAX = decltype(mm)::value, MRSO = decltype(mr)::value;
-      using Sh = OzShape<MMAX, MRSO>;
-      const size_t lds = (2 * static_cast<size_t>(Sh::NG) * Sh::W + 2 * Sh::NP + tab.np_new) * sizeof(cplx<T>) +
-                         static_cast<size_t>(Sh::W) * (Sh::ND + 1) * sizeof(T);
-      throw_if(lds > 64 * 1024, "angular grid too large for the OZ kernel");
-      hipLaunchKernelGGL((oz_pair_ct_kernel<T, MMAX, MRSO>), dim3(static_cast<unsigned>(nblk)), dim3(kWave), lds, s,
-                         p, dt, a);
-    });
-    MDFT_HIP_CHECK(hipGetLastError());
-    return;
+  if constexpr (std::is_same<T, double>::value) {
+    if (padw) {
+      // FP64: one pair per 64-lane block (the two-pair block holds 61 KB of
+      // LDS in FP64 and measured slower)
+      dispatch_oz(tab.mmax, tab.mrso, [&](auto mm, auto mr) {
+        constexpr int MMAX = decltype(mm)::value, MRSO = decltype(mr)::value;
+        using Sh = OzShape<MMAX, MRSO>;
+        const size_t lds = (2 * static_cast<size_t>(Sh::NG) * Sh::W + 2 * Sh::NP + tab.np_new) * sizeof(cplx<T>) +
+                           static_cast<size_t>(Sh::W) * (Sh::ND + 1) * sizeof(T);
+        throw_if(lds > 64 * 1024, "angular grid too large for the OZ kernel");
+        hipLaunchKernelGGL((oz_pair_ct_kernel<T, MMAX, MRSO>), dim3(static_cast<unsigned>(nblk)), dim3(kWave), lds,
+                           s, p, dt, a);
+      });
+      MDFT_HIP_CHECK(hipGetLastError());
+      return;
+    }
   }
   if constexpr (std::is_same<T, float>::value) {
     if (padw) {
