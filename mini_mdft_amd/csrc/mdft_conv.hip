@@ -625,6 +625,7 @@ __global__ __launch_bounds__(256) void oz_pairs_ct_kernel(cplx<T>* __restrict__ 
                                                           const OzPairMeta<T>* __restrict__ meta,
                                                           const cplx<T>* __restrict__ phase,
                                                           const int* __restrict__ dmap, OzArgs<T> a, int region) {
+  static_assert(std::is_same<T, float>::value, "the two-pair OZ kernel is the FP32 one");
   using S = OzShape<MMAX, MRSO>;
   constexpr int W = S::W, NP = S::NP, NC = oz_terms<MMAX, MRSO>();
   constexpr int NH = (MMAX + 1) * (MMAX + 2) / 2, NE = NH * W;
@@ -676,14 +677,32 @@ __global__ __launch_bounds__(256) void oz_pairs_ct_kernel(cplx<T>* __restrict__ 
     singular = (ixm == ix && iym == iy && izm == iz) && (ix == a.nx / 2 || iy == a.ny / 2 || iz == a.nz / 2);
     // gathers (:420-425), the two c rows and this thread's metadata
     const cplx<T> u = p[static_cast<long>(ip) * a.pstride + sq], v = p[static_cast<long>(ip) * a.pstride + smq];
-    cplx<T> lo[NCU], hi[NCU];
-    const unsigned crow = static_cast<unsigned>(mt.iq) * NC;
+    // the two c rows: with an even term count, two adjacent terms per lane
+    // and load (16-byte accesses: the rows start at multiples of 16 bytes, and
+    // so does ceff in LDS, after 2 (NG W + NP) complex values)
+    constexpr bool C2 = NC % 2 == 0;
+    constexpr int NC2 = NC / 2, NCU2 = (NC2 + NP - 1) / NP;
+    cplx<T> lo[C2 ? 1 : NCU], hi[C2 ? 1 : NCU];
+    float4 lo4[C2 ? NCU2 : 1], hi4[C2 ? NCU2 : 1];
+    if constexpr (C2) {
+      const float4* c4 = reinterpret_cast<const float4*>(a.ctab);
+      const unsigned crow2 = static_cast<unsigned>(mt.iq) * NC2;
 #pragma unroll
-    for (int k = 0; k < NCU; ++k) {
-      const int ia = ip + k * NP;
-      const unsigned at = crow + static_cast<unsigned>(k + 1 < NCU ? ia : min(ia, NC - 1));
-      lo[k] = ld_off(a.ctab, at);
-      hi[k] = ld_off(a.ctab, at + NC);
+      for (int k = 0; k < NCU2; ++k) {
+        const int j = ip + k * NP;
+        const unsigned at = crow2 + static_cast<unsigned>(k + 1 < NCU2 ? j : min(j, NC2 - 1));
+        lo4[k] = ld_off(c4, at);
+        hi4[k] = ld_off(c4, at + NC2);
+      }
+    } else {
+      const unsigned crow = static_cast<unsigned>(mt.iq) * NC;
+#pragma unroll
+      for (int k = 0; k < NCU; ++k) {
+        const int ia = ip + k * NP;
+        const unsigned at = crow + static_cast<unsigned>(k + 1 < NCU ? ia : min(ia, NC - 1));
+        lo[k] = ld_off(a.ctab, at);
+        hi[k] = ld_off(a.ctab, at + NC);
+      }
     }
     info = a.ipinfo[ip];
     ias = a.ia_start[ip];
@@ -710,10 +729,25 @@ __global__ __launch_bounds__(256) void oz_pairs_ct_kernel(cplx<T>* __restrict__ 
     for (int i = ip; i < 2 * S::NG * W; i += NP) G[i] = mkc<T>(0, 0);
     // c(|q|) rows iq, iq+1 lerped into LDS
     const T al = mt.al, bl = mt.bl;
+    if constexpr (C2) {
 #pragma unroll
-    for (int k = 0; k < NCU; ++k) {
-      const int ia = ip + k * NP;
-      if (ia < NC) ceff[ia] = mkc<T>(al * hi[k].x + bl * lo[k].x, al * hi[k].y + bl * lo[k].y);
+      for (int k = 0; k < NCU2; ++k) {
+        const int j = ip + k * NP;
+        if (j < NC2) {
+          float4 v;
+          v.x = al * hi4[k].x + bl * lo4[k].x;
+          v.y = al * hi4[k].y + bl * lo4[k].y;
+          v.z = al * hi4[k].z + bl * lo4[k].z;
+          v.w = al * hi4[k].w + bl * lo4[k].w;
+          reinterpret_cast<float4*>(ceff)[j] = v;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < NCU; ++k) {
+        const int ia = ip + k * NP;
+        if (ia < NC) ceff[ia] = mkc<T>(al * hi[k].x + bl * lo[k].x, al * hi[k].y + bl * lo[k].y);
+      }
     }
   }
   __syncthreads();
