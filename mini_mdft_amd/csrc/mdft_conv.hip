@@ -574,12 +574,18 @@ constexpr int oz_pairs_per_block() {
 // (bits 0-11) and to its mirror dP[(l^2 + l - mu) W + 2l - c] with the sign
 // (-1)^{mu + c - l} (bits 12-23, kDmapNeg), or, for a padding column c > 2l,
 // a zero at dP[(l^2 + l - mu) W + c] (kDmapZero).  Built once per
-// Convolution, as code[k NP + ip].
+// Convolution as one 16-byte record per thread ip: the codes of k = 0..2
+// (at most three entries per thread for mmax <= 6), then the projection's
+// packed (m, mup + mmax, mu2) of the ipinfo table.
 constexpr int kDmapNeg = 1 << 24, kDmapZero = 1 << 25, kDmapValid = 1 << 26;
+constexpr int kDmapPerThread = 3;
 
-std::vector<int> oz_dmap(int mmax, int np) {
+std::vector<int> oz_thread_records(const AngularGrid& g) {
+  const int mmax = g.mmax, np = g.np;
   const int W = 2 * mmax + 1, NE = (mmax + 1) * (mmax + 2) / 2 * W, nu = (NE + np - 1) / np;
-  std::vector<int> code(static_cast<size_t>(nu) * np, 0);
+  throw_if(nu > kDmapPerThread, "internal error: too many d entries per OZ thread");
+  std::vector<int> code(static_cast<size_t>(np) * 4, 0);
+  for (int ip = 0; ip < np; ++ip) code[static_cast<size_t>(ip) * 4 + 3] = g.m[ip] | ((g.mup[ip] + mmax) << 8) | (g.mu2[ip] << 16);
   for (int k = 0; k < nu; ++k)
     for (int ip = 0; ip < np; ++ip) {
       const int e = ip + k * np;
@@ -597,7 +603,7 @@ std::vector<int> oz_dmap(int mmax, int np) {
         v |= (base - mu * W + c) << 12;
         v |= kDmapZero;
       }
-      code[static_cast<size_t>(k) * np + ip] = v;
+      code[static_cast<size_t>(ip) * 4 + k] = v;
     }
   return code;
 }
@@ -630,6 +636,7 @@ __global__ __launch_bounds__(256) void oz_pairs_ct_kernel(cplx<T>* __restrict__ 
   constexpr int W = S::W, NP = S::NP, NC = oz_terms<MMAX, MRSO>();
   constexpr int NH = (MMAX + 1) * (MMAX + 2) / 2, NE = NH * W;
   constexpr int NU = (NE + NP - 1) / NP, NCU = (NC + NP - 1) / NP;
+  static_assert(NU <= kDmapPerThread, "d entries per thread of the thread records");
   extern __shared__ unsigned char lds_raw[];
   __shared__ cplx<T> phs[PB][W];  // e^{-i mup phi_q} of each pair
   const int tid = threadIdx.x;
@@ -658,13 +665,14 @@ __global__ __launch_bounds__(256) void oz_pairs_ct_kernel(cplx<T>* __restrict__ 
     // inputs that need only the pair index: the half d table and the phases
     const T* dgrp = dtab + grp * (PB * NE);  // block-uniform
     T dreg[NU];
-    int dcode[NU];
 #pragma unroll
     for (int k = 0; k < NU; ++k) {
       const int e = ip + k * NP;
       dreg[k] = ld_off(dgrp, static_cast<unsigned>(pp * NE + (k + 1 < NU ? e : min(e, NE - 1))));
-      dcode[k] = dmap[k * NP + ip];
     }
+    // this thread's LDS places of its d entries and its projection's info
+    const int4 rec = reinterpret_cast<const int4*>(dmap)[ip];
+    const int dcode[kDmapPerThread] = {rec.x, rec.y, rec.z};
     cplx<T> phv = mkc<T>(0, 0);
     const int amup = ip < MMAX ? MMAX - ip : ip - MMAX;
     if (ip < W) phv = phase[pair * (MMAX + 1) + amup];
@@ -704,7 +712,7 @@ __global__ __launch_bounds__(256) void oz_pairs_ct_kernel(cplx<T>* __restrict__ 
         hi[k] = ld_off(a.ctab, at + NC);
       }
     }
-    info = a.ipinfo[ip];
+    info = rec.w;
     ias = a.ia_start[ip];
     g0x = u.x;
     g0y = u.y;
@@ -1217,7 +1225,7 @@ const void* ConvDevice::pair_meta(hipStream_t s, const void** phases) const {
   if (!pmeta32) {
     MDFT_HIP_CHECK(hipMalloc(&pmeta32, std::max<size_t>(1, npairs * sizeof(OzPairMeta<float>))));
     MDFT_HIP_CHECK(hipMalloc(&pphase32, std::max<size_t>(1, npairs * (tab.mmax + 1) * sizeof(float2))));
-    dmap = upload(oz_dmap(tab.mmax, tab.ang.np));
+    dmap = upload(oz_thread_records(tab.ang));
     OzArgs<float> a = oz_args<float>(*this, 0.0);
     const unsigned nb = static_cast<unsigned>((npairs + 255) / 256);
     hipLaunchKernelGGL((oz_meta_kernel<float>), dim3(nb), dim3(256), 0, s, static_cast<OzPairMeta<float>*>(pmeta32),

@@ -36,7 +36,7 @@ struct ConvDevice {
   const void* pair_meta(hipStream_t s, const void** phases) const;
   mutable void* pmeta32 = nullptr;
   mutable void* pphase32 = nullptr;
-  mutable int* dmap = nullptr;  // LDS places of the half d table's entries (see oz_dmap)
+  mutable int* dmap = nullptr;  // per-thread records: LDS places of the half d table's entries, ipinfo
   // row width of the padded d layout when the compile-time OZ kernel serves
   // this (mmax, mrso) (2 mmax + 1), else 0 (compact rows, generic kernel)
   int padw = 0;
