@@ -576,16 +576,20 @@ constexpr int oz_pairs_per_block() {
 // a zero at dP[(l^2 + l - mu) W + c] (kDmapZero).  Built once per
 // Convolution as one 16-byte record per thread ip: the codes of k = 0..2
 // (at most three entries per thread for mmax <= 6), then the projection's
-// packed (m, mup + mmax, mu2) of the ipinfo table.
+// m | (mup + mmax) << 4 | mu2 << 8 | ia_start << 12 (its first OZ term).
 constexpr int kDmapNeg = 1 << 24, kDmapZero = 1 << 25, kDmapValid = 1 << 26;
 constexpr int kDmapPerThread = 3;
 
-std::vector<int> oz_thread_records(const AngularGrid& g) {
+std::vector<int> oz_thread_records(const AngularGrid& g, const std::vector<int>& ia_start) {
   const int mmax = g.mmax, np = g.np;
   const int W = 2 * mmax + 1, NE = (mmax + 1) * (mmax + 2) / 2 * W, nu = (NE + np - 1) / np;
-  throw_if(nu > kDmapPerThread, "internal error: too many d entries per OZ thread");
+  throw_if(nu > kDmapPerThread || 2 * mmax >= 16, "internal error: OZ thread records");
   std::vector<int> code(static_cast<size_t>(np) * 4, 0);
-  for (int ip = 0; ip < np; ++ip) code[static_cast<size_t>(ip) * 4 + 3] = g.m[ip] | ((g.mup[ip] + mmax) << 8) | (g.mu2[ip] << 16);
+  for (int ip = 0; ip < np; ++ip) {
+    throw_if(ia_start[ip] >= (1 << 19), "internal error: OZ term index of the thread records");
+    code[static_cast<size_t>(ip) * 4 + 3] =
+        g.m[ip] | ((g.mup[ip] + mmax) << 4) | (g.mu2[ip] << 8) | (ia_start[ip] << 12);
+  }
   for (int k = 0; k < nu; ++k)
     for (int ip = 0; ip < np; ++ip) {
       const int e = ip + k * np;
@@ -713,7 +717,6 @@ __global__ __launch_bounds__(256) void oz_pairs_ct_kernel(cplx<T>* __restrict__ 
       }
     }
     info = rec.w;
-    ias = a.ia_start[ip];
     g0x = u.x;
     g0y = u.y;
     g1x = v.x;
@@ -760,7 +763,8 @@ __global__ __launch_bounds__(256) void oz_pairs_ct_kernel(cplx<T>* __restrict__ 
   }
   __syncthreads();
 
-  const int m = info & 0xff, mupm = (info >> 8) & 0xff, mu2 = info >> 16;
+  const int m = info & 0xf, mupm = (info >> 4) & 0xf, mu2 = (info >> 8) & 0xf;
+  ias = info >> 12;
   const int chi = mupm - MMAX;  // the same field read as chi in the q frame
   const int row = grow_rt<MMAX, MRSO>(m) + mu2;
   // phased gathers g' = g e^{-i mup phi} into G[(m, mu2)][mup + m]
@@ -1225,7 +1229,7 @@ const void* ConvDevice::pair_meta(hipStream_t s, const void** phases) const {
   if (!pmeta32) {
     MDFT_HIP_CHECK(hipMalloc(&pmeta32, std::max<size_t>(1, npairs * sizeof(OzPairMeta<float>))));
     MDFT_HIP_CHECK(hipMalloc(&pphase32, std::max<size_t>(1, npairs * (tab.mmax + 1) * sizeof(float2))));
-    dmap = upload(oz_thread_records(tab.ang));
+    dmap = upload(oz_thread_records(tab.ang, tab.ia_start));
     OzArgs<float> a = oz_args<float>(*this, 0.0);
     const unsigned nb = static_cast<unsigned>((npairs + 255) / 256);
     hipLaunchKernelGGL((oz_meta_kernel<float>), dim3(nb), dim3(256), 0, s, static_cast<OzPairMeta<float>*>(pmeta32),
