@@ -272,6 +272,7 @@ __global__ __launch_bounds__(64) void proj2angl_wave_kernel(const cplx<TP>* __re
   const long r0 = rb * kWaveRows;
   const long r = min(r0 + lane, N - 1);
   const TC* w = tb.wts + th * NP;  // d f_m of this theta
+  const double wth = wq[th * A::kNjk];  // the angular weight of every (phi, psi) of this theta
 
   TC sr[A::kNphi][A::kNmu2], si[A::kNphi][A::kNmu2];
 #pragma unroll
@@ -357,7 +358,7 @@ __global__ __launch_bounds__(64) void proj2angl_wave_kernel(const cplx<TP>* __re
           const long g = base + static_cast<long>(rl) * A::kNo + jk;
           const double x = xv[b];
           const double vx = static_cast<double>(static_cast<TO>(stage[e]));
-          acc_ff += vx * wq[th * A::kNjk + jk] * (x * x - 1.0);
+          acc_ff += vx * wth * (x * x - 1.0);
           df[g] = static_cast<TO>(2.0 * rho0 * x * vx);
         }
       }
@@ -374,7 +375,7 @@ __global__ __launch_bounds__(64) void proj2angl_wave_kernel(const cplx<TP>* __re
         } else {
           const double x = static_cast<double>(xi[g]);
           const double vx = static_cast<double>(static_cast<TO>(v));
-          acc_ff += vx * wq[th * A::kNjk + jk] * (x * x - 1.0);
+          acc_ff += vx * wth * (x * x - 1.0);
           df[g] = static_cast<TO>(2.0 * rho0 * x * vx);
         }
       }
@@ -561,6 +562,10 @@ ProjTables::ProjTables(const AngularGrid& g)
   wbwd32 = upload(wb32);
   ipinfo = upload(info);
   yidx = upload(yi);
+  // the weights are w_theta times constants: every io of one theta has the
+  // same weight, which the compile-time proj2angl kernel reads once per theta
+  for (int io = 0; io < no; ++io)
+    throw_if(g.w[io] != g.w[(io / (nphi * npsi)) * nphi * npsi], "internal error: angular weights depend on phi or psi");
   w = upload(g.w);
 }
 
