@@ -509,8 +509,9 @@ static void energy_impl(mdft_solver_t h, int nb, const void* xi, void* df, doubl
 int mdft_solver_set_timing(mdft_solver_t h, int enable) {
   return guarded([&] {
     throw_if(!h, "null handle");
+    throw_if(enable < 0 || enable > 2, "stage timing mode must be 0, 1 or 2");
     if (enable)
-      h->timer = std::make_unique<StageTimer>();
+      h->timer = std::make_unique<StageTimer>(enable == 2);
     else
       h->timer.reset();
   });

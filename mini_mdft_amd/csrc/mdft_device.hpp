@@ -87,10 +87,13 @@ constexpr int kStageStart = 0, kStageConvStart = 1, kStageFftFwd = 2, kStageOz =
               kStageEnd = 5, kNumMarks = 6, kNumStages = 5;
 
 // hipEvent marks recorded on the work stream, one set per evaluation, read
-// back only when collected (no synchronisation between evaluations).
+// back only when collected (no synchronisation between evaluations).  With
+// oz_only, only the two marks around the OZ pass are recorded (each recorded
+// event leaves the GPU idle for a few microseconds between kernels), and only
+// that stage accumulates.
 class StageTimer {
  public:
-  StageTimer() = default;
+  explicit StageTimer(bool oz_only = false) : oz_only_(oz_only) {}
   ~StageTimer() {
     for (auto& set : sets_)
       for (hipEvent_t e : set) (void)hipEventDestroy(e);
@@ -107,12 +110,16 @@ class StageTimer {
     }
     cur_ = used_++;
   }
-  void mark(hipStream_t s, int i) { MDFT_HIP_CHECK(hipEventRecord(sets_[cur_][i], s)); }
+  void mark(hipStream_t s, int i) {
+    if (oz_only_ && i != kStageFftFwd && i != kStageOz) return;
+    MDFT_HIP_CHECK(hipEventRecord(sets_[cur_][i], s));
+  }
   // waits for the recorded evaluations and adds their stage times
   void collect() {
     for (int k = 0; k < used_; ++k) {
-      MDFT_HIP_CHECK(hipEventSynchronize(sets_[k][kStageEnd]));
+      MDFT_HIP_CHECK(hipEventSynchronize(sets_[k][oz_only_ ? kStageOz : kStageEnd]));
       for (int i = 0; i < kNumStages; ++i) {
+        if (oz_only_ && i != kStageFftFwd) continue;  // stage FftFwd -> Oz is the OZ pass
         float ms = 0.f;
         MDFT_HIP_CHECK(hipEventElapsedTime(&ms, sets_[k][i], sets_[k][i + 1]));
         acc_[i] += ms;
@@ -131,6 +138,7 @@ class StageTimer {
 
  private:
   static constexpr int kMaxSets = 512;
+  bool oz_only_ = false;
   std::vector<std::array<hipEvent_t, kNumMarks>> sets_;
   int used_ = 0, cur_ = 0;
   std::array<double, kNumStages> acc_{};

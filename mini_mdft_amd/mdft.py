@@ -334,9 +334,10 @@ class Solver:
 
     STAGES = ("angl2proj", "fft_forward", "oz", "fft_backward", "proj2angl_delta_f")
 
-    def set_timing(self, enable: bool) -> None:
-        """Per-stage hipEvent timers (the reference Solver's stage timers)."""
-        check(lib().mdft_solver_set_timing(self._h, 1 if enable else 0))
+    def set_timing(self, enable: bool, oz_only: bool = False) -> None:
+        """Per-stage hipEvent timers (the reference Solver's stage timers); with
+        oz_only, only the two events around the OZ pass are recorded."""
+        check(lib().mdft_solver_set_timing(self._h, (2 if oz_only else 1) if enable else 0))
 
     def stage_ms(self):
         """(accumulated ms per stage as a dict, number of evaluations); waits for them."""

@@ -55,3 +55,31 @@ def test_batch_errors():
     xi = torch.ones((3, sg.N, ag.m_no), dtype=torch.float64, device="cuda")
     with pytest.raises(M.MDFTError, match="capacity"):
         s.energy_cproj_batch(xi, torch.empty_like(xi))
+
+
+@pytest.mark.gpu
+def test_stage_timers():
+    """All five stage timers, or only the OZ pass's (mode used by the bench's
+    timed region), count every evaluation; timing never changes a result."""
+    import torch
+
+    sg, ag, conv = _conv((16, 16, 16), (10.0,) * 3, 3, MIXED_X6)
+    xi = torch.from_numpy(1.0 + 0.1 * np.random.default_rng(4).uniform(-1, 1, (sg.N, ag.m_no))).cuda()
+    s = M.Solver(conv, RHO0, PREF, MIXED_X6)
+    df0 = torch.empty_like(xi)
+    ff0 = s.energy_cproj(xi, df0)
+    ffd = torch.zeros(1, dtype=torch.float64, device="cuda")
+    for oz_only in (False, True):
+        s.set_timing(True, oz_only=oz_only)
+        df = torch.empty_like(xi)
+        for _ in range(3):
+            s.energy_cproj_async(xi, df, ffd)
+        ms, cnt = s.stage_ms()
+        assert cnt == 3
+        assert ms["oz"] > 0
+        others = [v for k, v in ms.items() if k != "oz"]
+        assert all(v == 0 for v in others) if oz_only else all(v > 0 for v in others)
+        assert float(ffd[0]) == ff0 and torch.equal(df, df0)
+    s.set_timing(False)
+    with pytest.raises(M.MDFTError, match="stage timing is off"):
+        s.stage_ms()
