@@ -60,7 +60,11 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kWaves = 4;  // waves per block: rows sharing one weight slice
 constexpr int kThreads = 64 * kWaves;
 constexpr int kK = 32;  // K per MFMA
-constexpr int kA2pRowsPerWave = 32;
+// angl2proj: row groups of 16 per wave (one group keeps the kernel at 140
+// registers, three waves per SIMD; two groups share each LDS weight fragment
+// over twice the rows but need 228 registers and measured 4% slower)
+constexpr int kA2pGroups = 1;
+constexpr int kA2pRowsPerWave = 16 * kA2pGroups;
 constexpr int kP2aRowsPerWave = 16;
 constexpr int kMaxA2pTiles = 16;  // 2 np <= 256
 constexpr int kMaxP2aTiles = 24;  // no <= 384
@@ -154,9 +158,9 @@ struct Stage {
   }
 };
 
-// P^T = Wa . delta_rho^T.  Block: 32 kWaves consecutive r; wave w: r0 + 32 w +
-// (0..31).  EVEN: no is even, so a row's K pairs are 16-byte aligned double2
-// loads.
+// P^T = Wa . delta_rho^T.  Block: kA2pRowsPerWave kWaves consecutive r; wave
+// w: r0 + kA2pRowsPerWave w + (0 .. kA2pRowsPerWave - 1), in groups u of 16.
+// EVEN: no is even, so a row's K pairs are 16-byte aligned double2 loads.
 template <int PX, int PW, int MT, bool FUSE, bool EVEN>
 __global__ __launch_bounds__(kThreads) void a2p_gemm_kernel(const double* __restrict__ o, float2* __restrict__ p,
                                                             const u32x4* __restrict__ wsw, GemmDims d, double rho0) {
@@ -165,21 +169,23 @@ __global__ __launch_bounds__(kThreads) void a2p_gemm_kernel(const double* __rest
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const long r0 = static_cast<long>(blockIdx.x) * (kWaves * kA2pRowsPerWave) + wave * kA2pRowsPerWave;
   const int slice = MT * 64 * PW;  // u32x4 per K-step in global memory
-  f32x4 acc[MT][2];
+  f32x4 acc[MT][kA2pGroups];
 #pragma unroll
-  for (int t = 0; t < MT; ++t) acc[t][0] = acc[t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int t = 0; t < MT; ++t)
+#pragma unroll
+    for (int u = 0; u < kA2pGroups; ++u) acc[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
 
   // rows r0 + 16 u + lane%16 (clamped), k = 32 s + 8 (lane/16) + (0..7) (clamped)
-  const double* rowp[2];
+  const double* rowp[kA2pGroups];
 #pragma unroll
-  for (int u = 0; u < 2; ++u) {
+  for (int u = 0; u < kA2pGroups; ++u) {
     const long r = r0 + 16 * u + (lane & 15);
     rowp[u] = o + (r < d.N ? r : d.N - 1) * d.no;
   }
-  auto load_data = [&](int s, double (&raw)[2][8]) {
+  auto load_data = [&](int s, double (&raw)[kA2pGroups][8]) {
     const int k0 = s * kK + 8 * (lane >> 4);
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < kA2pGroups; ++u) {
       if (EVEN) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -194,9 +200,9 @@ __global__ __launch_bounds__(kThreads) void a2p_gemm_kernel(const double* __rest
       }
     }
   };
-  auto pack_data = [&](const double (&raw)[2][8], u32x4 (&x)[2][PX]) {
+  auto pack_data = [&](const double (&raw)[kA2pGroups][8], u32x4 (&x)[kA2pGroups][PX]) {
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < kA2pGroups; ++u) {
       float v[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
@@ -209,8 +215,8 @@ __global__ __launch_bounds__(kThreads) void a2p_gemm_kernel(const double* __rest
   };
 
   St st;
-  double raw[2][8];
-  u32x4 x[2][PX];
+  double raw[kA2pGroups][8];
+  u32x4 x[kA2pGroups][PX];
   st.load(wsw, tid);
   load_data(0, raw);
   st.store(lds, tid);
@@ -227,7 +233,7 @@ __global__ __launch_bounds__(kThreads) void a2p_gemm_kernel(const double* __rest
     for (int t = 0; t < MT; ++t) {
       const Frag<PW> f = wl[t * 64 + lane];
 #pragma unroll
-      for (int u = 0; u < 2; ++u) acc[t][u] = mfma_split<PX, PW, true>(x[u], f, acc[t][u]);
+      for (int u = 0; u < kA2pGroups; ++u) acc[t][u] = mfma_split<PX, PW, true>(x[u], f, acc[t][u]);
     }
     if (more) {
       __syncthreads();
@@ -241,7 +247,7 @@ __global__ __launch_bounds__(kThreads) void a2p_gemm_kernel(const double* __rest
 #pragma unroll
   for (int t = 0; t < MT; ++t) {
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < kA2pGroups; ++u) {
       const long r = r0 + 16 * u + (lane & 15);
       const int ip = 8 * t + 2 * (lane >> 4);
       if (r < d.N) {
