@@ -73,11 +73,11 @@ def stage_bytes(N, no, np_, form):
 
 
 def timed(solver, xi, df, ffd, steps, warmup):
-    """Wall time of `steps` back-to-back evaluations, with HIP events recorded
-    only around the OZ launch (the roofline kernel); then the per-stage times
-    from a second, untimed run of `steps` evaluations with every stage
-    boundary marked (each recorded event idles the GPU for a few us, so the
-    timed run records two per evaluation rather than six)."""
+    """Wall time of `steps` back-to-back evaluations, with HIP events only on
+    the OZ launch (the roofline kernel), which records its own start and end;
+    then the per-stage times from a second, untimed run of `steps`
+    evaluations with every stage boundary marked (each event recorded between
+    kernels idles the GPU for a few us, so the timed run records none)."""
     import torch
 
     for _ in range(warmup):
@@ -366,8 +366,8 @@ def main():
     stages = {k: dict(ms=round(ms, 5), GBs=round(by[k] / (ms * 1e-3) / 1e9, 1)) for k, ms in x6["stages"].items()}
     eval_bytes = sum(by.values())
     # dominant kernel: the OZ pass (one launch per evaluation); its time is
-    # the HIP-event interval on the solver's stream around that launch,
-    # averaged over the timed region
+    # the interval between the start and end events that the launch records
+    # on the solver's stream, averaged over the timed region
     oz_ms = x6["oz_timed"]
     oz_bytes = by["oz"]
     npairs = 131076

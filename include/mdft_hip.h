@@ -184,9 +184,10 @@ int mdft_energy_cproj_batch_async(mdft_solver_t h, int nbatch, const void* xi, v
 /* Stage timers (the Solver's per-stage Kokkos::Timer printout,
  * src/mini-mdft.hpp:140-192), as hipEvents recorded on the work stream
  * without synchronising between evaluations.  set_timing(1) starts (and
- * resets) them, set_timing(2) records only the two events around the OZ pass
- * (the other stages read 0; each recorded event leaves the GPU idle for a few
- * microseconds between kernels), set_timing(0) stops them.  stage_ms waits
+ * resets) them, set_timing(2) times only the OZ pass, whose launch records its
+ * own start and end (hipExtLaunchKernel; the other stages read 0: each event
+ * recorded between kernels leaves the GPU idle for a few microseconds),
+ * set_timing(0) stops them.  stage_ms waits
  * for the recorded evaluations and returns the accumulated milliseconds of
  * ms[0..4] = angl2proj (delta_rho fused), forward 3D FFT, rotation/OZ pass,
  * backward 3D FFT, proj2angl + delta_f, and the number of evaluations in

@@ -23,6 +23,8 @@
 // neighbouring pairs share L2 lines.
 #include "mdft_conv.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -1240,9 +1242,28 @@ const void* ConvDevice::pair_meta(hipStream_t s, const void** phases) const {
   return pmeta32;
 }
 
+// launches kernel on s; with events, through hipExtLaunchKernel, so that the
+// dispatch itself records its start and end times (no event packets of
+// their own between the kernels)
+template <typename K, typename... Args>
+void launch_oz(K kernel, unsigned nb, unsigned threads, size_t lds, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1,
+               Args... args) {
+  if (ev0 || ev1)
+    hipExtLaunchKernelGGL(kernel, dim3(nb), dim3(threads), static_cast<std::uint32_t>(lds), s, ev0, ev1, 0u,
+                          args...);
+  else
+    hipLaunchKernelGGL(kernel, dim3(nb), dim3(threads), lds, s, args...);
+  MDFT_HIP_CHECK(hipGetLastError());
+}
+
 template <typename T>
-void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch) const {
-  if (npairs == 0) return;
+void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch, hipEvent_t ev0,
+                         hipEvent_t ev1) const {
+  if (npairs == 0) {
+    if (ev0) MDFT_HIP_CHECK(hipEventRecord(ev0, s));
+    if (ev1) MDFT_HIP_CHECK(hipEventRecord(ev1, s));
+    return;
+  }
   throw_if(tab.ang.np > 4 * kWave, "the OZ kernel handles at most 256 projections");
   throw_if(nbatch < 1, "OZ pass: bad batch");
   const T* dt = dtable<T>(s);
@@ -1261,10 +1282,8 @@ void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch)
         const size_t lds = (2 * static_cast<size_t>(Sh::NG) * Sh::W + 2 * Sh::NP + tab.np_new) * sizeof(cplx<T>) +
                            static_cast<size_t>(Sh::W) * (Sh::ND + 1) * sizeof(T);
         throw_if(lds > 64 * 1024, "angular grid too large for the OZ kernel");
-        hipLaunchKernelGGL((oz_pair_ct_kernel<T, MMAX, MRSO>), dim3(static_cast<unsigned>(nblk)), dim3(kWave), lds,
-                           s, p, dt, a);
+        launch_oz(oz_pair_ct_kernel<T, MMAX, MRSO>, static_cast<unsigned>(nblk), kWave, lds, s, ev0, ev1, p, dt, a);
       });
-      MDFT_HIP_CHECK(hipGetLastError());
       return;
     }
   }
@@ -1289,18 +1308,16 @@ void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch)
         const long nb = ag.chunk * kXcds * nbatch;
         throw_if(nb > 0x7fffffffL, "too many q pairs");
         const int threads = (PB * Sh::NP + kWave - 1) / kWave * kWave;
-        hipLaunchKernelGGL((oz_pairs_ct_kernel<T, MMAX, MRSO, PB>), dim3(static_cast<unsigned>(nb)), dim3(threads),
-                           PB * region, s, p, dt, mt, ph, dmap, ag, static_cast<int>(region));
+        launch_oz(oz_pairs_ct_kernel<T, MMAX, MRSO, PB>, static_cast<unsigned>(nb), static_cast<unsigned>(threads),
+                  PB * region, s, ev0, ev1, p, dt, mt, ph, dmap, ag, static_cast<int>(region));
       });
-      MDFT_HIP_CHECK(hipGetLastError());
       return;
     }
   }
   const size_t lds = oz_lds_bytes(sizeof(T));
   throw_if(lds > 64 * 1024, "angular grid too large for the OZ kernel");
   constexpr bool stage = std::is_same<T, float>::value;
-  hipLaunchKernelGGL((oz_pair_kernel<T, stage>), dim3(static_cast<unsigned>(nblk)), dim3(kWave), lds, s, p, dt, a);
-  MDFT_HIP_CHECK(hipGetLastError());
+  launch_oz(oz_pair_kernel<T, stage>, static_cast<unsigned>(nblk), kWave, lds, s, ev0, ev1, p, dt, a);
 }
 
 template <typename T>
@@ -1310,14 +1327,17 @@ void ConvDevice::execute(cplx<T>* p, double chop_eps, hipStream_t s, StageTimer*
   if (timer) timer->mark(s, kStageConvStart);
   fft.execute<T>(p, batch, -1, s);
   if (timer) timer->mark(s, kStageFftFwd);
-  oz_pass<T>(p, chop_eps, s, nbatch);
+  if (timer && timer->oz_only())
+    oz_pass<T>(p, chop_eps, s, nbatch, timer->event(kStageFftFwd), timer->event(kStageOz));
+  else
+    oz_pass<T>(p, chop_eps, s, nbatch);
   if (timer) timer->mark(s, kStageOz);
   fft.execute<T>(p, batch, +1, s);
   if (timer) timer->mark(s, kStageFftBwd);
 }
 
-template void ConvDevice::oz_pass<double>(cplx<double>*, double, hipStream_t, int) const;
-template void ConvDevice::oz_pass<float>(cplx<float>*, double, hipStream_t, int) const;
+template void ConvDevice::oz_pass<double>(cplx<double>*, double, hipStream_t, int, hipEvent_t, hipEvent_t) const;
+template void ConvDevice::oz_pass<float>(cplx<float>*, double, hipStream_t, int, hipEvent_t, hipEvent_t) const;
 template void ConvDevice::execute<double>(cplx<double>*, double, hipStream_t, StageTimer*, int) const;
 template void ConvDevice::execute<float>(cplx<float>*, double, hipStream_t, StageTimer*, int) const;
 

@@ -43,9 +43,11 @@ struct ConvDevice {
 
   // the per-(q, -q) rotation / OZ / inverse-rotation pass on FFT'd data;
   // gamma components below chop_eps in magnitude are set to 0.  With
-  // nbatch > 1, p holds nbatch fields as (np, nbatch, nx, ny, nz).
+  // nbatch > 1, p holds nbatch fields as (np, nbatch, nx, ny, nz).  With
+  // events, the launch records its own start and end times in ev0 and ev1.
   template <typename T>
-  void oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch = 1) const;
+  void oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch = 1, hipEvent_t ev0 = nullptr,
+               hipEvent_t ev1 = nullptr) const;
   // FFT, oz_pass, inverse FFT: in place on (np, nx, ny, nz) (or (np, nbatch,
   // nx, ny, nz)); marks the FFT / OZ stage boundaries on `timer` when given
   template <typename T>

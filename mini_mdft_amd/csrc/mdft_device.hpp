@@ -88,9 +88,10 @@ constexpr int kStageStart = 0, kStageConvStart = 1, kStageFftFwd = 2, kStageOz =
 
 // hipEvent marks recorded on the work stream, one set per evaluation, read
 // back only when collected (no synchronisation between evaluations).  With
-// oz_only, only the two marks around the OZ pass are recorded (each recorded
-// event leaves the GPU idle for a few microseconds between kernels), and only
-// that stage accumulates.
+// oz_only, mark() records nothing (each recorded event leaves the GPU idle
+// for a few microseconds between kernels): the OZ launch itself records the
+// set's FftFwd and Oz events as its start and end timestamps
+// (hipExtLaunchKernel), and only that stage accumulates.
 class StageTimer {
  public:
   explicit StageTimer(bool oz_only = false) : oz_only_(oz_only) {}
@@ -111,9 +112,12 @@ class StageTimer {
     cur_ = used_++;
   }
   void mark(hipStream_t s, int i) {
-    if (oz_only_ && i != kStageFftFwd && i != kStageOz) return;
+    if (oz_only_) return;
     MDFT_HIP_CHECK(hipEventRecord(sets_[cur_][i], s));
   }
+  bool oz_only() const { return oz_only_; }
+  // mark i of the current evaluation's set
+  hipEvent_t event(int i) const { return sets_[cur_][i]; }
   // waits for the recorded evaluations and adds their stage times
   void collect() {
     for (int k = 0; k < used_; ++k) {

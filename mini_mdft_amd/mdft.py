@@ -336,7 +336,8 @@ class Solver:
 
     def set_timing(self, enable: bool, oz_only: bool = False) -> None:
         """Per-stage hipEvent timers (the reference Solver's stage timers); with
-        oz_only, only the two events around the OZ pass are recorded."""
+        oz_only, only the OZ launch is timed, by the start and end events that
+        the launch itself records."""
         check(lib().mdft_solver_set_timing(self._h, (2 if oz_only else 1) if enable else 0))
 
     def stage_ms(self):
