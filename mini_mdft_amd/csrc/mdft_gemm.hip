@@ -183,20 +183,22 @@ __global__ __launch_bounds__(kThreads) void a2p_gemm_kernel(const double* __rest
     rowp[u] = o + (r < d.N ? r : d.N - 1) * d.no;
   }
   auto load_data = [&](int s, double (&raw)[kA2pGroups][8]) {
-    const int k0 = s * kK + 8 * (lane >> 4);
+    // operand slot j of lane group g = lane/16 holds k = 32 s + 8 (j/2) + 2 g + j%2
+    // (a2p_k_of): load q of the four lanes of a row then reads 64 contiguous bytes
+    const int k0 = s * kK + 2 * (lane >> 4);
 #pragma unroll
     for (int u = 0; u < kA2pGroups; ++u) {
       if (EVEN) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int k = min(k0 + 2 * q, d.no - 2);
+          const int k = min(k0 + 8 * q, d.no - 2);
           const double2 v = *reinterpret_cast<const double2*>(rowp[u] + k);
           raw[u][2 * q] = v.x;
           raw[u][2 * q + 1] = v.y;
         }
       } else {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) raw[u][j] = rowp[u][min(k0 + j, d.no - 1)];
+        for (int j = 0; j < 8; ++j) raw[u][j] = rowp[u][min(k0 + 8 * (j >> 1) + (j & 1), d.no - 1)];
       }
     }
   };
@@ -352,10 +354,18 @@ __global__ __launch_bounds__(kThreads) void p2a_gemm_kernel(const float2* __rest
   }
 }
 
+// The K index of angl2proj's operand slot j (0..7) of lane l in K-step s:
+// the slots of lane group l/16 interleave with the other groups' in pairs,
+// so that the four lanes of one row load 64 contiguous bytes per 16-byte
+// load (the sum over K is the same for any such permutation, applied to the
+// weights and the data alike).
+inline int a2p_k_of(int s, int l, int j) { return s * kK + 8 * (j >> 1) + 2 * (l >> 4) + (j & 1); }
+
 // Pre-swizzle a dense FP64 matrix into PW-part MFMA fragments over `tiles`
-// output tiles.  As the A operand (rows = m, cols = k): lane l of tile t,
-// k-step s holds M[16t + l%16][32s + 8(l/16) + j]; as the B operand (rows = k,
-// cols = n): M[32s + 8(l/16) + j][16t + l%16].  Out-of-range entries are 0.  The
+// output tiles.  As the A operand (angl2proj's weights; rows = m, cols = k):
+// slot j of lane l of tile t, k-step s holds M[16t + l%16][a2p_k_of(s, l, j)];
+// as the B operand (rows = k, cols = n): M[32s + 8(l/16) + j][16t + l%16].
+// Out-of-range entries are 0.  The
 // array is followed by kThreads zero vectors: the staging of the last slice
 // reads fewer than kThreads past its end.
 template <int PW>
@@ -368,7 +378,7 @@ std::vector<u32x4> swizzle(const std::vector<double>& M, int rows, int cols, boo
       for (int l = 0; l < 64; ++l) {
         uint32_t parts[PW][8] = {};
         for (int j = 0; j < 8; ++j) {
-          const int kk = s * kK + 8 * (l >> 4) + j, mn = 16 * t + (l & 15);
+          const int kk = as_a ? a2p_k_of(s, l, j) : s * kK + 8 * (l >> 4) + j, mn = 16 * t + (l & 15);
           const int rr = as_a ? mn : kk, cc = as_a ? kk : mn;
           const double v = (rr < rows && cc < cols) ? M[static_cast<size_t>(rr) * cols + cc] : 0.0;
           uint32_t pr[PW];
