@@ -58,14 +58,16 @@ def test_batch_errors():
 
 
 @pytest.mark.gpu
-def test_stage_timers():
+@pytest.mark.parametrize("prec", [MIXED_X6, FP64])
+def test_stage_timers(prec):
     """All five stage timers, or only the OZ pass's (mode used by the bench's
-    timed region), count every evaluation; timing never changes a result."""
+    timed region: the OZ launch records its own start and end), count every
+    evaluation; timing never changes a result."""
     import torch
 
-    sg, ag, conv = _conv((16, 16, 16), (10.0,) * 3, 3, MIXED_X6)
+    sg, ag, conv = _conv((16, 16, 16), (10.0,) * 3, 3, prec)
     xi = torch.from_numpy(1.0 + 0.1 * np.random.default_rng(4).uniform(-1, 1, (sg.N, ag.m_no))).cuda()
-    s = M.Solver(conv, RHO0, PREF, MIXED_X6)
+    s = M.Solver(conv, RHO0, PREF, prec)
     df0 = torch.empty_like(xi)
     ff0 = s.energy_cproj(xi, df0)
     ffd = torch.zeros(1, dtype=torch.float64, device="cuda")
