@@ -151,6 +151,20 @@ int mdft_delta_rho(const void* xi, void* delta_rho, size_t n, double rho0, int d
 int mdft_delta_f(const void* xi, const void* vexc, const double* w, void* delta_f, size_t nxyz, int no,
                  double rho0, double prefactor, double* ff, int dtype, void* stream);
 
+/* No counterpart in the reference.  mdft_delta_f for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same kernels and the same result bit
+ * for bit, with the partial sums and the reduced ff in a fixed device array
+ * of the library (1 MiB per GPU, held until the process ends) instead of
+ * memory from hipMallocAsync.  nxyz above MDFT_DELTA_F_NOALLOC_MAX_POINTS is
+ * rejected.  The stream must belong to the current device.  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (other streams,
+ * other host threads) race and must be serialised by the caller.  ff is a
+ * host pointer and the call synchronises the stream to fill it. */
+#define MDFT_DELTA_F_NOALLOC_MAX_POINTS 1048568
+int mdft_delta_f_noalloc(const void* xi, const void* vexc, const double* w, void* delta_f, size_t nxyz, int no,
+                         double rho0, double prefactor, double* ff, int dtype, void* stream);
+
 /* ---- Solver: one c-proj functional + gradient evaluation -------------------
  * src/mini-mdft.hpp:137-193 (Solver::energy_cproj_mrso): delta_rho ->
  * angl2proj -> Convolution::execute -> proj2angl -> get_delta_f, with

@@ -54,6 +54,7 @@ _SIGS = {
     "mdft_conv_fft_config": (_i, [_vp, _pi, _pi, _pi, _pi, _pi, _pi]),
     "mdft_delta_rho": (_i, [_vp, _vp, _sz, _d, _i, _vp]),
     "mdft_delta_f": (_i, [_vp, _vp, _vp, _vp, _sz, _i, _d, _d, _pd, _i, _vp]),
+    "mdft_delta_f_noalloc": (_i, [_vp, _vp, _vp, _vp, _sz, _i, _d, _d, _pd, _i, _vp]),
     "mdft_solver_create": (_i, [_vp, _d, _d, _i, ctypes.POINTER(_vp)]),
     "mdft_solver_workspace_bytes": (_i, [_vp, ctypes.POINTER(_sz)]),
     "mdft_solver_destroy": (_i, [_vp]),

@@ -5,7 +5,9 @@
 // common/MDFT_Asserts.hpp:15-28).
 #include <cfloat>
 #include <cstring>
+#include <algorithm>
 #include <memory>
+#include <mutex>
 #include <string>
 
 #include "../../include/mdft_hip.h"
@@ -408,6 +410,81 @@ int mdft_delta_f(const void* xi, const void* vexc, const double* w, void* delta_
     reduce_partials(part, nb, rho0 * prefactor, part + nb, s);
     MDFT_HIP_CHECK(hipMemcpyAsync(ff, part + nb, sizeof(double), hipMemcpyDeviceToHost, s));
     MDFT_HIP_CHECK(hipFreeAsync(part, s));
+    MDFT_HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// get_delta_f without allocation: the partial sums and the reduced ff live in
+// `scratch`, a file-scope device array (each GPU the library runs on holds
+// its own copy until the process ends).  Its device address on each GPU comes
+// from hipGetSymbolAddress, taken once per device with that device current;
+// the host-side name `scratch` is not a device address and is never passed to
+// a kernel or a copy.
+constexpr int kNoAllocScratchDoubles = 131072;  // 1 MiB per GPU
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+// one partial per delta_f block of grid points, plus the reduced value
+static_assert(static_cast<long>(MDFT_DELTA_F_NOALLOC_MAX_POINTS) == (kNoAllocScratchDoubles - 1) * 8L,
+              "the header's limit must match the scratch size");
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// the device address of `scratch` on the current device
+double* scratch_of_current_device() {
+  int dev = -1;
+  MDFT_HIP_CHECK(hipGetDevice(&dev));
+  throw_if(dev < 0 || dev >= kMaxScratchDevices, "delta_f_noalloc: device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    throw_if(e != hipSuccess || !p,
+             std::string("delta_f_noalloc: hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+extern "C" {
+
+int mdft_delta_f_noalloc(const void* xi, const void* vexc, const double* w, void* delta_f, size_t nxyz, int no,
+                         double rho0, double prefactor, double* ff, int dtype, void* stream) {
+  return guarded([&] {
+    // every argument and size check comes before the first HIP call
+    throw_if(!ff || no <= 0, "delta_f_noalloc: bad arguments");
+    check_dtype(dtype);
+    throw_if(nxyz > 0 && (!xi || !vexc || !w || !delta_f), "delta_f_noalloc: null argument");
+    throw_if(nxyz > static_cast<size_t>(MDFT_DELTA_F_NOALLOC_MAX_POINTS),
+             "delta_f_noalloc: nxyz = " + std::to_string(nxyz) + " exceeds the scratch limit of " +
+                 std::to_string(MDFT_DELTA_F_NOALLOC_MAX_POINTS) + " grid points");
+    throw_if(delta_f_rows_per_block() != 8, "delta_f_noalloc: scratch sized for 8 grid points per partial");
+    if (nxyz == 0) {
+      *ff = 0.0;
+      return;
+    }
+    hipStream_t s = as_stream(stream);
+    double* part = scratch_of_current_device();
+    const int nb = delta_f_blocks(static_cast<long>(nxyz));
+    // every partial [0, nb) is written by the delta_f pass and the value at
+    // part[nb] by the reduction, so nothing needs clearing between calls
+    if (dtype == MDFT_DOUBLE)
+      mdft::delta_f<double>(static_cast<const double*>(xi), static_cast<const double*>(vexc), w,
+                            static_cast<double*>(delta_f), static_cast<long>(nxyz), no, rho0, part,
+                            kNoAllocScratchDoubles - 1, s);
+    else
+      mdft::delta_f<float>(static_cast<const float*>(xi), static_cast<const float*>(vexc), w,
+                           static_cast<float*>(delta_f), static_cast<long>(nxyz), no, rho0, part,
+                           kNoAllocScratchDoubles - 1, s);
+    reduce_partials(part, nb, rho0 * prefactor, part + nb, s);
+    MDFT_HIP_CHECK(hipMemcpyAsync(ff, part + nb, sizeof(double), hipMemcpyDeviceToHost, s));
     MDFT_HIP_CHECK(hipStreamSynchronize(s));
   });
 }

@@ -662,6 +662,7 @@ void delta_rho(const T* xi, T* out, size_t n, double rho0, hipStream_t s) {
 }
 
 int delta_f_blocks(long nxyz) { return static_cast<int>((nxyz + kDfRows - 1) / kDfRows); }
+int delta_f_rows_per_block() { return kDfRows; }
 
 template <typename T>
 int delta_f(const T* xi, const T* vexc, const double* w, T* df, long nxyz, int no, double rho0, double* partials,

@@ -55,6 +55,8 @@ template <typename T>
 int delta_f(const T* xi, const T* vexc, const double* w, T* df, long nxyz, int no, double rho0, double* partials,
             int max_partials, hipStream_t s);
 int delta_f_blocks(long nxyz);
+// grid points (rows of no values) per delta_f block: one partial sum each
+int delta_f_rows_per_block();
 
 // ff = scale * sum(partials[0..count)), summed in a fixed order, into out[0];
 // with nbatch > 1, field j's count partials follow field j - 1's and its sum

@@ -122,3 +122,22 @@ def test_solvent_constants():
     n0 = M.SOLVENTS["spce"]["n0"]
     assert abs(M.rho0_of(n0, 2) - n0 * 2 / (8 * math.pi ** 2)) < 1e-18
     assert abs(M.prefactor_of(298.15, 2, n0) - (-2 * M.kT(298.15) * 4 * math.pi ** 2 / 2 / n0)) < 1e-9
+
+
+def test_delta_f_noalloc_rejects_before_any_device_call():
+    """Sizes above the scratch limit and bad arguments fail with a clear error,
+    checked on the host before the first HIP call (runs without a GPU)."""
+    L = _lib.lib()
+    ff = ctypes.c_double()
+    big = 1048568 + 1  # MDFT_DELTA_F_NOALLOC_MAX_POINTS + 1
+    one = ctypes.c_void_p(8)  # never dereferenced: the size check comes first
+    rc = L.mdft_delta_f_noalloc(one, one, one, one, big, 5, 0.1, -1.0, ctypes.byref(ff), _lib.DOUBLE, None)
+    assert rc != 0 and "exceeds the scratch limit of 1048568" in L.mdft_last_error().decode()
+    rc = L.mdft_delta_f_noalloc(one, one, one, one, 10, 0, 0.1, -1.0, ctypes.byref(ff), _lib.DOUBLE, None)
+    assert rc != 0 and "bad arguments" in L.mdft_last_error().decode()
+    rc = L.mdft_delta_f_noalloc(None, one, one, one, 10, 5, 0.1, -1.0, ctypes.byref(ff), _lib.DOUBLE, None)
+    assert rc != 0 and "null argument" in L.mdft_last_error().decode()
+    rc = L.mdft_delta_f_noalloc(one, one, one, one, 10, 5, 0.1, -1.0, ctypes.byref(ff), 7, None)
+    assert rc != 0 and "element type" in L.mdft_last_error().decode()
+    with open(_lib.HEADER) as f:
+        assert "#define MDFT_DELTA_F_NOALLOC_MAX_POINTS 1048568" in f.read()
