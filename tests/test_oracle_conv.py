@@ -243,3 +243,84 @@ def test_energy_is_the_composition(oracle):
     vexc = oracle.proj2angl(p, MMAX, MRSO).reshape(N, T["no"])
     df2, ff2 = oracle.delta_f(xi, vexc, T["w"], rho0, pref)
     assert ff == ff2 and np.array_equal(df, df2)
+
+
+# ---------------------------------------------------------------------------
+# Routing of the OZ product for m != n (common/MDFT_Convolution.hpp:273-292,
+# :466-497 and the Luc lookup h_p(m, n, mu/mrso, nu/mrso, khi),
+# common/IO/MDFT_ReadCLuc.hpp:102).  Worked out by hand from those lines: the
+# table row (m, khi, mu2, n, nu2) holds the file's tuple (m, n, mu = mrso mu2,
+# nu = mrso nu2, khi); the OZ sum for output ip = (m, khi, mu2) runs over
+# (n, nu2) and reads the rotated input at P(n, khi, |nu2|).  So with a single
+# nonzero tuple (m_t, n_t, mu_t, nu_t, +-khi_t):
+#   * gamma is nonzero only at (m_t, +-khi_t, mu_t / mrso); after the inverse
+#     rotation (a sum over khi within one (m, mu2) block) the output is
+#     nonzero only in the projections with m = m_t, mu2 = mu_t / mrso (every
+#     mup), and exactly zero elsewhere;
+#   * it reads only the rotated inputs at (n_t, +-khi_t, |nu_t| / mrso), which
+#     the forward rotation (a sum over mup within one (n, mu2) block) builds
+#     from the input projections with n = n_t, mu2 = |nu_t| / mrso alone.
+# A lookup with m and n (or mu and nu) transposed routes the output to the
+# other block and fails both checks (shown below by transposing the tuple).
+ROUTING_CASES = [
+    ((3, 1, 2, 0, 1), "nu >= 0 branch: conj(d_-q)"),
+    ((2, 3, 0, -2, 1), "nu < 0 branch: d_q, sign (-1)^(khi + nu)"),
+]
+
+
+def routing_table(mmax, mrso, nq, tup, seed=4):
+    """synthetic c(q) values kept only on the tuples (m, n, mu, nu, +-khi)."""
+    full = luc.synthetic_c(mmax, mrso, nq, seed=seed)
+    m, n, mu, nu, khi = tup
+    keep = {(m, n, mu, nu, khi), (m, n, mu, nu, -khi)}
+    mask = np.array([tuple(int(v) for v in t) in keep for t in full.tuples])
+    assert mask.sum() == 2
+    return luc.LucTable(full.tuples, np.where(mask[:, None], full.c, 0.0), full.dq)
+
+
+def routing_blocks(T, tup, mrso):
+    """(output block mask, input block mask) over ip for tuple (m, n, mu, nu, khi)."""
+    m, n, mu, nu, _ = tup
+    out_blk = (T["m"] == m) & (T["mu2"] == mu // mrso)
+    in_blk = (T["m"] == n) & (T["mu2"] == abs(nu) // mrso)
+    return out_blk, in_blk
+
+
+def check_routing(execute, T, tup, n_grid, mrso, exact=True):
+    rng = np.random.default_rng(17)
+    shape = (T["np"],) + tuple(n_grid)
+    p = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    out = execute(p)
+    out_blk, in_blk = routing_blocks(T, tup, mrso)
+    mags = np.abs(out).reshape(T["np"], -1).max(axis=1)
+    assert np.all(mags[~out_blk] == 0.0), np.nonzero(mags[~out_blk])
+    assert np.all(mags[out_blk] > 1e-6 * mags.max())
+    # inputs outside the (n, |nu| / mrso) block do not matter ...
+    q = p.copy()
+    q[~in_blk] = rng.standard_normal(q[~in_blk].shape)
+    out2 = execute(q)
+    if exact:
+        assert np.array_equal(out2, out)
+    else:
+        assert np.abs(out2 - out).max() <= 1e-13 * np.abs(out).max()
+    # ... and those inside it do (the check can fail)
+    q = p.copy()
+    q[in_blk] *= 1.5
+    assert np.abs(execute(q) - out).max() > 1e-3 * np.abs(out).max()
+
+
+@pytest.mark.parametrize("tup,why", ROUTING_CASES)
+def test_conv_routing_single_tuple(oracle, tup, why):
+    mmax, mrso, n, L = 3, 2, (5, 6, 4), (6.0, 7.0, 5.0)
+    T = oracle.angular_tables(mmax, mrso)
+    nq = luc.nq_for(luc.qmax_of_grid(n, L), DQ)
+    tab = routing_table(mmax, mrso, nq, tup)
+    C = oracle.Conv(n, L, mmax, mrso, tab.tuples, tab.c, tab.dq)
+    check_routing(C.execute, T, tup, n, mrso)
+    # the transposed tuple (n, m, nu, mu, khi) routes elsewhere: its output
+    # block is the input block above, so the support check above would fail
+    m_, n_, mu_, nu_, khi_ = tup
+    swapped = (n_, m_, abs(nu_), mu_, khi_)
+    out_a, in_a = routing_blocks(T, tup, mrso)
+    out_b, _ = routing_blocks(T, swapped, mrso)
+    assert not np.array_equal(out_a, out_b) and np.array_equal(out_b, in_a)
