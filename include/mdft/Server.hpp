@@ -7,6 +7,7 @@
 #include <array>
 #include <complex>
 #include <utility>
+#include <string>
 #include <vector>
 
 #include "Functional.hpp"
@@ -17,9 +18,11 @@ struct SolveResult {
   long long id;
   std::array<double, 4> F;  // F, F_id, F_ext, F_exc (kJ/mol)
   int iterations, evaluations;
-  int status;  // 0 converged, 1 max_iter, 2 no acceptable step, 3 rejected (budget), 4 non-finite start
+  int status;  // 0 converged, 1 max_iter, 2 no acceptable step, 3 rejected (budget), 4 non-finite start,
+               // 5 set-up failed on admission (error)
   double residual;          // max |h0 g|
   double queued_s, total_s;  // submit -> admission, submit -> end
+  std::string error;        // why admission failed (status 5), else empty
 };
 
 class Server {
@@ -62,6 +65,11 @@ class Server {
     int got = 0;
     Impl::check(mdft_server_poll(m_h, &r.id, r.F.data(), &r.iterations, &r.evaluations, &r.status, &r.residual,
                                  &r.queued_s, &r.total_s, &got));
+    if (got) {
+      char buf[1024];
+      Impl::check(mdft_server_poll_error(m_h, buf, static_cast<int>(sizeof(buf))));
+      r.error = buf;
+    }
     return got != 0;
   }
   // steps until nothing is active or waiting; the finished solves in order of completion

@@ -290,12 +290,15 @@ int mdft_lbfgs_minimize(mdft_functional_t h, double* xi, int m, int max_iter, do
  *     the slots per engine, the L-BFGS m, max_iter, gtol and the budget (B);
  *   submit: one solute (host arrays as mdft_vext takes them, copied); out_xi
  *     (device, N no doubles, may be NULL) receives the final xi; returns its id;
+ *     fails at once when the c(q) table does not cover the request (fewer q
+ *     rows than the grid's qmax needs, a tuple of mmax missing);
  *   step: admits what fits, then one tick of every engine (engines of
  *     different shapes run concurrently on their own streams); synchronises;
  *   poll: one finished solute per call (*got = 0 when none): F[4], the
  *     iterations and evaluations, status (0 converged, 1 max_iter, 2 no
  *     acceptable step, 3 rejected by the budget, 4 F or g not finite at the
- *     start point), max |h0 g|, and the seconds
+ *     start point, 5 its set-up failed on admission: engine or V_ext, the
+ *     text from mdft_server_poll_error), max |h0 g|, and the seconds
  *     from submit to admission and to the end;
  *   use_graphs: graph replay on (default) or off, before the first step. */
 typedef struct mdft_server_s* mdft_server_t;
@@ -311,6 +314,10 @@ int mdft_server_step(mdft_server_t h, int* active, int* pending);
 int mdft_server_poll(mdft_server_t h, long long* id, double* F, int* iterations, int* evaluations, int* status,
                      double* residual, double* queued_s, double* total_s, int* got);
 int mdft_server_info(mdft_server_t h, double* bytes_in_use, int* engines, int* active, int* pending);
+/* the error text of the result mdft_server_poll returned last: why its
+ * admission failed (status 5), else the empty string; truncated to buflen - 1
+ * characters */
+int mdft_server_poll_error(mdft_server_t h, char* buf, int buflen);
 
 #ifdef __cplusplus
 }

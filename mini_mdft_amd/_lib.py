@@ -80,6 +80,7 @@ _SIGS = {
     "mdft_server_step": (_i, [_vp, _pi, _pi]),
     "mdft_server_poll": (_i, [_vp, ctypes.POINTER(ctypes.c_longlong), _pd, _pi, _pi, _pi, _pd, _pd, _pd, _pi]),
     "mdft_server_info": (_i, [_vp, _pd, _pi, _pi, _pi]),
+    "mdft_server_poll_error": (_i, [_vp, ctypes.c_char_p, _i]),
 }
 
 

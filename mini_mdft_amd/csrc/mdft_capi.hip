@@ -740,6 +740,23 @@ struct mdft_server_s {
   LbfgsOptions opt;
   bool graphs = true;
   std::unique_ptr<Server> server;
+  std::string last_poll_error;  // of the result mdft_server_poll returned last
+
+  // the tuples an mmax needs (m, n <= mmax), in the table's order
+  LucTable luc_for(int mmax) const {
+    LucTable out;
+    out.nq = luc.nq;
+    out.dq = luc.dq;
+    for (int i = 0; i < luc.np_luc; ++i) {
+      const int* t = &luc.tuples[static_cast<size_t>(i) * 5];
+      if (t[0] > mmax || t[1] > mmax) continue;
+      out.tuples.insert(out.tuples.end(), t, t + 5);
+      out.c.insert(out.c.end(), luc.c.begin() + static_cast<long>(i) * luc.nq,
+                   luc.c.begin() + static_cast<long>(i + 1) * luc.nq);
+      ++out.np_luc;
+    }
+    return out;
+  }
 
   // device bytes of one solute, and an estimate of an engine's tables,
   // plans and slabs (d table, c(q), projections, trial and df slabs)
@@ -790,19 +807,8 @@ int mdft_server_create(int np_luc, const int* tuples, const double* c, int nq_ta
     mdft_server_s* sp = h.get();
     auto make = [sp](const Server::Shape& sh, size_t& table_bytes) {
       auto hold = std::make_shared<ShapeHold>();
-      // the tuples this mmax needs (m, n <= mmax), in the table's order
-      LucTable luc;
-      luc.nq = sp->luc.nq;
-      luc.dq = sp->luc.dq;
-      for (int i = 0; i < sp->luc.np_luc; ++i) {
-        const int* t = &sp->luc.tuples[static_cast<size_t>(i) * 5];
-        if (t[0] > sh.mmax || t[1] > sh.mmax) continue;
-        luc.tuples.insert(luc.tuples.end(), t, t + 5);
-        luc.c.insert(luc.c.end(), sp->luc.c.begin() + static_cast<long>(i) * luc.nq,
-                     sp->luc.c.begin() + static_cast<long>(i + 1) * luc.nq);
-        ++luc.np_luc;
-      }
-      hold->conv.reset(make_conv(sh.nx, sh.ny, sh.nz, sh.lx, sh.ly, sh.lz, sh.mmax, sp->mrso, sp->precision, luc));
+      hold->conv.reset(
+          make_conv(sh.nx, sh.ny, sh.nz, sh.lx, sh.ly, sh.lz, sh.mmax, sp->mrso, sp->precision, sp->luc_for(sh.mmax)));
       mdft_solver_t sv = nullptr;
       if (mdft_solver_create_batch(hold->conv.get(), sp->rho0, sp->prefactor, sp->precision, sp->capacity, &sv))
         throw Error(g_last_error);
@@ -846,6 +852,10 @@ int mdft_server_submit(mdft_server_t h, int nx, int ny, int nz, double lx, doubl
     throw_if(h->capacity > 1 && (static_cast<long>(nx) * ny * nz) % 64 != 0,
              "batched evaluation needs nx ny nz to be a multiple of 64");
     (void)make_angular_grid(mmax, h->mrso);  // validates mmax
+    // the c(q) table must cover the grid (q rows up to its qmax, every tuple
+    // of mmax): the host tables of the convolution are built (and dropped)
+    // here, so a request the engine could not build fails at submission
+    (void)make_conv_tables(nx, ny, nz, lx, ly, lz, mmax, h->mrso, h->luc_for(mmax));
     *id = h->server->submit(Server::Shape{nx, ny, nz, lx, ly, lz, mmax}, solute, nsolute, solvent, nsolvent, vmax,
                             out_xi);
   });
@@ -875,6 +885,17 @@ int mdft_server_poll(mdft_server_t h, long long* id, double* F, int* iterations,
     if (residual) *residual = r.r.residual;
     if (queued_s) *queued_s = r.queued_s;
     if (total_s) *total_s = r.total_s;
+    h->last_poll_error = r.error;
+  });
+}
+
+int mdft_server_poll_error(mdft_server_t h, char* buf, int buflen) {
+  return guarded([&] {
+    throw_if(!h || !buf || buflen <= 0, "server_poll_error: bad arguments");
+    const std::string& e = h->last_poll_error;
+    const size_t n = std::min(e.size(), static_cast<size_t>(buflen - 1));
+    std::memcpy(buf, e.data(), n);
+    buf[n] = '\0';
   });
 }
 

@@ -315,6 +315,12 @@ int Engine::admit(long long id, double* out_xi) {
   return j;
 }
 
+void Engine::cancel(int slot) {
+  if (slot < 0 || slot >= capacity_ || !slots_[slot]) return;
+  (void)hipStreamSynchronize(stream_);  // the slot's set-up work has ended
+  release_slot(slot);
+}
+
 double* Engine::vext_of(int slot) {
   throw_if(slot < 0 || slot >= capacity_ || !slots_[slot], "engine: no such slot");
   return slots_[slot]->V;
@@ -533,7 +539,7 @@ void Server::admit() {
     const bool have = e != engines_.end();
     const double need = static_cast<double>(fp.first) + (have ? 0.0 : static_cast<double>(fp.second));
     if (static_cast<double>(fp.first + fp.second) > budget_) {  // can never fit: reject
-      Result res{rq.id, LbfgsResult(), now_seconds() - rq.t_submit, now_seconds() - rq.t_submit, true};
+      Result res{rq.id, LbfgsResult(), now_seconds() - rq.t_submit, now_seconds() - rq.t_submit, true, ""};
       res.r.status = 3;
       results_.push_back(res);
       it = queue_.erase(it);
@@ -559,22 +565,41 @@ void Server::admit() {
         continue;
       }
     }
-    if (e == engines_.end()) {
-      Live lv;
-      lv.engine = make_(rq.shape, lv.table_bytes);
-      in_use_ += static_cast<double>(lv.table_bytes);
-      e = engines_.emplace(rq.shape, std::move(lv)).first;
+    // A request whose set-up fails (its engine cannot be built, its V_ext
+    // cannot be computed) ends with status 5 and the error text; it leaves
+    // the queue, so it cannot stall the requests behind it or the running
+    // solves.
+    int slot = -1;
+    Engine* eng = nullptr;
+    try {
+      if (e == engines_.end()) {
+        Live lv;
+        lv.engine = make_(rq.shape, lv.table_bytes);
+        in_use_ += static_cast<double>(lv.table_bytes);
+        e = engines_.emplace(rq.shape, std::move(lv)).first;
+      }
+      eng = e->second.engine.get();
+      slot = eng->admit(rq.id, rq.out_xi);
+      // V_ext of the solute into the slot, on the engine's stream
+      compute_vext(rq.shape.nx, rq.shape.ny, rq.shape.nz, rq.shape.lx, rq.shape.ly, rq.shape.lz,
+                   make_angular_grid(rq.shape.mmax, mrso_), rq.solute.data(), static_cast<int>(rq.solute.size() / 6),
+                   rq.solvent.data(), static_cast<int>(rq.solvent.size() / 6), rq.vmax, eng->vext_of(slot),
+                   eng->stream());
+    } catch (const std::exception& ex) {
+      // a failed runtime call (an allocation) stays the thread's last HIP
+      // error; clear it so that the next launch check does not report it
+      (void)hipGetLastError();
+      if (eng && slot >= 0) eng->cancel(slot);
+      const double t = now_seconds();
+      Result res{rq.id, LbfgsResult(), t - rq.t_submit, t - rq.t_submit, true, ex.what()};
+      res.r.status = 5;
+      results_.push_back(res);
+      it = queue_.erase(it);
+      continue;
     }
-    Engine& eng = *e->second.engine;
-    const int slot = eng.admit(rq.id, rq.out_xi);
-    in_use_ += static_cast<double>(eng.job_bytes());
+    in_use_ += static_cast<double>(eng->job_bytes());
     e->second.admitted_at[rq.id] = now_seconds();
     running_[rq.id] = rq;
-    // V_ext of the solute into the slot, on the engine's stream
-    compute_vext(rq.shape.nx, rq.shape.ny, rq.shape.nz, rq.shape.lx, rq.shape.ly, rq.shape.lz,
-                 make_angular_grid(rq.shape.mmax, mrso_), rq.solute.data(), static_cast<int>(rq.solute.size() / 6),
-                 rq.solvent.data(), static_cast<int>(rq.solvent.size() / 6), rq.vmax, eng.vext_of(slot),
-                 eng.stream());
     it = queue_.erase(it);
   }
 }
@@ -590,7 +615,7 @@ void Server::step() {
     for (auto& d : lv.engine->take_done()) {
       const Request& rq = running_[d.id];
       const double t = now_seconds();
-      results_.push_back(Result{d.id, d.r, lv.admitted_at[d.id] - rq.t_submit, t - rq.t_submit, false});
+      results_.push_back(Result{d.id, d.r, lv.admitted_at[d.id] - rq.t_submit, t - rq.t_submit, false, ""});
       lv.admitted_at.erase(d.id);
       running_.erase(d.id);
       in_use_ -= static_cast<double>(lv.engine->job_bytes());

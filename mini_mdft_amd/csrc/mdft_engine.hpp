@@ -6,6 +6,7 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <string>
 #include <tuple>
 
 #include "mdft_functional.hpp"
@@ -44,6 +45,8 @@ class Engine {
   // When out_xi (device, N no doubles) is given, the final point is copied
   // there when the solve ends.
   int admit(long long id, double* out_xi = nullptr);
+  // gives an admitted slot back before its first tick (its set-up failed)
+  void cancel(int slot);
   double* vext_of(int slot);
   // one trial (or start-point evaluation) for every active slot; returns the
   // number of slots that finished in this tick.  begin_tick enqueues the
@@ -119,6 +122,7 @@ class Server {
     LbfgsResult r;
     double queued_s, total_s;  // submit -> admit, submit -> done
     bool rejected;
+    std::string error;  // why admission failed (status 5), else empty
   };
   // builds the engines of a shape: (shape) -> engine and the bytes it holds
   // besides its slots (tables, plans, work buffers)

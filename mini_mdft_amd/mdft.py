@@ -471,7 +471,8 @@ class Server:
     pass of every stage, replayed as one HIP graph.  `table` is the c(q)
     (mini_mdft_amd.luc.LucTable) for every mmax that will be requested."""
 
-    STATUS = {0: "converged", 1: "max_iter", 2: "line search", 3: "rejected (HBM budget)", 4: "non-finite start"}
+    STATUS = {0: "converged", 1: "max_iter", 2: "line search", 3: "rejected (HBM budget)", 4: "non-finite start",
+              5: "admission failed (see error)"}
 
     def __init__(self, table, mrso: int, n0: float, temperature: float, precision: int = MIXED_X6,
                  capacity: int = 4, m: int = 5, max_iter: int = 400, gtol: float = 1e-5,
@@ -526,8 +527,10 @@ class Server:
         if not got.value:
             return None
         self._outs.pop(rid.value, None)
+        err = ctypes.create_string_buffer(1024)
+        check(lib().mdft_server_poll_error(self._h, err, len(err)))
         return dict(id=rid.value, F=tuple(F), iterations=it.value, evaluations=nev.value, status=st.value,
-                    residual=res.value, queued_s=tq.value, total_s=tt.value)
+                    residual=res.value, queued_s=tq.value, total_s=tt.value, error=err.value.decode())
 
     def info(self):
         b, e, a, p = ctypes.c_double(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

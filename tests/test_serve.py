@@ -119,7 +119,7 @@ def test_mixed_shapes_under_a_budget():
     """Two shapes and a budget for about two solutes at a time: everything
     finishes, the footprint never exceeds the budget, and a request larger
     than the budget is rejected (status 3)."""
-    tab = _table(nmax=3, n=(16, 16, 16), L=(12.0,) * 3)
+    tab = _table(nmax=5, n=(64, 64, 64), L=(30.0,) * 3)  # covers the large request too
     shapes = [((8, 8, 8), (10.0,) * 3, 3), ((16, 8, 4), (12.0, 8.0, 6.0), 2)]
     probe = M.Server(tab, MRSO, N0, TEMP, MIXED_X6, capacity=2, budget_bytes=1e12)
     probe.submit(*shapes[0], _solutes(1)[0], WATER, VMAX)
@@ -144,3 +144,39 @@ def test_mixed_shapes_under_a_budget():
     assert done[big]["status"] == 3
     assert all(done[i]["status"] == 0 for i in ids)
     assert 0 < peak <= budget
+
+
+def test_submit_rejects_requests_the_table_does_not_cover():
+    """A grid finer than the c(q) table (more q rows than it holds) or an mmax
+    above its tuples fails at submission (host checks, no device work), so it
+    never enters the queue."""
+    tab = _table()  # rows for the 8^3 / 10 A grid, tuples up to nmax 3
+    srv = M.Server(tab, MRSO, N0, TEMP, FP64, capacity=1)
+    with pytest.raises(M.MDFTError, match="fewer q rows"):
+        srv.submit((16, 16, 16), (5.0,) * 3, MMAX, _solutes(1)[0], WATER, VMAX)
+    with pytest.raises(M.MDFTError, match="lacks a tuple"):
+        srv.submit(N3, L3, 5, _solutes(1)[0], WATER, VMAX)
+    assert srv.info()["pending"] == 0
+
+
+@pytest.mark.gpu
+def test_failed_admission_does_not_stall_the_server():
+    """A request that passes the submission checks but whose engine cannot be
+    built (its projections need ~480 GB of device memory; the budget is set
+    above that on purpose) ends with status 5 and the error text, and the
+    valid solute submitted before it and the one after it both finish."""
+    import torch
+
+    tab = _table(nmax=3, n=(256, 256, 256), L=(200.0,) * 3)
+    no = M.AngularGrid(MMAX, MRSO).m_no
+    srv = M.Server(tab, MRSO, N0, TEMP, FP64, capacity=64, budget_bytes=1e16)
+    outs = [torch.zeros((512, no), dtype=torch.float64, device="cuda") for _ in range(2)]
+    sols = _solutes(2)
+    a = srv.submit(N3, L3, MMAX, sols[0], WATER, VMAX, out=outs[0])
+    bad = srv.submit((256, 256, 256), (200.0,) * 3, MMAX, sols[1], WATER, VMAX)
+    b = srv.submit(N3, L3, MMAX, sols[1], WATER, VMAX, out=outs[1])
+    res = srv.run()
+    print({k: (v["status"], v["error"][:80]) for k, v in res.items()}, flush=True)
+    assert res[bad]["status"] == 5 and "out of memory" in res[bad]["error"].lower()
+    assert res[a]["status"] == 0 and res[b]["status"] == 0
+    assert srv.info()["pending"] == 0 and srv.info()["active"] == 0
