@@ -48,6 +48,9 @@ TEMP, N0 = 298.15, 0.0332891
 C_SEED, XI_SEED, XI_AMP = 2024, 12345, 0.1
 OZ_FLOP_PER_PAIR = 52344  # rotations, OZ product, phases (both members of the pair)
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "oz_pairs_64c_mmax5_traffic.md")
+MFMA_SUMMARY = os.path.join(ROOT, "profiles", "a2p_gemm_64c_mmax5_mfma.md")
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16, vendor figure
+A2P_ALGO_FLOP = 4 * 88 * 330 * 64 ** 3  # dense [N x no] . [no x np] complex contraction, one direction
 # the solve leg: 20 sites within 5 A of the centre, charge scale 0.1, V_ext
 # capped at 100 kT, L-BFGS with 3 pairs to max |h0 g| < 1e-5 (3 pairs take
 # as many iterations as 5 here, 80-91, and move 4 fewer pair fields per pass)
@@ -65,9 +68,9 @@ def stage_bytes(N, no, np_, form):
     P = N * np_ * (16 if form == "fp64" else 8)  # the projections
     return {
         "angl2proj": O + P,
-        "fft_forward": 3 * 2 * P,  # z, y and x passes
+        "fft_forward": 2 * P,
         "oz": 2 * P,
-        "fft_backward": 3 * 2 * P,
+        "fft_backward": 2 * P,
         "proj2angl_delta_f": P + 2 * O,
     }
 
@@ -101,24 +104,67 @@ def timed(solver, xi, df, ffd, steps, warmup):
     return (t1 - t0), stages, oz_ms
 
 
+def host_cpu():
+    """(model name, logical CPUs this process may run on)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 1
+    return model, ncpu
+
+
+# the reference's stage boundaries (src/mini-mdft.hpp:140-183), as the oracle times them
+CPU_STAGES = ("delta_rho", "angl2proj", "convolution", "proj2angl", "delta_f")
+
+
 def cpu_baseline(tab, n, L, rho0, pref, evals):
+    """The FP64 oracle (C++/OpenMP restatement of the reference path) on the
+    host: 1 warm-up, then `evals` timed evaluations of the same workload; the
+    median and p99 of every stage and of the whole evaluation.  Threads: every
+    logical CPU this process may use, capped by OMP_NUM_THREADS when that is
+    set (a shared host gives each GPU a share of its cores)."""
+    import numpy as np
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     from mini_mdft_amd import rng
 
-    threads = min(16, os.cpu_count() or 1)
+    model, ncpu = host_cpu()
+    threads = ncpu
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        threads = max(1, min(threads, int(os.environ["OMP_NUM_THREADS"])))
     O.set_num_threads(threads)
     C = O.Conv(n, L, MMAX, MRSO, tab.tuples, tab.c, tab.dq)
     xi = rng.xi_field(XI_SEED, (n[0] * n[1] * n[2], 330), XI_AMP)
     C.energy_cproj(xi, rho0, pref)  # warm-up
-    t0 = time.perf_counter()
+    totals, stages = [], []
     for _ in range(evals):
-        ff, df = C.energy_cproj(xi, rho0, pref)
-    dt = time.perf_counter() - t0
-    return dict(value=round(evals / dt, 4), unit="evals/s", cores=threads, kind="port",
-                sample=f"{evals} full FP64 evaluations of the same 64^3 mmax=5 workload after 1 warm-up "
-                       f"(oracle: C++/OpenMP restatement of the reference, {threads} threads, {dt:.1f} s)"), ff, df
+        t0 = time.perf_counter()
+        ff, df, st = C.energy_cproj(xi, rho0, pref, stages=True)
+        totals.append(time.perf_counter() - t0)
+        stages.append(st)
+    stages = np.asarray(stages) * 1e3
+    per_stage = {k: {"median_ms": round(float(np.median(stages[:, i])), 2),
+                     "p99_ms": round(float(percentile(list(stages[:, i]), 99)), 2)}
+                 for i, k in enumerate(CPU_STAGES)}
+    med = float(np.median(totals))
+    return dict(value=round(1.0 / med, 4), unit="evals/s", cores=threads, kind="port",
+                sample=f"{evals} timed FP64 evaluations of the same 64^3 mmax=5 workload after 1 warm-up "
+                       f"(oracle: C++/OpenMP restatement of the reference path, not the Kokkos+FFTW build; "
+                       f"{threads} threads; value = 1 / median)",
+                cpu_model=model, logical_cpus_visible=os.cpu_count(), logical_cpus_allowed=ncpu,
+                median_ms=round(med * 1e3, 1), p99_ms=round(percentile(totals, 99) * 1e3, 1),
+                stages=per_stage), ff, df
 
 
 def percentile(v, q):
@@ -283,6 +329,19 @@ def stream_leg(seconds, rate=2.0, seed=11):
                         "after the window"}
 
 
+def mfma_from_summary():
+    """(BF16 MFMA flops per angl2proj launch, busy fraction) from the committed
+    rocprofv3 counter summary, or (None, None)."""
+    try:
+        with open(MFMA_SUMMARY) as f:
+            text = f.read()
+        fl = re.search(r"mfma_bf16_flops_per_launch\s*=\s*([0-9.eE+]+)", text)
+        bz = re.search(r"mfma_busy_frac\s*=\s*([0-9.eE+]+)", text)
+        return (float(fl.group(1)) if fl else None), (float(bz.group(1)) if bz else None)
+    except OSError:
+        return None, None
+
+
 def traffic_from_summary():
     """HBM bytes per OZ launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary (None if absent)."""
@@ -300,8 +359,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-evals", type=int, default=2, help="timed CPU-baseline evaluations (0: skip)")
-    ap.add_argument("--solves", type=int, default=8, help="seeded solutes of the p99_solve leg (0: skip)")
+    ap.add_argument("--cpu-evals", type=int, default=5, help="timed CPU-baseline evaluations (0: skip)")
+    ap.add_argument("--solves", type=int, default=32, help="seeded solutes per form of the p99_solve leg (0: skip)")
     ap.add_argument("--lbfgs-m", type=int, default=SOLVE_M, help="stored L-BFGS pairs of the solve legs")
     ap.add_argument("--serve", type=int, default=64, help="solutes of the mixed-size batch leg (0: skip)")
     ap.add_argument("--stream", type=float, default=30.0, help="seconds of the Poisson request stream (0: skip)")
@@ -364,6 +423,13 @@ def main():
 
     by = stage_bytes(N, no, np_, "mixed")
     stages = {k: dict(ms=round(ms, 5), GBs=round(by[k] / (ms * 1e-3) / 1e9, 1)) for k, ms in x6["stages"].items()}
+    # a stage above the HBM peak by its algorithmic bytes would mean a wrong byte model
+    over_peak = [k for k, v in stages.items() if v["GBs"] > HBM_PEAK_GBS]
+    if over_peak:
+        print(f"[bench] WARNING: stages above the HBM peak by algorithmic bytes: {over_peak}", file=sys.stderr)
+    by64 = stage_bytes(N, no, np_, "fp64")
+    stages64 = {k: dict(ms=round(ms, 5), GBs=round(by64[k] / (ms * 1e-3) / 1e9, 1))
+                for k, ms in f64["stages"].items()}
     eval_bytes = sum(by.values())
     # dominant kernel: the OZ pass (one launch per evaluation); its time is
     # the interval between the start and end events that the launch records
@@ -377,7 +443,22 @@ def main():
                     peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
                     traffic=traffic_from_summary(), algorithmic_bytes=oz_bytes, launch_ms=round(oz_ms, 5),
                     fp32_tflops=round(oz_tflops, 2), fp32_frac=round(oz_tflops / FP32_PEAK_TFLOPS, 4),
-                    eval_hbm_frac=round(eval_bytes * value / 1e9 / HBM_PEAK_GBS, 4))
+                    eval_hbm_frac=round(eval_bytes * value / 1e9 / HBM_PEAK_GBS, 4),
+                    eval_bytes=eval_bytes)
+    # the projection GEMM on the matrix cores: executed BF16 flops per launch
+    # from the counter summary over the live angl2proj time of this run
+    mflops, mbusy = mfma_from_summary()
+    a2p_ms = x6["stages"]["angl2proj"]
+    roofline_mfma = None
+    if mflops:
+        ach = mflops / (a2p_ms * 1e-3) / 1e12
+        roofline_mfma = dict(kernel="a2p_gemm_kernel<3,3,12> (BF16x6 angl2proj)", bound="mfma",
+                             achieved=round(ach, 1), peak=MFMA_BF16_PEAK_TFLOPS, unit="TFLOP/s",
+                             frac=round(ach / MFMA_BF16_PEAK_TFLOPS, 4), busy_frac_counter=mbusy,
+                             executed_flop_per_launch=mflops, algorithmic_flop=A2P_ALGO_FLOP,
+                             algorithmic_tflops=round(A2P_ALGO_FLOP / (a2p_ms * 1e-3) / 1e12, 1),
+                             launch_ms=round(a2p_ms, 5), hbm_GBs=stages["angl2proj"]["GBs"],
+                             source=os.path.relpath(MFMA_SUMMARY, ROOT))
 
     p99 = None
     if args.solves > 0:
@@ -435,12 +516,14 @@ def main():
         "bf16x3": {"evals_per_s": round(args.steps / x3["sec"], 2), "dF_vs_fp64_kJmol": dF3,
                    "grad_rel_l2_vs_fp64": grad_rel3, "meets_dF_bar": abs(dF3) <= 0.01},
         "stages_ms_GBs": stages,
-        "stages_fp64_ms": {k: round(v, 5) for k, v in f64["stages"].items()},
+        "stages_fp64_ms_GBs": stages64,
+        "fp64_eval_hbm_frac": round(sum(by64.values()) * fp64_rate / 1e9 / HBM_PEAK_GBS, 4),
         "fft_autotuned": {"mixed_x6": x6["fft"], "fp64": f64["fft"]},
         "p99_solve": p99,
         "served_batch": served,
         "served_stream": streamed,
         "roofline": roofline,
+        "roofline_mfma": roofline_mfma,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line, ensure_ascii=False))
