@@ -207,6 +207,18 @@ int mdft_energy_cproj_batch_async(mdft_solver_t h, int nbatch, const void* xi, v
  * backward 3D FFT, proj2angl + delta_f, and the number of evaluations in
  * *count. */
 int mdft_solver_set_timing(mdft_solver_t h, int enable);
+
+/* No counterpart in the reference (its path is FP64 throughout).  The r-mean
+ * (q = 0) mode of ff evaluated in FP64: proj2angl's epilogue also sums
+ * xi^2 - 1 and vexc over r for every orientation, the q = 0 value is carried
+ * through angl2proj, the OZ product at q = 0 and proj2angl in FP64, and it
+ * replaces the mixed path's r-mean contribution to ff (df is unchanged).  With
+ * xi near 1 this mode carries almost all of ff, and an FP32 path's ~1e-7
+ * relative error on it would reach ff whole.  On by default for MDFT_MIXED_X6;
+ * available (off by default) for MDFT_FP64 and MDFT_FP32, where it changes ff
+ * only by rounding; not available for MDFT_MIXED (proj2angl on the matrix
+ * cores) or angular grids without a compile-time proj2angl kernel. */
+int mdft_solver_set_mean_fp64(mdft_solver_t h, int on);
 int mdft_solver_stage_ms(mdft_solver_t h, double* ms, long* count);
 
 /* ---- utilities --------------------------------------------------------------

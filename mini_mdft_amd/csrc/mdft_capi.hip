@@ -12,6 +12,7 @@
 
 #include "../../include/mdft_hip.h"
 #include "mdft_conv.hpp"
+#include "mdft_dc.hpp"
 #include "mdft_engine.hpp"
 #include "mdft_functional.hpp"
 #include "mdft_gemm.hpp"
@@ -87,12 +88,47 @@ struct mdft_solver_s {
   double* ffdev = nullptr;
   size_t bytes = 0;
   std::unique_ptr<StageTimer> timer;  // set while stage timing is on
+  // the r-mean (q = 0) mode of ff in FP64 (on by default for MDFT_MIXED_X6)
+  bool mean_on = false;
+  std::unique_ptr<MeanModeFp64> mean;
+  double* colpart = nullptr;  // proj2angl column sums: row blocks x 2 no
+  double* meanwork = nullptr;
+  double* extra = nullptr;    // capacity x no terms
+  size_t mean_bytes = 0;
   ~mdft_solver_s() {
     if (pbuf) (void)hipFree(pbuf);
     if (partials) (void)hipFree(partials);
     if (ffdev) (void)hipFree(ffdev);
+    if (colpart) (void)hipFree(colpart);
+    if (meanwork) (void)hipFree(meanwork);
+    if (extra) (void)hipFree(extra);
   }
 };
+
+namespace {
+
+// whether this solver's proj2angl can write the column sums of the mean mode
+bool mean_mode_supported(const mdft_solver_s* h) {
+  if (h->precision == MDFT_MIXED) return false;  // proj2angl on the matrix cores
+  if (h->gemm) return h->gemm->has_column_sums();
+  return proj2angl_has_column_sums(*h->tab);
+}
+
+void enable_mean_mode(mdft_solver_s* h) {
+  if (h->mean) return;
+  const ConvTables& t = h->conv->dev->tab;
+  h->mean = std::make_unique<MeanModeFp64>(t.ang, t, h->rho0);
+  const size_t ncol = static_cast<size_t>(proj2angl_row_blocks(h->N * h->capacity)) * 2 * t.ang.no;
+  const size_t nwork = MeanModeFp64::work_doubles(h->N, t.ang.no, h->capacity);
+  MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&h->colpart), ncol * sizeof(double)));
+  MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&h->meanwork), nwork * sizeof(double)));
+  const size_t nextra = static_cast<size_t>(h->capacity) * h->mean->extra_per_field();
+  MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&h->extra), nextra * sizeof(double)));
+  h->mean_bytes = (ncol + nwork + nextra) * sizeof(double);
+  h->bytes += h->mean_bytes;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -517,6 +553,10 @@ int mdft_solver_create_batch(mdft_conv_t conv, double rho0, double prefactor, in
     MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&h->partials), h->nparts * sizeof(double)));
     MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&h->ffdev), capacity * sizeof(double)));
     h->bytes = pbytes + (h->nparts + capacity) * sizeof(double);
+    if (precision == MDFT_MIXED_X6 && mean_mode_supported(h.get())) {
+      enable_mean_mode(h.get());
+      h->mean_on = true;
+    }
     *out = h.release();
   });
 }
@@ -548,6 +588,7 @@ static void energy_impl(mdft_solver_t h, int nb, const void* xi, void* df, doubl
   const double rho0 = h->rho0;
   const double eps = chop_eps_of(h->precision);
   const long Nb = h->N * nb;
+  double* colp = h->mean_on ? h->colpart : nullptr;
   StageTimer* tm = h->timer.get();
   if (tm) {
     tm->begin_eval();
@@ -559,7 +600,7 @@ static void energy_impl(mdft_solver_t h, int nb, const void* xi, void* df, doubl
       angl2proj_factorized<double, double, double>(*h->tab, Nb, static_cast<const double*>(xi), p, &rho0, s);
       h->conv->dev->execute<double>(p, eps, s, tm, nb);
       proj2angl_factorized<double, double, double>(*h->tab, Nb, p, nullptr, static_cast<const double*>(xi),
-                                                   static_cast<double*>(df), rho0, h->partials, s);
+                                                   static_cast<double*>(df), rho0, h->partials, s, colp);
       break;
     }
     case MDFT_FP32: {
@@ -567,7 +608,7 @@ static void energy_impl(mdft_solver_t h, int nb, const void* xi, void* df, doubl
       angl2proj_factorized<float, float, float>(*h->tab, Nb, static_cast<const float*>(xi), p, &rho0, s);
       h->conv->dev->execute<float>(p, eps, s, tm, nb);
       proj2angl_factorized<float, float, float>(*h->tab, Nb, p, nullptr, static_cast<const float*>(xi),
-                                                static_cast<float*>(df), rho0, h->partials, s);
+                                                static_cast<float*>(df), rho0, h->partials, s, colp);
       break;
     }
     default: {
@@ -575,12 +616,26 @@ static void energy_impl(mdft_solver_t h, int nb, const void* xi, void* df, doubl
       h->gemm->angl2proj(Nb, static_cast<const double*>(xi), p, &rho0, s);
       h->conv->dev->execute<float>(p, eps, s, tm, nb);
       h->gemm->proj2angl(Nb, p, nullptr, static_cast<const double*>(xi), static_cast<double*>(df), rho0, h->partials,
-                         s);
+                         s, colp);
       break;
     }
   }
-  reduce_partials(h->partials, h->field_parts, rho0 * h->prefactor, ffd, s, nb, ffstride);
+  if (colp) h->mean->run(colp, h->N, nb, h->meanwork, h->extra, s);
+  reduce_partials(h->partials, h->field_parts, rho0 * h->prefactor, ffd, s, nb, ffstride, colp ? h->extra : nullptr,
+                  colp ? h->mean->extra_per_field() : 0);
   if (tm) tm->mark(s, kStageEnd);
+}
+
+int mdft_solver_set_mean_fp64(mdft_solver_t h, int on) {
+  return guarded([&] {
+    throw_if(!h, "null handle");
+    if (on) {
+      throw_if(!mean_mode_supported(h),
+               "solver: the FP64 mean mode needs proj2angl on the vector ALUs and a compile-time angular grid");
+      enable_mean_mode(h);
+    }
+    h->mean_on = on != 0;
+  });
 }
 
 int mdft_solver_set_timing(mdft_solver_t h, int enable) {
@@ -768,7 +823,9 @@ struct mdft_server_s {
     const size_t proj = N * ag.np * capacity * (precision == MDFT_FP64 ? 16 : 8);
     const size_t dtab = (N / 2 + 8) * static_cast<size_t>((sh.mmax + 1) * (sh.mmax + 1)) * (2 * sh.mmax + 1) * ts;
     const size_t slabs = 2 * static_cast<size_t>(capacity) * n * 8 + n * ts;
-    return {job, proj + dtab + slabs + (size_t(64) << 20)};
+    // the column sums of the FP64 mean mode (MDFT_MIXED_X6)
+    const size_t colp = precision == MDFT_MIXED_X6 ? (N * capacity / 64 + 1) * 2 * ag.no * 8 : 0;
+    return {job, proj + dtab + slabs + colp + (size_t(64) << 20)};
   }
 };
 

@@ -427,6 +427,32 @@ void dispatch_split(int px, int pw, F&& f) {
 
 }  // namespace
 
+void projection_matrices(const AngularGrid& g, std::vector<double>& wa, std::vector<double>& wp) {
+  const int nt = g.ntheta, nph = g.nphi, nps = g.npsi, np = g.np, no = g.no;
+  wa.assign(static_cast<size_t>(2 * np) * no, 0.0);
+  wp.assign(static_cast<size_t>(2 * np) * no, 0.0);
+  const double inv = 1.0 / static_cast<double>(nph * nps);
+  for (int ip = 0; ip < np; ++ip) {
+    const int m = g.m[ip], mup = g.mup[ip], mu2 = g.mu2[ip];
+    for (int th = 0; th < nt; ++th) {
+      const double dv = g.wigner_d[th * np + ip];
+      const double af = g.fm[m] * g.wtheta[th] * dv * inv;
+      const double bf = dv * g.fm[m] * (mu2 == 0 ? 1.0 : 2.0);
+      for (int j = 0; j < nph; ++j)
+        for (int k = 0; k < nps; ++k) {
+          // angles reduced exactly: mup j mod nphi and mu2 k mod npsi
+          const int tj = ((mup * j) % nph + nph) % nph, tk = (mu2 * k) % nps;
+          const double a = 2.0 * M_PI * (static_cast<double>(tj) / nph + static_cast<double>(tk) / nps);
+          const int io = (th * nph + j) * nps + k;
+          wa[static_cast<size_t>(2 * ip) * no + io] = af * std::cos(a);
+          wa[static_cast<size_t>(2 * ip + 1) * no + io] = af * std::sin(a);
+          wp[static_cast<size_t>(2 * ip) * no + io] = bf * std::cos(a);
+          wp[static_cast<size_t>(2 * ip + 1) * no + io] = bf * std::sin(a);
+        }
+    }
+  }
+}
+
 struct ProjGemm::Impl {
   int px, pw;
   bool p2a_mfma;  // proj2angl on the matrix cores (BF16x3) or the FP32 VALU kernel
@@ -456,7 +482,7 @@ ProjGemm::ProjGemm(const AngularGrid& g, int data_parts, int weight_parts, bool 
   I.p2a_mfma = p2a_mfma;
   I.no = g.no;
   I.np = g.np;
-  const int nt = g.ntheta, nph = g.nphi, nps = g.npsi, np = g.np, no = g.no;
+  const int nps = g.npsi, np = g.np, no = g.no;
   throw_if(nps % 2 == 0, "npsi must be odd");
   I.a_mt = tile_bucket((2 * np + 15) / 16);
   I.a_kt = (no + kK - 1) / kK;
@@ -465,28 +491,8 @@ ProjGemm::ProjGemm(const AngularGrid& g, int data_parts, int weight_parts, bool 
   I.a2p_fits = I.a_mt <= kMaxA2pTiles && no >= 2;
   I.p2a_fits = I.p_mt <= kMaxP2aTiles;
   if (!I.a2p_gemm() || !I.p2a_gemm()) I.fb = std::make_unique<ProjTables>(g);
-  // Wa: (2np) x no; Wp: (2np) x no as well (k = 2ip + c, n = io)
-  std::vector<double> wa(static_cast<size_t>(2 * np) * no), wp(static_cast<size_t>(2 * np) * no);
-  const double inv = 1.0 / static_cast<double>(nph * nps);
-  for (int ip = 0; ip < np; ++ip) {
-    const int m = g.m[ip], mup = g.mup[ip], mu2 = g.mu2[ip];
-    for (int th = 0; th < nt; ++th) {
-      const double dv = g.wigner_d[th * np + ip];
-      const double af = g.fm[m] * g.wtheta[th] * dv * inv;
-      const double bf = dv * g.fm[m] * (mu2 == 0 ? 1.0 : 2.0);
-      for (int j = 0; j < nph; ++j)
-        for (int k = 0; k < nps; ++k) {
-          // angles reduced exactly: mup j mod nphi and mu2 k mod npsi
-          const int tj = ((mup * j) % nph + nph) % nph, tk = (mu2 * k) % nps;
-          const double a = 2.0 * M_PI * (static_cast<double>(tj) / nph + static_cast<double>(tk) / nps);
-          const int io = (th * nph + j) * nps + k;
-          wa[static_cast<size_t>(2 * ip) * no + io] = af * std::cos(a);
-          wa[static_cast<size_t>(2 * ip + 1) * no + io] = af * std::sin(a);
-          wp[static_cast<size_t>(2 * ip) * no + io] = bf * std::cos(a);
-          wp[static_cast<size_t>(2 * ip + 1) * no + io] = bf * std::sin(a);
-        }
-    }
-  }
+  std::vector<double> wa, wp;
+  projection_matrices(g, wa, wp);
   if (I.a2p_gemm())
     dispatch_split(I.px, I.pw, [&](auto, auto pwc) {
       I.wa = upload(swizzle<decltype(pwc)::value>(wa, 2 * np, no, true, I.a_mt));
@@ -534,12 +540,15 @@ int ProjGemm::p2a_blocks(long N) const {
   return static_cast<int>((N + rows - 1) / rows);
 }
 
+bool ProjGemm::has_column_sums() const { return !impl_->p2a_gemm() && proj2angl_has_column_sums(*impl_->fb); }
+
 int ProjGemm::proj2angl(long N, const cplx<float>* p, double* o, const double* xi, double* df, double rho0,
-                        double* partials, hipStream_t s) {
+                        double* partials, hipStream_t s, double* colpart) {
   if (N == 0) return 0;
   const Impl& I = *impl_;
+  throw_if(colpart != nullptr && !has_column_sums(), "proj2angl: no column sums in this form");
   if (!I.p2a_gemm()) {
-    proj2angl_factorized<float, double, float>(*I.fb, N, p, o, xi, df, rho0, partials, s);
+    proj2angl_factorized<float, double, float>(*I.fb, N, p, o, xi, df, rho0, partials, s, colpart);
     return proj2angl_blocks(*I.fb, N);
   }
   const GemmDims d{N, I.no, I.np, I.p_kt};

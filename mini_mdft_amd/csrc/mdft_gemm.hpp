@@ -2,10 +2,19 @@
 #pragma once
 
 #include <memory>
+#include <vector>
 
 #include "mdft_proj.hpp"
 
 namespace mdft {
+
+// The dense FP64 projection matrices, (2 np) x no each (rows 2 ip + (re, im)):
+// angl2proj p[ip] = sum_io (wa[2ip][io] + i wa[2ip+1][io]) o[io] (the rfft2,
+// unscramble and theta quadrature of
+// common/MDFT_OrientationProjectionTransform.hpp:267-380 in one matrix), and
+// proj2angl o[io] = sum_ip Re p[ip] wp[2ip][io] + Im p[ip] wp[2ip+1][io]
+// (:388-484 with the c2r "Re" convention).
+void projection_matrices(const AngularGrid& g, std::vector<double>& wa, std::vector<double>& wp);
 
 struct ProjGemm {
   // angl2proj split (data parts, weight parts): (2, 2) BF16x3, three products
@@ -22,9 +31,12 @@ struct ProjGemm {
   void angl2proj(long N, const double* o, cplx<float>* p, const double* rho0, hipStream_t s);
   // p -> o; with xi != nullptr get_delta_f runs in the epilogue instead of
   // storing o.  Returns the number of FP64 partials written.
+  // colpart: per-row-block column sums (proj2angl_factorized), FP32 VALU
+  // proj2angl on compile-time grids only (has_column_sums)
   int proj2angl(long N, const cplx<float>* p, double* o, const double* xi, double* df, double rho0,
-                double* partials, hipStream_t s);
+                double* partials, hipStream_t s, double* colpart = nullptr);
   int p2a_blocks(long N) const;
+  bool has_column_sums() const;
 
  private:
   struct Impl;

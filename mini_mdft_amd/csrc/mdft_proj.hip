@@ -260,7 +260,8 @@ __global__ __launch_bounds__(64) void proj2angl_wave_kernel(const cplx<TP>* __re
                                                             const TO* __restrict__ xi, TO* __restrict__ df,
                                                             const double* __restrict__ wq, long N, long nrb,
                                                             TabPtrs<TC> tb, double rho0,
-                                                            double* __restrict__ partials) {
+                                                            double* __restrict__ partials,
+                                                            double* __restrict__ colpart) {
   using A = AngSizes<MMAX, MRSO>;
   constexpr int NP = A::np();
   __shared__ TC stage[kWaveRows * A::kNjk];
@@ -338,6 +339,48 @@ __global__ __launch_bounds__(64) void proj2angl_wave_kernel(const cplx<TP>* __re
   double acc_ff = 0.0;
   const long base = r0 * A::kNo + th * A::kNjk;
   constexpr int kE = kWaveRows * A::kNjk / 64;  // elements per lane
+  if (colpart != nullptr && xi != nullptr) {
+    // get_delta_f with the column sums: lane jk owns orientation th * nphi
+    // npsi + jk and walks the block's rows, so that its sums over r of
+    // xi^2 - 1 and of vexc build up in registers (each row's 55 values are
+    // one contiguous run of xi and df)
+    constexpr int kRB = 32;  // rows per batch of loads in flight
+    const int nrow = static_cast<int>(min(static_cast<long>(kWaveRows), N - r0));
+    const bool on = lane < A::kNjk;
+    const int jk = on ? lane : 0;
+    double cs = 0.0, cv = 0.0;
+    for (int rl0 = 0; rl0 < kWaveRows; rl0 += kRB) {
+      double xv[kRB];
+#pragma unroll
+      for (int b = 0; b < kRB; ++b) {
+        const int rl = min(rl0 + b, nrow - 1);
+        xv[b] = static_cast<double>(xi[base + static_cast<long>(rl) * A::kNo + jk]);
+      }
+#pragma unroll
+      for (int b = 0; b < kRB; ++b) {
+        const int rl = rl0 + b;
+        if (on && rl < nrow) {
+          const double x = xv[b];
+          const double vx = static_cast<double>(static_cast<TO>(stage[rl * A::kNjk + jk]));
+          const double x2m1 = x * x - 1.0;
+          acc_ff += vx * wth * x2m1;
+          df[base + static_cast<long>(rl) * A::kNo + jk] = static_cast<TO>(2.0 * rho0 * x * vx);
+          cs += x2m1;
+          cv += vx;
+        }
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc_ff += __shfl_xor(acc_ff, off);
+    if (lane == 0) partials[rb * A::kNth + th] = acc_ff;
+    if (on) {
+      // colpart[rb][0][io] = sum_r (xi^2 - 1), colpart[rb][1][io] = sum_r vexc
+      double* cp = colpart + rb * (2 * A::kNo) + th * A::kNjk + jk;
+      cp[0] = cs;
+      cp[A::kNo] = cv;
+    }
+    return;
+  }
   if (xi != nullptr && r0 + kWaveRows <= N) {
     // full block: batches of kB elements per lane with all their xi loads in
     // flight before the arithmetic.  The mixed forms (FP32 arithmetic, FP64
@@ -433,7 +476,9 @@ __global__ __launch_bounds__(kThreads) void delta_f_kernel(const T* __restrict__
 constexpr int kRedThreads = 1024;
 __global__ __launch_bounds__(kRedThreads) void reduce_partials_kernel(const double* __restrict__ partials, int count,
                                                                       double scale, double* __restrict__ out,
-                                                                      long out_stride) {
+                                                                      long out_stride,
+                                                                      const double* __restrict__ extra,
+                                                                      int nextra) {
   partials += static_cast<long>(blockIdx.x) * count;
   out += blockIdx.x * out_stride;
   constexpr int T = kRedThreads;
@@ -447,6 +492,8 @@ __global__ __launch_bounds__(kRedThreads) void reduce_partials_kernel(const doub
     a3 += v3;
   }
   for (; i < count; i += T) a0 += partials[i];
+  if (extra)  // the extra terms of this field, after the partials, in the same fixed order
+    for (int e = threadIdx.x; e < nextra; e += T) a1 += extra[static_cast<long>(blockIdx.x) * nextra + e];
   __shared__ double red[T];
   red[threadIdx.x] = (a0 + a1) + (a2 + a3);
   __syncthreads();
@@ -629,21 +676,26 @@ int proj2angl_blocks(const ProjTables& t, long N) {
   return static_cast<int>((N + kRT - 1) / kRT);
 }
 
+bool proj2angl_has_column_sums(const ProjTables& t) { return dispatch_angular(t.mmax, t.mrso, [](auto, auto) {}); }
+
+long proj2angl_row_blocks(long N) { return wave_row_blocks(N); }
+
 template <typename TP, typename TO, typename TC>
 void proj2angl_factorized(const ProjTables& t, long N, const cplx<TP>* p, TO* o, const TO* xi, TO* df,
-                          double rho0, double* partials, hipStream_t s) {
+                          double rho0, double* partials, hipStream_t s, double* colpart) {
   if (N == 0) return;
   const long nwb = wave_blocks(t, N);
   throw_if(nwb > 0x7fffffffL, "grid too large");
   const bool compiled = dispatch_angular(t.mmax, t.mrso, [&](auto mm, auto mr) {
     hipLaunchKernelGGL((proj2angl_wave_kernel<decltype(mm)::value, decltype(mr)::value, TP, TO, TC>),
                        dim3(static_cast<unsigned>(nwb)), dim3(64), 0, s, p, o, xi, df, t.w, N, wave_row_blocks(N),
-                       tabs_of<TC>(t, false), rho0, partials);
+                       tabs_of<TC>(t, false), rho0, partials, colpart);
   });
   if (compiled) {
     MDFT_HIP_CHECK(hipGetLastError());
     return;
   }
+  throw_if(colpart != nullptr, "proj2angl: column sums need a compile-time angular grid");
   const long nblk = (N + kRT - 1) / kRT;
   throw_if(nblk > 0x7fffffffL, "grid too large");
   const size_t lds = p2a_lds<TC>(t);
@@ -676,9 +728,9 @@ int delta_f(const T* xi, const T* vexc, const double* w, T* df, long nxyz, int n
 }
 
 void reduce_partials(const double* partials, int count, double scale, double* out, hipStream_t s, int nbatch,
-                     long out_stride) {
+                     long out_stride, const double* extra, int nextra) {
   hipLaunchKernelGGL(reduce_partials_kernel, dim3(nbatch), dim3(kRedThreads), 0, s, partials, count, scale, out,
-                     out_stride);
+                     out_stride, extra, nextra);
   MDFT_HIP_CHECK(hipGetLastError());
 }
 
@@ -698,11 +750,14 @@ template void angl2proj_factorized<float, float, float>(const ProjTables&, long,
 template void angl2proj_factorized<double, float, float>(const ProjTables&, long, const double*, cplx<float>*,
                                                          const double*, hipStream_t);
 template void proj2angl_factorized<double, double, double>(const ProjTables&, long, const cplx<double>*, double*,
-                                                           const double*, double*, double, double*, hipStream_t);
+                                                           const double*, double*, double, double*, hipStream_t,
+                                                           double*);
 template void proj2angl_factorized<float, float, float>(const ProjTables&, long, const cplx<float>*, float*,
-                                                        const float*, float*, double, double*, hipStream_t);
+                                                        const float*, float*, double, double*, hipStream_t,
+                                                        double*);
 template void proj2angl_factorized<float, double, float>(const ProjTables&, long, const cplx<float>*, double*,
-                                                         const double*, double*, double, double*, hipStream_t);
+                                                         const double*, double*, double, double*, hipStream_t,
+                                                         double*);
 template void delta_rho<double>(const double*, double*, size_t, double, hipStream_t);
 template void delta_rho<float>(const float*, float*, size_t, double, hipStream_t);
 template int delta_f<double>(const double*, const double*, const double*, double*, long, int, double, double*, int,

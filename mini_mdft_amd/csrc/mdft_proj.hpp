@@ -39,9 +39,16 @@ void angl2proj_factorized(const ProjTables& t, long N, const TI* o, cplx<TP>* p,
 // proj2angl (:388-484).  With xi != nullptr the orientation output is not
 // stored: get_delta_f (common/MDFT_Solvent.hpp:613-660) runs in the epilogue,
 // writing df and one FP64 partial sum per block into `partials`.
+// With colpart != nullptr (and xi, compile-time angular grids only) the
+// epilogue also writes, per block of 64 rows rb, the column sums over its
+// rows: colpart[rb][0][io] = sum_r (xi^2 - 1), colpart[rb][1][io] = sum_r vexc.
 template <typename TP, typename TO, typename TC>
 void proj2angl_factorized(const ProjTables& t, long N, const cplx<TP>* p, TO* o, const TO* xi, TO* df,
-                          double rho0, double* partials, hipStream_t s);
+                          double rho0, double* partials, hipStream_t s, double* colpart = nullptr);
+// whether proj2angl_factorized can write column sums for this grid, and the
+// row blocks (64 rows each) it writes them for
+bool proj2angl_has_column_sums(const ProjTables& t);
+long proj2angl_row_blocks(long N);
 // partial sums written by the fused proj2angl over N points (one per row
 // block, or per (row block, theta)); for N a multiple of 64 the partials of
 // point ranges [j N, (j + 1) N) of a batch are contiguous
@@ -61,8 +68,10 @@ int delta_f_rows_per_block();
 // ff = scale * sum(partials[0..count)), summed in a fixed order, into out[0];
 // with nbatch > 1, field j's count partials follow field j - 1's and its sum
 // goes to out[j out_stride]
+// extra (device, nbatch x nextra values, may be null): field j's nextra
+// values are added to its sum, before the scaling
 void reduce_partials(const double* partials, int count, double scale, double* out, hipStream_t s, int nbatch = 1,
-                     long out_stride = 1);
+                     long out_stride = 1, const double* extra = nullptr, int nextra = 0);
 
 template <typename T>
 void fill_uniform(T* x, size_t n, unsigned long long seed, unsigned long long offset, double lo, double hi,

@@ -334,6 +334,11 @@ class Solver:
 
     STAGES = ("angl2proj", "fft_forward", "oz", "fft_backward", "proj2angl_delta_f")
 
+    def set_mean_fp64(self, on: bool) -> None:
+        """The r-mean (q = 0) mode of ff in FP64 (default on for MDFT_MIXED_X6;
+        available for MDFT_FP64 / MDFT_FP32, where it changes ff by rounding only)."""
+        check(lib().mdft_solver_set_mean_fp64(self._h, 1 if on else 0))
+
     def set_timing(self, enable: bool, oz_only: bool = False) -> None:
         """Per-stage hipEvent timers (the reference Solver's stage timers); with
         oz_only, only the OZ launch is timed, by the start and end events that

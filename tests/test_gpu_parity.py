@@ -207,10 +207,13 @@ def test_energy_fp64_parity(oracle, n, L, mmax):
     assert solver.energy_cproj(xt, df2) == ff and torch.equal(df, df2)
 
 
-@pytest.mark.parametrize("prec,tol", [(FP32, 1e-3), (MIXED, 1e-3), (MIXED_X6, 1e-3)])
-def test_energy_reduced_precision(oracle, prec, tol):
+@pytest.mark.parametrize("prec,tol,tol_df", [(FP32, 1e-3, 1e-3), (MIXED, 1e-3, 1e-3), (MIXED_X6, 1e-5, 1e-5)])
+def test_energy_reduced_precision(oracle, prec, tol, tol_df):
     """FP32 chops gamma at FLT_EPSILON like the reference's float build;
-    MIXED keeps the FP64 threshold."""
+    MIXED keeps the FP64 threshold.  MDFT_MIXED_X6 is held to 1e-5 of |ff|
+    here (measured 3.7e-6 = 2.7e-7 kJ/mol: at 8^3 ff is a sum of terms that
+    cancel to 0.07); the 0.01 kJ/mol bar at the benchmark sizes is checked by
+    tests/test_gpu_configs.py."""
     n, L, mmax = (8, 8, 8), (10.0, 10.0, 10.0), 5
     g, ag, mp, tab, ocv = _problem(oracle, n, L, mmax, seed=11)
     conv = M.Convolution.from_table(tab, g, ag, mp, prec)
@@ -223,8 +226,9 @@ def test_energy_reduced_precision(oracle, prec, tol):
     xt = _to(xi, dt)
     df = torch.empty_like(xt)
     ff = solver.energy_cproj(xt, df)
+    print(f"{prec}: ff {ff!r} oracle {ff_ref!r} rel {abs(ff - ff_ref) / abs(ff_ref):.2e}", flush=True)
     assert abs(ff - ff_ref) <= tol * abs(ff_ref)
-    assert rel_err(_np(df).astype(np.float64), df_ref) < tol
+    assert rel_err(_np(df).astype(np.float64), df_ref) < tol_df
 
 
 def test_energy_equals_stagewise_api(oracle):
@@ -299,3 +303,46 @@ def _prime_factors(n):
             n //= p
         p += 1
     return out
+
+
+# ---------------------------------------------------------------------------
+# The FP64 mean mode (the r-mean / q = 0 mode of ff carried in FP64).  In the
+# FP64 form it recomputes that mode along a second FP64 route (column sums,
+# the dense projection matrices, the OZ product at q = 0 with R(0) and the
+# first c(q) row), so ff must not move beyond rounding: this pins the route
+# (a wrong sign, row or store rule at q = 0 moves ff by ~1, the mode being
+# ~99% of ff).  df is not touched.
+MEAN_CASES = [((8, 8, 8), (10.0, 10.0, 10.0), 3, 1), ((6, 5, 4), (6.0, 6.0, 6.0), 5, 1),
+              ((16, 16, 16), (12.0, 12.0, 12.0), 5, 1), ((1, 4, 6), (5.0, 6.0, 7.0), 3, 1),
+              ((4, 4, 4), (6.0, 6.0, 6.0), 5, 2), ((8, 4, 2), (9.0, 5.0, 3.0), 4, 3)]
+
+
+@pytest.mark.parametrize("n,L,mmax,nb", MEAN_CASES)
+def test_mean_mode_fp64_changes_ff_by_rounding_only(oracle, n, L, mmax, nb):
+    g, ag, mp, tab, ocv = _problem(oracle, n, L, mmax, seed=13)
+    conv = M.Convolution.from_table(tab, g, ag, mp, FP64)
+    n0 = M.SOLVENTS["spce"]["n0"]
+    rho0, pref = M.rho0_of(n0, MRSO), M.prefactor_of(298.15, MRSO, n0)
+    solver = M.Solver(conv, rho0, pref, FP64, capacity=nb)
+    xi = np.concatenate([rng.xi_field(12345 + j, (g.N, ag.m_no)) for j in range(nb)])
+    xt = _to(xi, torch.float64).view(nb, g.N, ag.m_no)
+    df0, df1 = torch.empty_like(xt), torch.empty_like(xt)
+    ff0 = solver.energy_cproj_batch(xt, df0) if nb > 1 else [solver.energy_cproj(xt[0], df0[0])]
+    solver.set_mean_fp64(True)
+    ff1 = solver.energy_cproj_batch(xt, df1) if nb > 1 else [solver.energy_cproj(xt[0], df1[0])]
+    assert torch.equal(df0, df1)
+    for j in range(nb):
+        ff_ref, _ = ocv.energy_cproj(xi[j * g.N:(j + 1) * g.N], rho0, pref)
+        assert abs(ff1[j] - ff0[j]) <= 1e-12 * abs(ff0[j]), (j, ff1[j], ff0[j])
+        assert abs(ff1[j] - ff_ref) <= 1e-10 * abs(ff_ref)
+
+
+def test_mean_mode_availability():
+    g = M.SpatialGrid((4, 4, 4), (6.0, 6.0, 6.0))
+    ag = M.AngularGrid(3, MRSO)
+    tab = luc.synthetic_c(3, MRSO, luc.nq_for(luc.qmax_of_grid((4, 4, 4), (6.0,) * 3), DQ), seed=1)
+    conv32 = M.Convolution.from_table(tab, g, ag, M.OrientationProjectionMap(ag), FP32)
+    with pytest.raises(M.MDFTError, match="mean mode"):
+        M.Solver(conv32, 0.001, -1.0, MIXED).set_mean_fp64(True)
+    M.Solver(conv32, 0.001, -1.0, MIXED).set_mean_fp64(False)
+    M.Solver(conv32, 0.001, -1.0, MIXED_X6).set_mean_fp64(False)
