@@ -251,10 +251,9 @@ __global__ __launch_bounds__(kWave) void oz_dtable_kernel(T* __restrict__ dtab, 
   }
 }
 
-// The OZ pass for angular grids with compile-time (mmax, mrso), FP64 (FP32
-// runs oz_pairs_ct_kernel): one pair per 64-lane block as in oz_pair_kernel,
-// with every LDS access a lane base plus a compile-time offset and every loop
-// unrolled.  Layouts (W = 2 mmax + 1):
+// The per-pair LDS layouts of the OZ pass for angular grids with compile-time
+// (mmax, mrso) (oz_pairs_ct_kernel), where every LDS access is a lane base
+// plus a compile-time offset and every loop is unrolled (W = 2 mmax + 1):
 //   G, Gm:  rows (m, mu2) of W entries, entry mup + m: the phased gathers,
 //           later gamma with entry chi + m (zero padding past 2m+1);
 //   Dq, Dmq: the rotated vectors in projection order;
@@ -306,213 +305,6 @@ __device__ __forceinline__ int tofs_rt(int n) {
   return a;
 }
 
-template <typename T, int MMAX, int MRSO>
-__global__ __launch_bounds__(kWave) void oz_pair_ct_kernel(cplx<T>* __restrict__ p, const T* __restrict__ dtab,
-                                                           OzArgs<T> a) {
-  using S = OzShape<MMAX, MRSO>;
-  constexpr int W = S::W;
-  extern __shared__ unsigned char lds_raw[];
-  cplx<T>* G = reinterpret_cast<cplx<T>*>(lds_raw);
-  cplx<T>* Gm = G + S::NG * W;
-  cplx<T>* Dq = Gm + S::NG * W;
-  cplx<T>* Dmq = Dq + S::NP;
-  cplx<T>* ceff = Dmq + S::NP;
-  T* dP0 = reinterpret_cast<T*>(ceff + a.np_new);  // leading zero row, then ND rows
-  T* dP = dP0 + W;
-  __shared__ cplx<T> ph[W];  // e^{-i mup phi_q}
-  const int lane = threadIdx.x;
-
-  int job;
-  const long pair = oz_block_pair(a, job);
-  if (pair >= a.npairs) return;  // uniform across the block
-  p += static_cast<long>(job) * a.N;
-  const int pk = a.pairs[pair];
-  const int ix = pk & 1023, iy = (pk >> 10) & 1023, iz = (pk >> 20) & 1023;
-  const int ixm = ix == 0 ? 0 : a.nx - ix, iym = iy == 0 ? 0 : a.ny - iy, izm = iz == 0 ? 0 : a.nz - iz;
-  const long sq = (static_cast<long>(ix) * a.ny + iy) * a.nz + iz;
-  const long smq = (static_cast<long>(ixm) * a.ny + iym) * a.nz + izm;
-  const double q0 = a.kxyz[ix], q1 = a.kxyz[a.nx + iy], q2 = a.kxyz[a.nx + a.ny + iz];
-
-  // gathers (:420-425) and lane metadata first (plain scalars: arrays of
-  // HIP's vector classes are kept in scratch memory)
-  T g0x[S::NSLOT], g0y[S::NSLOT], g1x[S::NSLOT], g1y[S::NSLOT];
-  int info[S::NSLOT], ias[S::NSLOT];
-#pragma unroll
-  for (int t = 0; t < S::NSLOT; ++t) {
-    const int ip = min(lane + t * kWave, S::NP - 1);
-    const cplx<T> u = p[static_cast<long>(ip) * a.pstride + sq], v = p[static_cast<long>(ip) * a.pstride + smq];
-    g0x[t] = u.x;
-    g0y[t] = u.y;
-    g1x[t] = v.x;
-    g1y[t] = v.y;
-    info[t] = a.ipinfo[ip];
-    ias[t] = a.ia_start[ip];
-  }
-  // padded d rows of this pair
-  {
-    constexpr int ND = S::ND * W, NU = (ND + kWave - 1) / kWave;
-    const T* src = dtab + pair * ND;
-    T dreg[NU];
-#pragma unroll
-    for (int u = 0; u < NU; ++u) dreg[u] = src[min(lane + u * kWave, ND - 1)];
-#pragma unroll
-    for (int u = 0; u < NU; ++u)
-      if (lane + u * kWave < ND) dP[lane + u * kWave] = dreg[u];
-    if (lane < W) dP0[lane] = T(0);
-  }
-  // zero G and Gm (the padding past 2m+1 must stay zero; the valid entries
-  // are written after the barrier)
-  for (int i = lane; i < 2 * S::NG * W; i += kWave) G[i] = mkc<T>(0, 0);
-  if (lane < W) {
-    const double2 e = phase_pow(q0, q1, lane - MMAX);
-    ph[lane] = mkc<T>(static_cast<T>(e.x), static_cast<T>(-e.y));
-  }
-  // c(|q|) rows iq, iq+1 lerped into LDS, kCBatch rows per lane in flight
-  {
-    const double qn = sqrt(q0 * q0 + q1 * q1 + q2 * q2);
-    const double eff = qn / a.dq;
-    int iq = static_cast<int>(eff);
-    const double alpha = eff - static_cast<double>(iq);
-    if (iq > a.nq - 2) iq = a.nq - 2;  // unreachable for |q| <= qmax (nq has 1.5 rows of margin)
-    const T al = static_cast<T>(alpha), bl = static_cast<T>(1.0 - alpha);
-    const cplx<T>* c0 = a.ctab + static_cast<long>(iq) * a.np_new;
-    const cplx<T>* c1 = c0 + a.np_new;
-    for (int ib = lane; ib < a.np_new; ib += kCBatch * kWave) {
-      T lx[kCBatch], ly[kCBatch], hx[kCBatch], hy[kCBatch];
-#pragma unroll
-      for (int u = 0; u < kCBatch; ++u) {
-        const int ia = min(ib + u * kWave, a.np_new - 1);
-        const cplx<T> lo = c0[ia], hi = c1[ia];
-        lx[u] = lo.x;
-        ly[u] = lo.y;
-        hx[u] = hi.x;
-        hy[u] = hi.y;
-      }
-#pragma unroll
-      for (int u = 0; u < kCBatch; ++u) {
-        const int ia = ib + u * kWave;
-        if (ia < a.np_new) ceff[ia] = mkc<T>(al * hx[u] + bl * lx[u], al * hy[u] + bl * ly[u]);
-      }
-    }
-  }
-  __syncthreads();
-
-  // phased gathers g' = g e^{-i mup phi} into G[(m, mu2)][mup + m]
-#pragma unroll
-  for (int t = 0; t < S::NSLOT; ++t) {
-    const int ip = lane + t * kWave;
-    if (ip < S::NP) {
-      const int m = info[t] & 0xff, mupm = (info[t] >> 8) & 0xff, mu2 = info[t] >> 16;
-      const cplx<T> e = ph[mupm];
-      const int at = (grow_rt<MMAX, MRSO>(m) + mu2) * W + mupm - MMAX + m;
-      G[at] = cmul<T>(mkc<T>(g0x[t], g0y[t]), e);
-      Gm[at] = cmul<T>(mkc<T>(g1x[t], g1y[t]), e);
-    }
-  }
-  __syncthreads();
-
-  // rotation to the q frame (:427-443): Dq(m, chi, mu2) = sum_mup g'_q d_{mup,chi},
-  // Dmq(m, chi, mu2) = (-1)^m sum_mup g'_-q d_{mup,-chi}
-#pragma unroll
-  for (int t = 0; t < S::NSLOT; ++t) {
-    const int ip = lane + t * kWave;
-    if (ip < S::NP) {
-      const int m = info[t] & 0xff, chi = ((info[t] >> 8) & 0xff) - MMAX, mu2 = info[t] >> 16;
-      const int row = grow_rt<MMAX, MRSO>(m) + mu2;
-      const cplx<T>* g0 = G + row * W;
-      const cplx<T>* g1 = Gm + row * W;
-      const T* d0 = dP + m * m * W + (chi + m);
-      const T* d1 = dP + m * m * W + (m - chi);
-      T ar = 0, ai = 0, br = 0, bi = 0;
-#pragma unroll
-      for (int j = 0; j < W; ++j) {
-        const cplx<T> x = g0[j], y = g1[j];
-        const T u = d0[j * W], v = d1[j * W];
-        ar = fma(x.x, u, ar);
-        ai = fma(x.y, u, ai);
-        br = fma(y.x, v, br);
-        bi = fma(y.y, v, bi);
-      }
-      Dq[ip] = mkc<T>(ar, ai);
-      Dmq[ip] = (m & 1) ? mkc<T>(-br, -bi) : mkc<T>(br, bi);
-    }
-  }
-  __syncthreads();
-
-  // OZ product (:468-496) and chop (:514-517): output (m, chi, mu2), terms
-  // (n, nu2) for n >= |chi| in the order of the table rows; gamma goes to
-  // G[(m, mu2)][chi + m]
-#pragma unroll
-  for (int t = 0; t < S::NSLOT; ++t) {
-    const int ip = lane + t * kWave;
-    if (ip < S::NP) {
-      const int m = info[t] & 0xff, chi = ((info[t] >> 8) & 0xff) - MMAX, mu2 = info[t] >> 16;
-      const int achi = chi < 0 ? -chi : chi;
-      const cplx<T>* ce = ceff + (ias[t] - tofs_rt<MMAX, MRSO>(achi));
-      cplx<T> s0 = mkc<T>(0, 0), s1 = mkc<T>(0, 0);
-#pragma unroll
-      for (int n = 0; n <= MMAX; ++n) {
-        if (n >= achi) {
-          const int h = n / MRSO, kn = n / MRSO + 1;
-          const cplx<T>* cn = ce + S::tofs(n);
-          const int srcb = S::mbase(n) + (chi + n) * kn;
-#pragma unroll
-          for (int k = 0; k <= 2 * h; ++k) {
-            const int nu2 = k - h;
-            const cplx<T> c = cn[k];
-            if (nu2 < 0) {
-              const int src = srcb - nu2;
-              s0 = cfma<T>(s0, c, Dq[src]);
-              s1 = cfma<T>(s1, c, Dmq[src]);
-            } else {
-              const int src = srcb + nu2;
-              s0 = cfma_conjb<T>(s0, c, Dmq[src]);
-              s1 = cfma_conjb<T>(s1, c, Dq[src]);
-            }
-          }
-        }
-      }
-      const int at = (grow_rt<MMAX, MRSO>(m) + mu2) * W + chi + m;
-      G[at] = chop<T>(s0, a.chop_eps);
-      Gm[at] = chop<T>(s1, a.chop_eps);
-    }
-  }
-  __syncthreads();
-
-  // inverse rotation and stores (:518-541): out(m, mup, mu2) =
-  // e^{+i mup phi} sum_chi gamma d_{mup,chi}
-  const bool self = (ixm == ix && iym == iy && izm == iz);
-  const bool singular = self && (ix == a.nx / 2 || iy == a.ny / 2 || iz == a.nz / 2);
-#pragma unroll
-  for (int t = 0; t < S::NSLOT; ++t) {
-    const int ip = lane + t * kWave;
-    if (ip < S::NP) {
-      const int m = info[t] & 0xff, mupm = (info[t] >> 8) & 0xff, mu2 = info[t] >> 16;
-      const int mup = mupm - MMAX;
-      const int row = grow_rt<MMAX, MRSO>(m) + mu2;
-      const cplx<T>* g0 = G + row * W;
-      const cplx<T>* g1 = Gm + row * W;
-      const T* dr = dP + (m * m + mup + m) * W;
-      const T* drm = dr + 2 * m;  // d_{mup,-chi} = drm[-j]; reads past the row start meet gamma's zeros
-      T ar = 0, ai = 0, br = 0, bi = 0;
-#pragma unroll
-      for (int j = 0; j < W; ++j) {
-        const cplx<T> x = g0[j], y = g1[j];
-        const T u = dr[j], v = drm[-j];
-        ar = fma(x.x, u, ar);
-        ai = fma(x.y, u, ai);
-        br = fma(y.x, v, br);
-        bi = fma(y.y, v, bi);
-      }
-      const cplx<T> e = ph[mupm];  // e^{-i mup phi}; the output takes its conjugate
-      const cplx<T> o0 = cmul<T>(mkc<T>(ar, ai), cconj<T>(e));
-      cplx<T> o1 = cmul<T>(mkc<T>(br, bi), cconj<T>(e));
-      if (m & 1) o1 = mkc<T>(-o1.x, -o1.y);
-      p[static_cast<long>(ip) * a.pstride + sq] = o0;
-      p[static_cast<long>(ip) * a.pstride + smq] = singular ? cconj<T>(o1) : o1;
-    }
-  }
-}
 
 // The compile-time OZ kernel with PB pairs per block and one thread per
 // Per-pair set-up values of the FP32 compile-time OZ kernel, computed once
@@ -564,10 +356,11 @@ constexpr int oz_terms() {
   return t;
 }
 
-// pairs per OZ block: two when their outputs fit 256 lanes
-template <int NP>
+// pairs per OZ block: two when their outputs fit 256 lanes (FP32); one in
+// FP64, whose pair holds twice the LDS
+template <typename T, int NP>
 constexpr int oz_pairs_per_block() {
-  return 2 * NP <= 256 ? 2 : 1;
+  return std::is_same<T, float>::value && 2 * NP <= 256 ? 2 : 1;
 }
 
 // Where the half d table's entries go in LDS: entry e = ip + k NP of a
@@ -630,14 +423,14 @@ __device__ __forceinline__ V ld_off(const V* base, unsigned idx) {
 // gathers of every projection, the pair's half d table (rows mup >= 0), the
 // two c(q) rows (np_new entries each, a compile-time count) and the phases.
 // The mirrored d rows (d^l_{-mup,-chi} = (-1)^{mup-chi} d^l_{mup,chi}), the
-// phases and the lerped c row then go to LDS, and the arithmetic of every
-// output is that of oz_pair_ct_kernel.
+// phases and the lerped c row then go to LDS.  FP32 grids up to 128
+// projections take two pairs per block; FP64 takes one (its pair's LDS region
+// is twice as large, and two pairs per block measured 6% slower at 64^3).
 template <typename T, int MMAX, int MRSO, int PB>
 __global__ __launch_bounds__(256) void oz_pairs_ct_kernel(cplx<T>* __restrict__ p, const T* __restrict__ dtab,
                                                           const OzPairMeta<T>* __restrict__ meta,
                                                           const cplx<T>* __restrict__ phase,
                                                           const int* __restrict__ dmap, OzArgs<T> a, int region) {
-  static_assert(std::is_same<T, float>::value, "the two-pair OZ kernel is the FP32 one");
   using S = OzShape<MMAX, MRSO>;
   constexpr int W = S::W, NP = S::NP, NC = oz_terms<MMAX, MRSO>();
   constexpr int NH = (MMAX + 1) * (MMAX + 2) / 2, NE = NH * W;
@@ -694,7 +487,7 @@ __global__ __launch_bounds__(256) void oz_pairs_ct_kernel(cplx<T>* __restrict__ 
     // the two c rows: with an even term count, two adjacent terms per lane
     // and load (16-byte accesses: the rows start at multiples of 16 bytes, and
     // so does ceff in LDS, after 2 (NG W + NP) complex values)
-    constexpr bool C2 = NC % 2 == 0;
+    constexpr bool C2 = std::is_same<T, float>::value && NC % 2 == 0;
     constexpr int NC2 = NC / 2, NCU2 = (NC2 + NP - 1) / NP;
     cplx<T> lo[C2 ? 1 : NCU], hi[C2 ? 1 : NCU];
     float4 lo4[C2 ? NCU2 : 1], hi4[C2 ? NCU2 : 1];
@@ -1178,7 +971,8 @@ ConvDevice::ConvDevice(int nx, int ny, int nz, double lx, double ly, double lz, 
 ConvDevice::~ConvDevice() {
   for (void* q : {static_cast<void*>(kxyz), static_cast<void*>(pairs), static_cast<void*>(ia_start),
                   static_cast<void*>(ipinfo), static_cast<void*>(ctab64), static_cast<void*>(ctab32),
-                  static_cast<void*>(rcoef), dtab64, dtab32, pmeta32, pphase32, static_cast<void*>(dmap)})
+                  static_cast<void*>(rcoef), dtab64, dtab32, pmeta32, pphase32, pmeta64, pphase64,
+                  static_cast<void*>(dmap)})
     if (q) (void)hipFree(q);
 }
 
@@ -1213,8 +1007,8 @@ const T* ConvDevice::dtable(hipStream_t s) const {
   void*& slot = std::is_same<T, double>::value ? dtab64 : dtab32;
   if (!slot) {
     const size_t nR = static_cast<size_t>(tab.mmax + 1) * (4 * (tab.mmax + 1) * (tab.mmax + 1) - 1) / 3;
-    // the FP32 compile-time kernel reads half tables (rows mup >= 0)
-    const int half = padw && std::is_same<T, float>::value ? 1 : 0;
+    // the compile-time kernel reads half tables (rows mup >= 0)
+    const int half = padw ? 1 : 0;
     const size_t rows = half ? static_cast<size_t>(tab.mmax + 1) * (tab.mmax + 2) / 2
                              : static_cast<size_t>(tab.mmax + 1) * (tab.mmax + 1);
     const size_t row = padw ? rows * padw : nR;
@@ -1227,19 +1021,23 @@ const T* ConvDevice::dtable(hipStream_t s) const {
   return static_cast<const T*>(slot);
 }
 
+template <typename T>
 const void* ConvDevice::pair_meta(hipStream_t s, const void** phases) const {
-  if (!pmeta32) {
-    MDFT_HIP_CHECK(hipMalloc(&pmeta32, std::max<size_t>(1, npairs * sizeof(OzPairMeta<float>))));
-    MDFT_HIP_CHECK(hipMalloc(&pphase32, std::max<size_t>(1, npairs * (tab.mmax + 1) * sizeof(float2))));
-    dmap = upload(oz_thread_records(tab.ang, tab.ia_start));
-    OzArgs<float> a = oz_args<float>(*this, 0.0);
+  constexpr bool f64 = std::is_same<T, double>::value;
+  void*& meta = f64 ? pmeta64 : pmeta32;
+  void*& phase = f64 ? pphase64 : pphase32;
+  if (!meta) {
+    MDFT_HIP_CHECK(hipMalloc(&meta, std::max<size_t>(1, npairs * sizeof(OzPairMeta<T>))));
+    MDFT_HIP_CHECK(hipMalloc(&phase, std::max<size_t>(1, npairs * (tab.mmax + 1) * sizeof(cplx<T>))));
+    if (!dmap) dmap = upload(oz_thread_records(tab.ang, tab.ia_start));
+    OzArgs<T> a = oz_args<T>(*this, 0.0);
     const unsigned nb = static_cast<unsigned>((npairs + 255) / 256);
-    hipLaunchKernelGGL((oz_meta_kernel<float>), dim3(nb), dim3(256), 0, s, static_cast<OzPairMeta<float>*>(pmeta32),
-                       static_cast<cplx<float>*>(pphase32), a);
+    hipLaunchKernelGGL((oz_meta_kernel<T>), dim3(nb), dim3(256), 0, s, static_cast<OzPairMeta<T>*>(meta),
+                       static_cast<cplx<T>*>(phase), a);
     MDFT_HIP_CHECK(hipGetLastError());
   }
-  *phases = pphase32;
-  return pmeta32;
+  *phases = phase;
+  return meta;
 }
 
 // launches kernel on s; with events, through hipExtLaunchKernel, so that the
@@ -1272,47 +1070,30 @@ void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch,
   a.pstride = a.N * nbatch;
   const long nblk = a.chunk * kXcds * nbatch;
   throw_if(nblk > 0x7fffffffL, "too many q pairs");
-  if constexpr (std::is_same<T, double>::value) {
-    if (padw) {
-      // FP64: one pair per 64-lane block (the two-pair block holds 61 KB of
-      // LDS in FP64 and measured slower)
-      dispatch_oz(tab.mmax, tab.mrso, [&](auto mm, auto mr) {
-        constexpr int MMAX = decltype(mm)::value, MRSO = decltype(mr)::value;
-        using Sh = OzShape<MMAX, MRSO>;
-        const size_t lds = (2 * static_cast<size_t>(Sh::NG) * Sh::W + 2 * Sh::NP + tab.np_new) * sizeof(cplx<T>) +
-                           static_cast<size_t>(Sh::W) * (Sh::ND + 1) * sizeof(T);
-        throw_if(lds > 64 * 1024, "angular grid too large for the OZ kernel");
-        launch_oz(oz_pair_ct_kernel<T, MMAX, MRSO>, static_cast<unsigned>(nblk), kWave, lds, s, ev0, ev1, p, dt, a);
-      });
-      return;
-    }
-  }
-  if constexpr (std::is_same<T, float>::value) {
-    if (padw) {
-      const cplx<T>* ph = nullptr;
-      const OzPairMeta<T>* mt = static_cast<const OzPairMeta<T>*>(pair_meta(s, reinterpret_cast<const void**>(&ph)));
-      dispatch_oz(tab.mmax, tab.mrso, [&](auto mm, auto mr) {
-        constexpr int MMAX = decltype(mm)::value, MRSO = decltype(mr)::value;
-        using Sh = OzShape<MMAX, MRSO>;
-        constexpr int PB = oz_pairs_per_block<Sh::NP>();
-        constexpr int NC = oz_terms<MMAX, MRSO>();
-        throw_if(tab.np_new != NC, "internal error: OZ term count");
-        throw_if(static_cast<double>(tab.nq) * NC * sizeof(cplx<T>) >= 4294967296.0, "c(q) table too large");
-        const size_t pair_lds = (2 * static_cast<size_t>(Sh::NG) * Sh::W + 2 * Sh::NP + NC) * sizeof(cplx<T>) +
-                                static_cast<size_t>(Sh::W) * (Sh::ND + 1) * sizeof(T);
-        const size_t region = (pair_lds + 15) / 16 * 16;
-        throw_if(PB * region > 64 * 1024, "angular grid too large for the OZ kernel");
-        OzArgs<T> ag = a;
-        const long ngroups = (npairs + PB - 1) / PB;
-        ag.chunk = (ngroups + kXcds - 1) / kXcds;
-        const long nb = ag.chunk * kXcds * nbatch;
-        throw_if(nb > 0x7fffffffL, "too many q pairs");
-        const int threads = (PB * Sh::NP + kWave - 1) / kWave * kWave;
-        launch_oz(oz_pairs_ct_kernel<T, MMAX, MRSO, PB>, static_cast<unsigned>(nb), static_cast<unsigned>(threads),
-                  PB * region, s, ev0, ev1, p, dt, mt, ph, dmap, ag, static_cast<int>(region));
-      });
-      return;
-    }
+  if (padw) {
+    const cplx<T>* ph = nullptr;
+    const OzPairMeta<T>* mt = static_cast<const OzPairMeta<T>*>(pair_meta<T>(s, reinterpret_cast<const void**>(&ph)));
+    dispatch_oz(tab.mmax, tab.mrso, [&](auto mm, auto mr) {
+      constexpr int MMAX = decltype(mm)::value, MRSO = decltype(mr)::value;
+      using Sh = OzShape<MMAX, MRSO>;
+      constexpr int PB = oz_pairs_per_block<T, Sh::NP>();
+      constexpr int NC = oz_terms<MMAX, MRSO>();
+      throw_if(tab.np_new != NC, "internal error: OZ term count");
+      throw_if(static_cast<double>(tab.nq) * NC * sizeof(cplx<T>) >= 4294967296.0, "c(q) table too large");
+      const size_t pair_lds = (2 * static_cast<size_t>(Sh::NG) * Sh::W + 2 * Sh::NP + NC) * sizeof(cplx<T>) +
+                              static_cast<size_t>(Sh::W) * (Sh::ND + 1) * sizeof(T);
+      const size_t region = (pair_lds + 15) / 16 * 16;
+      throw_if(PB * region > 64 * 1024, "angular grid too large for the OZ kernel");
+      OzArgs<T> ag = a;
+      const long ngroups = (npairs + PB - 1) / PB;
+      ag.chunk = (ngroups + kXcds - 1) / kXcds;
+      const long nb = ag.chunk * kXcds * nbatch;
+      throw_if(nb > 0x7fffffffL, "too many q pairs");
+      const int threads = (PB * Sh::NP + kWave - 1) / kWave * kWave;
+      launch_oz(oz_pairs_ct_kernel<T, MMAX, MRSO, PB>, static_cast<unsigned>(nb), static_cast<unsigned>(threads),
+                PB * region, s, ev0, ev1, p, dt, mt, ph, dmap, ag, static_cast<int>(region));
+    });
+    return;
   }
   const size_t lds = oz_lds_bytes(sizeof(T));
   throw_if(lds > 64 * 1024, "angular grid too large for the OZ kernel");

@@ -33,9 +33,12 @@ struct ConvDevice {
   // per-pair set-up values of the FP32 compile-time OZ kernel (packed index,
   // c(q) row, interpolation weights; and in *phases the phases of mup =
   // 0..mmax), built on first use on stream s
+  template <typename T>
   const void* pair_meta(hipStream_t s, const void** phases) const;
   mutable void* pmeta32 = nullptr;
   mutable void* pphase32 = nullptr;
+  mutable void* pmeta64 = nullptr;
+  mutable void* pphase64 = nullptr;
   mutable int* dmap = nullptr;  // per-thread records: LDS places of the half d table's entries, ipinfo
   // row width of the padded d layout when the compile-time OZ kernel serves
   // this (mmax, mrso) (2 mmax + 1), else 0 (compact rows, generic kernel)

@@ -140,6 +140,128 @@ __global__ __launch_bounds__(kThreads) void angl2proj_kernel(const TI* __restric
   }
 }
 
+// angl2proj_kernel for an angular grid fixed at compile time (the grids of
+// dispatch_angular): the same three steps with every extent a constant, so
+// that the index arithmetic folds and the short loops unroll.
+template <int MMAX, int MRSO, int RT>
+struct A2pSizes {
+  static constexpr int kNth = MMAX + 1, kNphi = 2 * MMAX + 1, kNmu2 = MMAX / MRSO + 1;
+  static constexpr int kNpsi = 2 * (MMAX / MRSO) + 1, kNo = kNth * kNphi * kNpsi;
+  static constexpr int kNp() {
+    int n = 0;
+    for (int m = 0; m <= MMAX; ++m) n += (2 * m + 1) * (m / MRSO + 1);
+    return n;
+  }
+  static constexpr int kCells = RT * kNth * kNphi * kNmu2;
+  static constexpr int kRegA = (2 * kCells > RT * kNo) ? 2 * kCells : RT * kNo;
+};
+
+// rows per block of the compile-time angl2proj: 4 in FP64, whose tile holds
+// twice the bytes (more blocks per CU), 8 in FP32
+template <typename TC>
+constexpr int a2p_ct_rows() {
+  return sizeof(TC) == 8 ? 4 : 8;
+}
+
+template <int MMAX, int MRSO, typename TI, typename TP, typename TC>
+__global__ __launch_bounds__(kThreads) void angl2proj_ct_kernel(const TI* __restrict__ o, cplx<TP>* __restrict__ p,
+                                                                long N, TabPtrs<TC> tb, int fuse, double rho0) {
+  constexpr int kRT = a2p_ct_rows<TC>();
+  using Z = A2pSizes<MMAX, MRSO, kRT>;
+  constexpr int NTH = Z::kNth, NPHI = Z::kNphi, NMU2 = Z::kNmu2, NPSI = Z::kNpsi, NO = Z::kNo, NP = Z::kNp();
+  constexpr int CELLS = Z::kCells;
+  __shared__ TC A[Z::kRegA];  // o tile, then Y
+  __shared__ cplx<TC> X[CELLS];
+  __shared__ cplx<TC> ephi[NPHI];
+  __shared__ cplx<TC> epsi[NMU2 * NPSI];
+  __shared__ TC wts[NTH * NP];
+  cplx<TC>* Y = reinterpret_cast<cplx<TC>*>(A);
+
+  const int tid = threadIdx.x;
+  const long r0 = static_cast<long>(blockIdx.x) * kRT;
+  const int rv = static_cast<int>(min(static_cast<long>(kRT), N - r0));
+
+  if (tid < NPHI) ephi[tid] = tb.ephi[tid];
+  if (tid < NMU2 * NPSI) epsi[tid] = tb.epsi[tid];
+  for (int i = tid; i < NTH * NP; i += kThreads) wts[i] = tb.wts[i];
+  const TI* src = o + r0 * NO;
+  if (rv == kRT) {  // a full tile: every load of the thread in flight at once
+    constexpr int kPer = (kRT * NO + kThreads - 1) / kThreads;
+    double v[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int i = tid + u * kThreads;
+      v[u] = static_cast<double>(src[i < kRT * NO ? i : 0]);
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int i = tid + u * kThreads;
+      if (i < kRT * NO) A[i] = static_cast<TC>(fuse ? rho0 * (v[u] * v[u] - 1.0) : v[u]);
+    }
+  } else {
+    for (int i = tid; i < rv * NO; i += kThreads) {
+      const double x = static_cast<double>(src[i]);
+      A[i] = static_cast<TC>(fuse ? rho0 * (x * x - 1.0) : x);
+    }
+  }
+  __syncthreads();
+
+  // X[r][theta][j][mu2] = sum_k o[r][theta][j][k] e^{+2 pi i mu2 k/npsi}
+  const int nx_items = rv * NTH * NPHI * NMU2;
+  for (int it = tid; it < nx_items; it += kThreads) {
+    const int mu2 = it % NMU2;
+    const int rtj = it / NMU2;  // (r*ntheta + theta)*nphi + j
+    const TC* row = A + (rtj / (NTH * NPHI)) * NO + (rtj % (NTH * NPHI)) * NPSI;
+    const cplx<TC>* e = epsi + mu2 * NPSI;
+    TC re = 0, im = 0;
+#pragma unroll
+    for (int k = 0; k < NPSI; ++k) {
+      re = fma(row[k], e[k].x, re);
+      im = fma(row[k], e[k].y, im);
+    }
+    X[it] = mkc<TC>(re, im);
+  }
+  __syncthreads();
+
+  // Y[r][theta][mup][mu2] = sum_j X[r][theta][j][mu2] e^{+2 pi i mup j/nphi}
+  for (int it = tid; it < nx_items; it += kThreads) {
+    const int mu2 = it % NMU2;
+    const int rtm = it / NMU2;
+    const int imup = rtm % NPHI;
+    const int rt = rtm / NPHI;
+    const int mup = imup - MMAX;
+    const cplx<TC>* xs = X + rt * NPHI * NMU2 + mu2;
+    cplx<TC> acc = mkc<TC>(0, 0);
+    int t = 0;  // mup j mod nphi
+    const int step = mup < 0 ? mup + NPHI : mup;
+#pragma unroll
+    for (int j = 0; j < NPHI; ++j) {
+      acc = cfma<TC>(acc, xs[j * NMU2], ephi[t]);
+      t += step;
+      if (t >= NPHI) t -= NPHI;
+    }
+    Y[it] = acc;
+  }
+  __syncthreads();
+
+  // p[ip][r] = sum_theta W[theta][ip] Y[r][theta][mup(ip)][mu2(ip)]
+  for (int it = tid; it < NP * rv; it += kThreads) {
+    const int ip = it / rv, r = it - ip * rv;
+    const int info = tb.ipinfo[ip];
+    const int imup = info & 0xff, mu2 = (info >> 8) & 0xff;
+    const cplx<TC>* ys = Y + (r * NTH * NPHI + imup) * NMU2 + mu2;
+    TC re = 0, im = 0;
+#pragma unroll
+    for (int th = 0; th < NTH; ++th) {
+      const TC wv = wts[th * NP + ip];
+      const cplx<TC> y = ys[th * NPHI * NMU2];
+      re = fma(wv, y.x, re);
+      im = fma(wv, y.y, im);
+    }
+    p[static_cast<long>(ip) * N + r0 + r] = mkc<TP>(static_cast<TP>(re), static_cast<TP>(im));
+  }
+}
+
 template <typename TP, typename TO, typename TC>
 __global__ __launch_bounds__(kThreads) void proj2angl_kernel(const cplx<TP>* __restrict__ p, TO* __restrict__ o,
                                                              const TO* __restrict__ xi, TO* __restrict__ df,
@@ -342,8 +464,10 @@ __global__ __launch_bounds__(64) void proj2angl_wave_kernel(const cplx<TP>* __re
   if (colpart != nullptr && xi != nullptr) {
     // get_delta_f with the column sums: lane jk owns orientation th * nphi
     // npsi + jk and walks the block's rows, so that its sums over r of
-    // xi^2 - 1 and of vexc build up in registers (each row's 55 values are
-    // one contiguous run of xi and df)
+    // xi^2 - 1 and of vexc build up in registers (each row's nphi npsi values
+    // are one contiguous run of xi and df; for the mixed forms this walk also
+    // measured faster than the row-major one below, 0.358 against 0.371 ms
+    // at 64^3, and in FP64 the same, 1.11 against 1.10 ms)
     constexpr int kRB = 32;  // rows per batch of loads in flight
     const int nrow = static_cast<int>(min(static_cast<long>(kWaveRows), N - r0));
     const bool on = lane < A::kNjk;
@@ -629,18 +753,6 @@ ProjTables::~ProjTables() {
     if (q) (void)hipFree(q);
 }
 
-template <typename TI, typename TP, typename TC>
-void angl2proj_factorized(const ProjTables& t, long N, const TI* o, cplx<TP>* p, const double* drho_rho0,
-                          hipStream_t s) {
-  if (N == 0) return;
-  const long nblk = (N + kRT - 1) / kRT;
-  throw_if(nblk > 0x7fffffffL, "grid too large");
-  const size_t lds = a2p_lds<TC>(t);
-  throw_if(lds > 160 * 1024, "angular grid too large for the projection tile");
-  hipLaunchKernelGGL((angl2proj_kernel<TI, TP, TC>), dim3(static_cast<unsigned>(nblk)), dim3(kThreads), lds, s, o, p,
-                     N, dims_of(t), tabs_of<TC>(t, true), drho_rho0 ? 1 : 0, drho_rho0 ? *drho_rho0 : 0.0);
-  MDFT_HIP_CHECK(hipGetLastError());
-}
 
 namespace {
 
@@ -670,6 +782,37 @@ long wave_blocks(const ProjTables& t, long N) {
 }
 
 }  // namespace
+
+template <typename TI, typename TP, typename TC>
+void angl2proj_factorized(const ProjTables& t, long N, const TI* o, cplx<TP>* p, const double* drho_rho0,
+                          hipStream_t s) {
+  if (N == 0) return;
+  const long nblk = (N + kRT - 1) / kRT;
+  throw_if(nblk > 0x7fffffffL, "grid too large");
+  bool compiled = false;
+  dispatch_angular(t.mmax, t.mrso, [&](auto mm, auto mr) {
+    constexpr int MMAX = decltype(mm)::value, MRSO = decltype(mr)::value, RT = a2p_ct_rows<TC>();
+    using Z = A2pSizes<MMAX, MRSO, RT>;
+    const long nblk_ct = (N + RT - 1) / RT;
+    constexpr size_t kLds = (Z::kRegA + Z::kNth * Z::kNp()) * sizeof(TC) +
+                            (Z::kCells + Z::kNphi + Z::kNmu2 * Z::kNpsi) * sizeof(cplx<TC>);
+    if constexpr (kLds <= 64 * 1024) {  // else the run-time kernel below
+      hipLaunchKernelGGL((angl2proj_ct_kernel<MMAX, MRSO, TI, TP, TC>), dim3(static_cast<unsigned>(nblk_ct)),
+                         dim3(kThreads), 0, s, o, p, N, tabs_of<TC>(t, true), drho_rho0 ? 1 : 0,
+                         drho_rho0 ? *drho_rho0 : 0.0);
+      compiled = true;
+    }
+  });
+  if (compiled) {
+    MDFT_HIP_CHECK(hipGetLastError());
+    return;
+  }
+  const size_t lds = a2p_lds<TC>(t);
+  throw_if(lds > 160 * 1024, "angular grid too large for the projection tile");
+  hipLaunchKernelGGL((angl2proj_kernel<TI, TP, TC>), dim3(static_cast<unsigned>(nblk)), dim3(kThreads), lds, s, o, p,
+                     N, dims_of(t), tabs_of<TC>(t, true), drho_rho0 ? 1 : 0, drho_rho0 ? *drho_rho0 : 0.0);
+  MDFT_HIP_CHECK(hipGetLastError());
+}
 
 int proj2angl_blocks(const ProjTables& t, long N) {
   if (dispatch_angular(t.mmax, t.mrso, [](auto, auto) {})) return static_cast<int>(wave_row_blocks(N) * t.ntheta);
