@@ -23,7 +23,7 @@ class Convolution {
               const OrientationProjectionMapType& map, const int np_luc, Precision precision = Precision::Default,
               ExecutionSpace exec = ExecutionSpace())
       : m_exec(exec), m_precision(precision_code<ScalarType>(precision)) {
-    (void)map;
+    Impl::check_map(map, angular_grid);
     Impl::check(mdft_conv_create(filename.c_str(), spatial_grid.m_nx, spatial_grid.m_ny, spatial_grid.m_nz,
                                  spatial_grid.m_lx, spatial_grid.m_ly, spatial_grid.m_lz, angular_grid.m_mmax,
                                  angular_grid.m_molrotsymorder, np_luc, m_precision, &m_h));
@@ -36,7 +36,7 @@ class Convolution {
               const OrientationProjectionMapType& map, Precision precision = Precision::Default,
               ExecutionSpace exec = ExecutionSpace())
       : m_exec(exec), m_precision(precision_code<ScalarType>(precision)) {
-    (void)map;
+    Impl::check_map(map, angular_grid);
     Impl::Throw_If(tuples.size() % 5 != 0 || c.size() != tuples.size() / 5 * static_cast<size_t>(nq_table),
                    "c(q) table shape does not match the tuples");
     Impl::check(mdft_conv_create_from_table(

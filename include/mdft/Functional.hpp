@@ -13,14 +13,6 @@
 
 namespace MDFT {
 
-// One site: charge (e), LJ sigma (A), epsilon (kJ/mol), position (A); solvent
-// sites are in the molecular frame (Solute::read_solute,
-// common/MDFT_Solute.hpp:126-191; Solvents::set_solvent,
-// common/MDFT_Solvent.hpp:417-532).
-struct Site {
-  double q, sigma, epsilon, x, y, z;
-};
-
 // AngularGrid's rotation matrices, rot[io] row major (common/MDFT_Grid.hpp:295-318)
 template <typename ExecutionSpace, typename ScalarType>
 std::vector<std::array<double, 9>> rotation_matrices(const AngularGrid<ExecutionSpace, ScalarType>& ag) {

@@ -1,7 +1,8 @@
 // SpatialGrid and AngularGrid (common/MDFT_Grid.hpp:23-104 and :106-334),
 // same names, constructor arguments, members and errors.  The tables are built
-// by the library in FP64 on the host (mdft_kproj, mdft_angular_tables) and
-// kept here as host vectors; the transforms build their own device copies.
+// by the library in FP64 on the host (mdft_kproj, mdft_angular_tables); the
+// orientation weights m_w are a device view, as the reference's View1DType
+// (:120, :162) that get_delta_f reads, the other tables host vectors.
 #pragma once
 
 #include <algorithm>
@@ -62,9 +63,11 @@ struct AngularGrid {
   IntType m_mmax, m_molrotsymorder, m_ntheta, m_nphi, m_npsi, m_no, m_np;
   ScalarType m_dphi, m_dpsi;
   ScalarType m_quadrature_norm = 8.0 * M_PI * M_PI;
-  // theta nodes (acos of the Gauss-Legendre nodes, descending), their
-  // weights (sum 1), and the orientation weights w(io) (sum 8 pi^2 / mrso)
-  std::vector<ScalarType> m_thetaofntheta, m_wthetaofntheta, m_w;
+  // theta nodes (acos of the Gauss-Legendre nodes, descending) and their
+  // weights (sum 1)
+  std::vector<ScalarType> m_thetaofntheta, m_wthetaofntheta;
+  // the orientation weights w(io) (sum 8 pi^2 / mrso), on the device
+  View<ScalarType, 1> m_w;
 
   AngularGrid(int mmax, int molrotsymorder) : m_mmax(mmax), m_molrotsymorder(molrotsymorder) {
     Impl::Throw_If(mmax <= 0, "The grid mmax must be greater than zero.");
@@ -79,7 +82,8 @@ struct AngularGrid {
                                     nullptr, nullptr, nullptr, nullptr));
     m_thetaofntheta.assign(th.begin(), th.end());
     m_wthetaofntheta.assign(wth.begin(), wth.end());
-    m_w.assign(w.begin(), w.end());
+    m_w = View<ScalarType, 1>("w", m_no);
+    deep_copy(m_w, std::vector<ScalarType>(w.begin(), w.end()));
   }
 };
 

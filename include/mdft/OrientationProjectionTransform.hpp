@@ -48,6 +48,16 @@ struct OrientationProjectionMap {
   std::vector<ScalarType> m_wigner_small_d;
 };
 
+namespace Impl {
+// the map passed to a transform or convolution must describe its angular grid
+template <typename MapType, typename AngularGridType>
+void check_map(const MapType& map, const AngularGridType& angular_grid) {
+  Throw_If(map.m_mmax != angular_grid.m_mmax || map.m_mrso != angular_grid.m_molrotsymorder ||
+               map.m_np != angular_grid.m_np,
+           "the OrientationProjectionMap was built for another angular grid");
+}
+}  // namespace Impl
+
 template <typename ExecutionSpace, typename ScalarType>
 class OrientationProjectionTransform {
   using SpatialGridType = SpatialGrid<ExecutionSpace, ScalarType>;
@@ -60,7 +70,7 @@ class OrientationProjectionTransform {
                                  const OrientationProjectionMapType& map, Precision precision = Precision::Default,
                                  ExecutionSpace exec = ExecutionSpace())
       : m_exec(exec), m_precision(precision_code<ScalarType>(precision)) {
-    (void)map;
+    Impl::check_map(map, angular_grid);
     Impl::check(mdft_opt_create(spatial_grid.m_nx, spatial_grid.m_ny, spatial_grid.m_nz, angular_grid.m_mmax,
                                 angular_grid.m_molrotsymorder, m_precision, &m_h));
     m_no = static_cast<size_t>(angular_grid.m_no);

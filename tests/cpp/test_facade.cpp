@@ -93,9 +93,8 @@ constexpr double tol() {
 
 // ---- grids and maps (host) -------------------------------------------------
 template <typename T>
-void test_grids() {
+void test_spatial_grid() {
   using SG = MDFT::SpatialGrid<execution_space, T>;
-  using AG = MDFT::AngularGrid<execution_space, T>;
   SG grid({8, 6, 5}, {10, 9, 7.5});
   EXPECT_TRUE(grid.m_nx == 8 && grid.m_ny == 6 && grid.m_nz == 5);
   EXPECT_TRUE(std::abs(grid.m_dv - T(10.0 / 8 * 9.0 / 6 * 7.5 / 5)) <= 1e-6 * grid.m_dv);
@@ -104,10 +103,16 @@ void test_grids() {
     const double k = (i <= 4 ? i : i - 8) * 2.0 * M_PI / 10.0;
     EXPECT_TRUE(std::abs(grid.m_kx[i] - T(k)) <= 1e-6);
   }
+}
+
+// AngularGrid (its weights are a device view) and OrientationProjectionMap
+template <typename T>
+void test_angular_grid() {
+  using AG = MDFT::AngularGrid<execution_space, T>;
   AG ag(5, 2);
   EXPECT_TRUE(ag.m_ntheta == 6 && ag.m_nphi == 11 && ag.m_npsi == 5 && ag.m_no == 330 && ag.m_np == 88);
   double sw = 0.0;
-  for (T v : ag.m_w) sw += v;
+  for (T v : MDFT::create_mirror_view_and_copy(ag.m_w)) sw += v;
   EXPECT_TRUE(std::abs(sw - 8.0 * M_PI * M_PI / 2.0) <= 1e-5 * sw);
   MDFT::OrientationProjectionMap<execution_space, T> map(ag);
   EXPECT_TRUE(static_cast<int>(map.m().size()) == 88 && map.wigner_small_d().size() == 6u * 88u);
@@ -345,6 +350,93 @@ void test_excess_functional(const std::string& path, int np_luc) {
 }
 
 
+// Solver::energy_cproj_mrso (src/mini-mdft.hpp:137-193) written against the
+// facade as a caller of the reference writes it: Solvents holds xi, rho0 and
+// the prefactor, the stages run one by one through the reference's views
+// (delta_rho and vexc (N, ntheta, nphi, npsi), delta_rho_p (np, nx, ny, nz),
+// vexc re-viewed as (nx, ny, nz, no)), get_delta_f reads the angular grid's
+// device weights.  FP64 to 1e-10 of the oracle; the float weight view and the
+// map / grid mismatch checks.
+void test_solvent_dropin(const std::string& path, int np_luc) {
+  using View4DType = MDFT::View<double, 4>;
+  using ComplexView4DType = MDFT::View<std::complex<double>, 4>;
+  const int n = 8;
+  const double L = 10.0;
+  execution_space exec;
+  MDFT::SpatialGrid<execution_space, double> grid({n, n, n}, {L, L, L});
+  MDFT::AngularGrid<execution_space, double> ag(3, 2);
+  MDFT::OrientationProjectionMap<execution_space, double> map(ag);
+  MDFT::Thermo<double> thermo(298.15);
+  MDFT::Solvents<execution_space, double> solvents(grid, ag, "spce", thermo);
+  auto& solvent = solvents.m_solvents.at(0);
+  const int no = ag.m_no, np = ag.m_np;
+  const size_t total = static_cast<size_t>(n) * n * n * no;
+  // Solvents' values (common/MDFT_Solvent.hpp:417-440, :366-368)
+  const double kT = 1.38064852e-23 * 6.02214129e23 * 298.15 * 1e-3;
+  EXPECT_TRUE(std::abs(thermo.m_kbt - kT) <= 1e-15 * kT && solvent.m_npluc[3] == np_luc);
+  EXPECT_TRUE(std::abs(solvent.m_rho0 - 0.0332891 / (4 * M_PI * M_PI)) <= 1e-15 * solvent.m_rho0);
+  EXPECT_TRUE(std::abs(solvents.m_prefactor + 2 * kT * 2 * M_PI * M_PI / 0.0332891) <= 1e-12 * kT / 0.0332891);
+  EXPECT_TRUE(solvent.m_xi.extent(0) == static_cast<size_t>(n) && solvent.m_xi.extent(3) == static_cast<size_t>(no));
+  for (double v : MDFT::create_mirror_view_and_copy(solvent.m_xi)) EXPECT_TRUE(v == 1.0);
+  EXPECT_TRUE(solvent.m_site.size() == 3 && solvent.m_site[1].q == 0.4238 && solvent.m_site[2].x == -0.816495);
+  std::mt19937_64 gen(9);
+  std::uniform_real_distribution<double> U(-1.0, 1.0);
+  std::vector<double> xi(total);
+  for (double& v : xi) v = 1.0 + 0.1 * U(gen);
+  MDFT::deep_copy(solvent.m_xi, xi);
+
+  MDFT::OrientationProjectionTransform<execution_space, double> op_transform(grid, ag, map);
+  MDFT::Convolution<execution_space, double> conv(path, grid, ag, map, solvent.m_npluc[ag.m_mmax]);
+  View4DType delta_rho("delta_rho", n * n * n, ag.m_ntheta, ag.m_nphi, ag.m_npsi);
+  View4DType vexc_o("vexc", n * n * n, ag.m_ntheta, ag.m_nphi, ag.m_npsi);
+  View4DType df("df", n, n, n, no);
+  ComplexView4DType delta_rho_p("delta_rho_p", np, n, n, n);
+  MDFT::get_delta_rho(exec, solvent.m_xi, delta_rho, solvent.m_rho0);
+  op_transform.angl2proj(delta_rho, delta_rho_p);
+  conv.execute(delta_rho_p);
+  op_transform.proj2angl(delta_rho_p, vexc_o);
+  View4DType vexc(vexc_o.data(), n, n, n, no);
+  double ff = 0.0;
+  MDFT::get_delta_f(exec, solvent.m_xi, vexc, ag.m_w, df, ff, solvent.m_rho0, solvents.m_prefactor);
+  exec.fence();
+
+  const double qmax = std::sqrt(3.0) * M_PI * n / L;
+  LucFile f = read_luc(path, np_luc, qmax);
+  std::vector<int> cols[5];
+  for (int j = 0; j < 5; ++j)
+    for (int i = 0; i < np_luc; ++i) cols[j].push_back(f.tuples[i * 5 + j]);
+  void* oc = orc_conv_create(n, n, n, L, L, L, 3, 2, np_luc, cols[0].data(), cols[1].data(), cols[2].data(),
+                             cols[3].data(), cols[4].data(), reinterpret_cast<const double*>(f.c.data()), f.nq, f.dq);
+  std::vector<double> df_ref(total);
+  double ff_ref = 0.0;
+  orc_energy_cproj(oc, xi.data(), solvent.m_rho0, solvents.m_prefactor, df_ref.data(), &ff_ref, nullptr);
+  orc_conv_destroy(oc);
+  std::printf("    F_exc = %.12g kJ/mol (oracle %.12g)\n", ff, ff_ref);
+  EXPECT_TRUE(std::abs(ff - ff_ref) <= 1e-10 * std::abs(ff_ref));
+  EXPECT_TRUE(rel_err(MDFT::create_mirror_view_and_copy(df), df_ref) < 1e-10);
+
+  // a float weight view: w rounded to float, the sum in FP64
+  MDFT::AngularGrid<execution_space, float> agf(3, 2);
+  double ff32 = 0.0;
+  MDFT::get_delta_f(exec, solvent.m_xi, vexc, agf.m_w, df, ff32, solvent.m_rho0, solvents.m_prefactor);
+  EXPECT_TRUE(std::abs(ff32 - ff) <= 1e-6 * std::abs(ff));
+
+  // the reference's error paths
+  MDFT::AngularGrid<execution_space, double> ag5(5, 2), ag6(6, 2);
+  MDFT::OrientationProjectionMap<execution_space, double> map5(ag5);
+  expect_throw([&] { MDFT::OrientationProjectionTransform<execution_space, double> t(grid, ag, map5); },
+               "built for another angular grid");
+  expect_throw([&] { MDFT::Convolution<execution_space, double> c(path, grid, ag, map5, np_luc); },
+               "built for another angular grid");
+  expect_throw([&] { MDFT::Solvents<execution_space, double> s(grid, ag6, "tip3p", thermo); },
+               "solvent tip3p only avail with mmax between 0 and 5");
+  expect_throw([&] { MDFT::Solvents<execution_space, double> s(grid, ag, "argon", thermo); },
+               "is not implemented yet");
+  MDFT::Solvents<execution_space, double> half(grid, ag, "tip3p-m", thermo, 0.5);
+  EXPECT_TRUE(half.m_solvents.at(0).m_name == "tip3p" && half.m_solvents.at(0).m_n0 == 0.03349459 * 0.5 &&
+              half.m_solvents.at(0).m_site[0].q == -0.95);
+}
+
 // The full functional through the facade: V_ext of one ion seen by a one-site
 // solvent against the closed form, F and g against host sums over the oracle's
 // excess evaluation, and a few L-BFGS iterations lowering F.
@@ -406,8 +498,9 @@ void test_functional(const std::string& path, int np_luc) {
   const double dv = L * L * L / N;
   double fid = 0.0, fext = 0.0;
   std::vector<double> g_ref(total);
+  const auto wh = MDFT::create_mirror_view_and_copy(ag.m_w);
   for (size_t i = 0; i < total; ++i) {
-    const double w = ag.m_w[i % no], x2 = xi[i] * xi[i];
+    const double w = wh[i % no], x2 = xi[i] * xi[i];
     fid += w * (x2 * std::log(x2) - x2 + 1.0);
     fext += w * x2 * vh[i];
     g_ref[i] = dv * w * (2 * rho0 * xi[i] * (kT * std::log(x2) + vh[i]) + 2 * pref / N * df_ref[i]);
@@ -495,17 +588,10 @@ void test_server(const std::string& path, int np_luc) {
 
 int main(int argc, char** argv) {
   const bool host_only = argc > 1 && std::strcmp(argv[1], "--host-only") == 0;
-  run("SpatialGrid/AngularGrid/OrientationProjectionMap<float>", [] { test_grids<float>(); });
-  run("SpatialGrid/AngularGrid/OrientationProjectionMap<double>", [] { test_grids<double>(); });
+  run("SpatialGrid<float>", [] { test_spatial_grid<float>(); });
+  run("SpatialGrid<double>", [] { test_spatial_grid<double>(); });
   run("Errors<float>", [] { test_errors<float>(); });
   run("Errors<double>", [] { test_errors<double>(); });
-  run("Convolution: missing Luc file", [] {
-    MDFT::SpatialGrid<execution_space, double> grid({4, 4, 4}, {5, 5, 5});
-    MDFT::AngularGrid<execution_space, double> ag(3, 2);
-    MDFT::OrientationProjectionMap<execution_space, double> map(ag);
-    expect_throw([&] { MDFT::Convolution<execution_space, double> c("/nonexistent/c.luc", grid, ag, map, 252); },
-                 "does not exist");
-  });
   if (!host_only) {
     if (argc < 3) {
       std::printf("usage: test_facade --host-only | test_facade <luc_file> <np_luc>\n");
@@ -513,6 +599,15 @@ int main(int argc, char** argv) {
     }
     const std::string path = argv[1];
     const int np_luc = std::atoi(argv[2]);
+    run("AngularGrid/OrientationProjectionMap<float>", [] { test_angular_grid<float>(); });
+    run("AngularGrid/OrientationProjectionMap<double>", [] { test_angular_grid<double>(); });
+    run("Convolution: missing Luc file", [] {
+      MDFT::SpatialGrid<execution_space, double> grid({4, 4, 4}, {5, 5, 5});
+      MDFT::AngularGrid<execution_space, double> ag(3, 2);
+      MDFT::OrientationProjectionMap<execution_space, double> map(ag);
+      expect_throw([&] { MDFT::Convolution<execution_space, double> c("/nonexistent/c.luc", grid, ag, map, 252); },
+                   "does not exist");
+    });
     run("OrientationProjectionTransform<float>", [] { test_angl2proj_proj2angl<float>(3, 4, 2, 3, 2, MDFT::Precision::Default); });
     run("OrientationProjectionTransform<double>", [] { test_angl2proj_proj2angl<double>(3, 4, 2, 3, 2, MDFT::Precision::Default); });
     run("OrientationProjectionTransform<double> mmax=5", [] { test_angl2proj_proj2angl<double>(4, 3, 5, 5, 2, MDFT::Precision::Default); });
@@ -526,6 +621,7 @@ int main(int argc, char** argv) {
     run("get_delta_rho/get_delta_f<float>", [] { test_solvent_functions<float>(); });
     run("get_delta_rho/get_delta_f<double>", [] { test_solvent_functions<double>(); });
     run("ExcessFunctional energy_cproj_mrso 8^3", [&] { test_excess_functional(path, np_luc); });
+    run("Solvents + stages, the reference's energy_cproj_mrso 8^3", [&] { test_solvent_dropin(path, np_luc); });
     run("Functional: V_ext, eval, minimize 8^3", [&] { test_functional(path, np_luc); });
     run("Server: alone == in a batch 8^3", [&] { test_server(path, np_luc); });
   }
