@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstring>
 #include <type_traits>
 
 namespace mdft {
@@ -651,6 +652,257 @@ __global__ __launch_bounds__(256) void oz_pairs_ct_kernel(cplx<T>* __restrict__ 
   }
 }
 
+// The OZ pass over stars, for compile-time grids.  A star is the set of
+// (q, -q) pairs whose q differ only in the signs of their components: the
+// pairs of (+-kx, +-ky, +-kz), at most four.  The k grid holds -k as the exact
+// negation of k, so every pair of a star has the same |q| bit for bit, hence
+// the same c(q) row iq, the same interpolation weights and the same lerped
+// row.  One block takes one star: the lerped row goes to LDS once, from loads
+// spread over all the block's threads, and serves every pair of the star
+// (four pairs per c(q) row instead of one).  Each pair keeps its own LDS region
+// and the per-pair arithmetic of oz_pairs_ct_kernel (gathers, phases, half d
+// table, rotation, OZ product, chop, inverse rotation, stores), so both
+// kernels give the same bits.  Stars are ordered by (|ix|, |iy|, |iz|): the
+// pairs of consecutive stars lie on the same z lines at neighbouring +-iz, and
+// a line's cache lines serve the blocks of consecutive stars.  Blocks past the
+// last star (the XCD order pads the grid) request nothing.
+template <typename T, int MMAX, int MRSO, int THREADS>
+__global__ __launch_bounds__(THREADS) void oz_star_ct_kernel(cplx<T>* __restrict__ p, const T* __restrict__ dtab,
+                                                             const OzPairMeta<T>* __restrict__ meta,
+                                                             const cplx<T>* __restrict__ phase,
+                                                             const int* __restrict__ dmap,
+                                                             const int4* __restrict__ stars, long nstars,
+                                                             OzArgs<T> a, int cbytes, int region) {
+  using S = OzShape<MMAX, MRSO>;
+  constexpr int W = S::W, NP = S::NP, NC = oz_terms<MMAX, MRSO>();
+  constexpr int NH = (MMAX + 1) * (MMAX + 2) / 2, NE = NH * W;
+  constexpr int NU = (NE + NP - 1) / NP, NCU = (NC + THREADS - 1) / THREADS;
+  constexpr int PS = 4;  // pairs of a star
+  static_assert(PS * NP <= THREADS, "one thread per (pair, projection)");
+  static_assert(NU <= kDmapPerThread, "d entries per thread of the thread records");
+  extern __shared__ unsigned char lds_raw[];
+  __shared__ cplx<T> phs[PS][W];  // e^{-i mup phi_q} of each pair
+  const int tid = threadIdx.x;
+  const int pp = tid / NP;  // pair of this thread within the star
+  const int ip = tid - pp * NP;
+  const int ppc = pp < PS ? pp : 0;
+  int job;
+  const long star = oz_block_pair(a, job);
+  const bool sval = star < nstars;  // block-uniform
+  int4 st = make_int4(-1, -1, -1, -1);
+  if (sval) st = stars[star];
+  const int pair = ppc == 0 ? st.x : ppc == 1 ? st.y : ppc == 2 ? st.z : st.w;
+  const bool active = sval && pp < PS && pair >= 0;  // uniform over the threads of one pair
+  cplx<T>* ceff = reinterpret_cast<cplx<T>*>(lds_raw);
+  cplx<T>* G = reinterpret_cast<cplx<T>*>(lds_raw + cbytes + static_cast<long>(ppc) * region);
+  cplx<T>* Gm = G + S::NG * W;
+  cplx<T>* Dq = Gm + S::NG * W;
+  cplx<T>* Dmq = Dq + NP;
+  T* dP0 = reinterpret_cast<T*>(Dmq + NP);  // leading zero row, then ND rows
+  T* dP = dP0 + W;
+  cplx<T>* ph = phs[ppc];
+  p += static_cast<long>(job) * a.N;
+
+  // the star's c(q) rows iq and iq + 1, over all threads of the block (the
+  // first pair of a star is always present)
+  constexpr bool C2 = std::is_same<T, float>::value && NC % 2 == 0;
+  constexpr int NC2 = NC / 2, NCU2 = (NC2 + THREADS - 1) / THREADS;
+  cplx<T> lo[C2 ? 1 : NCU], hi[C2 ? 1 : NCU];
+  float4 lo4[C2 ? NCU2 : 1], hi4[C2 ? NCU2 : 1];
+  T al = 0, bl = 0;
+  if (sval) {
+    const OzPairMeta<T> m0 = meta[st.x];  // block-uniform
+    al = m0.al;
+    bl = m0.bl;
+    if constexpr (C2) {
+      const float4* c4 = reinterpret_cast<const float4*>(a.ctab);
+      const unsigned crow2 = static_cast<unsigned>(m0.iq) * NC2;
+#pragma unroll
+      for (int k = 0; k < NCU2; ++k) {
+        const int j = tid + k * THREADS;
+        const unsigned at = crow2 + static_cast<unsigned>(min(j, NC2 - 1));
+        lo4[k] = ld_off(c4, at);
+        hi4[k] = ld_off(c4, at + NC2);
+      }
+    } else {
+      const unsigned crow = static_cast<unsigned>(m0.iq) * NC;
+#pragma unroll
+      for (int k = 0; k < NCU; ++k) {
+        const int ia = tid + k * THREADS;
+        const unsigned at = crow + static_cast<unsigned>(min(ia, NC - 1));
+        lo[k] = ld_off(a.ctab, at);
+        hi[k] = ld_off(a.ctab, at + NC);
+      }
+    }
+  }
+
+  int info = 0, ias = 0, pk = 0;
+  T g0x = 0, g0y = 0, g1x = 0, g1y = 0;
+  long sq = 0, smq = 0;
+  bool singular = false;
+  if (active) {
+    // this pair's half d table and phase (they need only the pair index)
+    T dreg[NU];
+    const T* dpair = dtab + static_cast<long>(pair) * NE;
+#pragma unroll
+    for (int k = 0; k < NU; ++k) {
+      const int e = ip + k * NP;
+      dreg[k] = ld_off(dpair, static_cast<unsigned>(min(e, NE - 1)));
+    }
+    const int4 rec = reinterpret_cast<const int4*>(dmap)[ip];
+    const int dcode[kDmapPerThread] = {rec.x, rec.y, rec.z};
+    cplx<T> phv = mkc<T>(0, 0);
+    const int amup = ip < MMAX ? MMAX - ip : ip - MMAX;
+    if (ip < W) phv = phase[static_cast<long>(pair) * (MMAX + 1) + amup];
+    pk = meta[pair].pk;
+    const int ix = pk & 1023, iy = (pk >> 10) & 1023, iz = (pk >> 20) & 1023;
+    const int ixm = ix == 0 ? 0 : a.nx - ix, iym = iy == 0 ? 0 : a.ny - iy, izm = iz == 0 ? 0 : a.nz - iz;
+    sq = (static_cast<long>(ix) * a.ny + iy) * a.nz + iz;
+    smq = (static_cast<long>(ixm) * a.ny + iym) * a.nz + izm;
+    singular = (ixm == ix && iym == iy && izm == iz) && (ix == a.nx / 2 || iy == a.ny / 2 || iz == a.nz / 2);
+    // gathers (:420-425)
+    const cplx<T> u = p[static_cast<long>(ip) * a.pstride + sq], v = p[static_cast<long>(ip) * a.pstride + smq];
+    info = rec.w;
+    g0x = u.x;
+    g0y = u.y;
+    g1x = v.x;
+    g1y = v.y;
+    // d rows (l, mu >= 0) and their mirrors (l, -mu), at the LDS places
+    // the thread's map entries give; padding stays zero
+#pragma unroll
+    for (int k = 0; k < NU; ++k) {
+      const int code = dcode[k];
+      if (code & kDmapValid) {
+        const T dv = dreg[k];
+        dP[code & 0xfff] = dv;
+        dP[(code >> 12) & 0xfff] = (code & kDmapZero) ? T(0) : ((code & kDmapNeg) ? -dv : dv);
+      }
+    }
+    if (ip < W) {
+      dP0[ip] = T(0);
+      ph[ip] = ip < MMAX ? cconj<T>(phv) : phv;
+    }
+    // zero G and Gm (the padding past 2m+1 must stay zero)
+    for (int i = ip; i < 2 * S::NG * W; i += NP) G[i] = mkc<T>(0, 0);
+  }
+  // c(|q|) rows iq, iq+1 lerped into LDS (the expression of oz_pairs_ct_kernel)
+  if (sval) {
+    if constexpr (C2) {
+#pragma unroll
+      for (int k = 0; k < NCU2; ++k) {
+        const int j = tid + k * THREADS;
+        if (j < NC2) {
+          float4 v;
+          v.x = al * hi4[k].x + bl * lo4[k].x;
+          v.y = al * hi4[k].y + bl * lo4[k].y;
+          v.z = al * hi4[k].z + bl * lo4[k].z;
+          v.w = al * hi4[k].w + bl * lo4[k].w;
+          reinterpret_cast<float4*>(ceff)[j] = v;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < NCU; ++k) {
+        const int ia = tid + k * THREADS;
+        if (ia < NC) ceff[ia] = mkc<T>(al * hi[k].x + bl * lo[k].x, al * hi[k].y + bl * lo[k].y);
+      }
+    }
+  }
+  __syncthreads();
+
+  const int m = info & 0xf, mupm = (info >> 4) & 0xf, mu2 = (info >> 8) & 0xf;
+  ias = info >> 12;
+  const int chi = mupm - MMAX;  // the same field read as chi in the q frame
+  const int row = grow_rt<MMAX, MRSO>(m) + mu2;
+  // phased gathers g' = g e^{-i mup phi} into G[(m, mu2)][mup + m]
+  if (active) {
+    const cplx<T> e = ph[mupm];
+    const int at = row * W + mupm - MMAX + m;
+    G[at] = cmul<T>(mkc<T>(g0x, g0y), e);
+    Gm[at] = cmul<T>(mkc<T>(g1x, g1y), e);
+  }
+  __syncthreads();
+
+  // rotation to the q frame (:427-443)
+  if (active) {
+    const cplx<T>* g0 = G + row * W;
+    const cplx<T>* g1 = Gm + row * W;
+    const T* d0 = dP + m * m * W + (chi + m);
+    const T* d1 = dP + m * m * W + (m - chi);
+    T ar = 0, ai = 0, br = 0, bi = 0;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const cplx<T> x = g0[j], y = g1[j];
+      const T u = d0[j * W], v = d1[j * W];
+      ar = fma(x.x, u, ar);
+      ai = fma(x.y, u, ai);
+      br = fma(y.x, v, br);
+      bi = fma(y.y, v, bi);
+    }
+    Dq[ip] = mkc<T>(ar, ai);
+    Dmq[ip] = (m & 1) ? mkc<T>(-br, -bi) : mkc<T>(br, bi);
+  }
+  __syncthreads();
+
+  // OZ product (:468-496) and chop (:514-517)
+  if (active) {
+    const int achi = chi < 0 ? -chi : chi;
+    const cplx<T>* ce = ceff + (ias - tofs_rt<MMAX, MRSO>(achi));
+    cplx<T> s0 = mkc<T>(0, 0), s1 = mkc<T>(0, 0);
+#pragma unroll
+    for (int n = 0; n <= MMAX; ++n) {
+      if (n >= achi) {
+        const int h = n / MRSO, kn = n / MRSO + 1;
+        const cplx<T>* cn = ce + S::tofs(n);
+        const int srcb = S::mbase(n) + (chi + n) * kn;
+#pragma unroll
+        for (int k = 0; k <= 2 * h; ++k) {
+          const int nu2 = k - h;
+          const cplx<T> c = cn[k];
+          if (nu2 < 0) {
+            const int src = srcb - nu2;
+            s0 = cfma<T>(s0, c, Dq[src]);
+            s1 = cfma<T>(s1, c, Dmq[src]);
+          } else {
+            const int src = srcb + nu2;
+            s0 = cfma_conjb<T>(s0, c, Dmq[src]);
+            s1 = cfma_conjb<T>(s1, c, Dq[src]);
+          }
+        }
+      }
+    }
+    const int at = row * W + chi + m;
+    G[at] = chop<T>(s0, a.chop_eps);
+    Gm[at] = chop<T>(s1, a.chop_eps);
+  }
+  __syncthreads();
+
+  // inverse rotation and stores (:518-541)
+  if (active) {
+    const int mup = mupm - MMAX;
+    const cplx<T>* g0 = G + row * W;
+    const cplx<T>* g1 = Gm + row * W;
+    const T* dr = dP + (m * m + mup + m) * W;
+    const T* drm = dr + 2 * m;  // d_{mup,-chi} = drm[-j]; reads past the row start meet gamma's zeros
+    T ar = 0, ai = 0, br = 0, bi = 0;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const cplx<T> x = g0[j], y = g1[j];
+      const T u = dr[j], v = drm[-j];
+      ar = fma(x.x, u, ar);
+      ai = fma(x.y, u, ai);
+      br = fma(y.x, v, br);
+      bi = fma(y.y, v, bi);
+    }
+    const cplx<T> e = ph[mupm];  // e^{-i mup phi}; the output takes its conjugate
+    const cplx<T> o0 = cmul<T>(mkc<T>(ar, ai), cconj<T>(e));
+    cplx<T> o1 = cmul<T>(mkc<T>(br, bi), cconj<T>(e));
+    if (m & 1) o1 = mkc<T>(-o1.x, -o1.y);
+    p[static_cast<long>(ip) * a.pstride + sq] = o0;
+    p[static_cast<long>(ip) * a.pstride + smq] = singular ? cconj<T>(o1) : o1;
+  }
+}
+
 // One pair per 64-lane block (per-evaluation pass).
 template <typename T, bool STAGE_C>
 __global__ __launch_bounds__(kWave) void oz_pair_kernel(cplx<T>* __restrict__ p, const T* __restrict__ dtab,
@@ -906,6 +1158,26 @@ std::vector<int> qpairs_memory_order(int nx, int ny, int nz) {
   return out;
 }
 
+std::vector<int> oz_stars(const std::vector<int>& packed, int nx, int ny, int nz) {
+  auto fold = [](int i, int n) { return i <= n / 2 ? i : n - i; };
+  const long hx = nx / 2 + 1, hy = ny / 2 + 1, hz = nz / 2 + 1;
+  std::vector<int> slot(static_cast<size_t>(hx * hy * hz) * 4, -1);
+  for (size_t i = 0; i < packed.size(); ++i) {
+    const int pk = packed[i];
+    const int ix = pk & 1023, iy = (pk >> 10) & 1023, iz = (pk >> 20) & 1023;
+    const long key = (fold(ix, nx) * hy + fold(iy, ny)) * hz + fold(iz, nz);
+    int* s = &slot[static_cast<size_t>(key) * 4];
+    int k = 0;
+    while (k < 4 && s[k] >= 0) ++k;
+    throw_if(k == 4, "internal error: more than four pairs in a star");
+    s[k] = static_cast<int>(i);
+  }
+  std::vector<int> out;
+  for (size_t key = 0; key < slot.size() / 4; ++key)
+    if (slot[key * 4] >= 0) out.insert(out.end(), slot.begin() + key * 4, slot.begin() + key * 4 + 4);
+  return out;
+}
+
 ConvDevice::ConvDevice(int nx, int ny, int nz, double lx, double ly, double lz, int mmax, int mrso,
                        const LucTable& luc)
     : tab(make_conv_tables(nx, ny, nz, lx, ly, lz, mmax, mrso, luc)), fft(nx, ny, nz) {
@@ -959,6 +1231,14 @@ ConvDevice::ConvDevice(int nx, int ny, int nz, double lx, double ly, double lz, 
   coef.insert(coef.end(), &rc.c[0][0], &rc.c[0][0] + 5 * 13);
   coef.insert(coef.end(), &rc.d[0][0], &rc.d[0][0] + 5 * 13);
   if (dispatch_oz(mmax, mrso, [](auto, auto) {})) padw = 2 * mmax + 1;
+  {
+    const std::vector<int> st = oz_stars(packed, nx, ny, nz);
+    nstars = static_cast<long>(st.size() / 4);
+    long in_stars = 0;
+    for (int v : st) in_stars += v >= 0;
+    throw_if(in_stars != npairs, "internal error: the stars do not cover the pair list");
+    stars = upload(st);
+  }
   kxyz = upload(k3);
   pairs = upload(packed);
   ia_start = upload(tab.ia_start);
@@ -972,7 +1252,7 @@ ConvDevice::~ConvDevice() {
   for (void* q : {static_cast<void*>(kxyz), static_cast<void*>(pairs), static_cast<void*>(ia_start),
                   static_cast<void*>(ipinfo), static_cast<void*>(ctab64), static_cast<void*>(ctab32),
                   static_cast<void*>(rcoef), dtab64, dtab32, pmeta32, pphase32, pmeta64, pphase64,
-                  static_cast<void*>(dmap)})
+                  static_cast<void*>(dmap), static_cast<void*>(stars)})
     if (q) (void)hipFree(q);
 }
 
@@ -1036,6 +1316,28 @@ const void* ConvDevice::pair_meta(hipStream_t s, const void** phases) const {
                        static_cast<cplx<T>*>(phase), a);
     MDFT_HIP_CHECK(hipGetLastError());
   }
+  bool& checked = f64 ? stars_checked64 : stars_checked32;
+  if (!checked) {
+    // the star kernel lerps one c(q) row per star: every pair of a star
+    // must have the same row and weights, bit for bit (once per Convolution)
+    std::vector<OzPairMeta<T>> hm(static_cast<size_t>(npairs));
+    std::vector<int> hs(static_cast<size_t>(nstars) * 4);
+    MDFT_HIP_CHECK(hipMemcpyAsync(hm.data(), meta, hm.size() * sizeof(OzPairMeta<T>), hipMemcpyDeviceToHost, s));
+    MDFT_HIP_CHECK(hipMemcpyAsync(hs.data(), stars, hs.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    MDFT_HIP_CHECK(hipStreamSynchronize(s));
+    for (long i = 0; i < nstars; ++i) {
+      const OzPairMeta<T>& m0 = hm[hs[4 * i]];
+      for (int k = 1; k < 4; ++k) {
+        const int j = hs[4 * i + k];
+        if (j < 0) continue;
+        const OzPairMeta<T>& mj = hm[j];
+        throw_if(mj.iq != m0.iq || std::memcmp(&mj.al, &m0.al, sizeof(T)) != 0 ||
+                     std::memcmp(&mj.bl, &m0.bl, sizeof(T)) != 0,
+                 "internal error: the pairs of a star have different c(q) rows or weights");
+      }
+    }
+    checked = true;
+  }
   *phases = phase;
   return meta;
 }
@@ -1084,6 +1386,24 @@ void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch,
                               static_cast<size_t>(Sh::W) * (Sh::ND + 1) * sizeof(T);
       const size_t region = (pair_lds + 15) / 16 * 16;
       throw_if(PB * region > 64 * 1024, "angular grid too large for the OZ kernel");
+      constexpr int TH = (4 * Sh::NP + kWave - 1) / kWave * kWave;
+      const size_t cbytes = (static_cast<size_t>(NC) * sizeof(cplx<T>) + 15) / 16 * 16;
+      const size_t star_region =
+          ((2 * static_cast<size_t>(Sh::NG) * Sh::W + 2 * Sh::NP) * sizeof(cplx<T>) +
+           static_cast<size_t>(Sh::W) * (Sh::ND + 1) * sizeof(T) + 15) / 16 * 16;
+      if constexpr (TH <= 1024) {
+        if (cbytes + 4 * star_region + 4 * Sh::W * sizeof(cplx<T>) <= 64 * 1024) {
+          OzArgs<T> ag = a;
+          ag.chunk = (nstars + kXcds - 1) / kXcds;
+          const long nb = ag.chunk * kXcds * nbatch;
+          throw_if(nb > 0x7fffffffL, "too many q pairs");
+          launch_oz(oz_star_ct_kernel<T, MMAX, MRSO, TH>, static_cast<unsigned>(nb), static_cast<unsigned>(TH),
+                    cbytes + 4 * star_region, s, ev0, ev1, p, dt, mt, ph, dmap,
+                    reinterpret_cast<const int4*>(stars), nstars, ag, static_cast<int>(cbytes),
+                    static_cast<int>(star_region));
+          return;
+        }
+      }
       OzArgs<T> ag = a;
       const long ngroups = (npairs + PB - 1) / PB;
       ag.chunk = (ngroups + kXcds - 1) / kXcds;

@@ -40,6 +40,11 @@ struct ConvDevice {
   mutable void* pmeta64 = nullptr;
   mutable void* pphase64 = nullptr;
   mutable int* dmap = nullptr;  // per-thread records: LDS places of the half d table's entries, ipinfo
+  // stars of the pair list (pairs whose q differ only in component signs,
+  // which share |q| and so the c(q) row): four pair indices per star, -1 pads
+  int* stars = nullptr;
+  long nstars = 0;
+  mutable bool stars_checked32 = false, stars_checked64 = false;
   // row width of the padded d layout when the compile-time OZ kernel serves
   // this (mmax, mrso) (2 mmax + 1), else 0 (compact rows, generic kernel)
   int padw = 0;
@@ -61,5 +66,11 @@ struct ConvDevice {
 // designation, sorted by (ix, iy, iz) so that consecutive pairs are
 // neighbours in memory.
 std::vector<int> qpairs_memory_order(int nx, int ny, int nz);
+
+// The stars of a pair list given as packed ix | iy << 10 | iz << 20: the
+// pairs whose (ix, iy, iz) fold to the same (|ix|, |iy|, |iz|) (i folds to
+// min(i, n - i)), as four indices into the list per star (list order within
+// a star, -1 pads), the stars ordered by (|ix|, |iy|, |iz|).
+std::vector<int> oz_stars(const std::vector<int>& packed, int nx, int ny, int nz);
 
 }  // namespace mdft
