@@ -207,15 +207,18 @@ __device__ __forceinline__ double2 phase_pow(double q0, double q1, int mup) {
 // after; all (mmax+1)^2 rows, or with half only the (mmax+1)(mmax+2)/2 rows
 // of mup >= 0 (the rows of -mup follow from
 // d^l_{-mup,-chi} = (-1)^{mup-chi} d^l_{mup,chi}, which the values of the
-// FP64 recursion satisfy exactly).
+// FP64 recursion satisfy exactly).  With sel, row b is the table of pair
+// sel[4 b] (the first pair of star b), for the star kernel: the pairs of a
+// star share theta_q, and d depends on theta_q alone.
 template <typename T>
-__global__ __launch_bounds__(kWave) void oz_dtable_kernel(T* __restrict__ dtab, OzArgs<T> a, int padw, int half) {
+__global__ __launch_bounds__(kWave) void oz_dtable_kernel(T* __restrict__ dtab, OzArgs<T> a, int padw, int half,
+                                                          const int* __restrict__ sel, long nrows) {
   extern __shared__ unsigned char lds_raw[];
   double2* Rd = reinterpret_cast<double2*>(lds_raw);
   const int lane = threadIdx.x;
   const long pair = blockIdx.x;
-  if (pair >= a.npairs) return;
-  const int pk = a.pairs[pair];
+  if (pair >= nrows) return;
+  const int pk = a.pairs[sel ? sel[4 * pair] : pair];
   const int ix = pk & 1023, iy = (pk >> 10) & 1023, iz = (pk >> 20) & 1023;
   const double q0 = a.kxyz[ix], q1 = a.kxyz[a.nx + iy], q2 = a.kxyz[a.nx + a.ny + iz];
   rotation_recursion(Rd, a.rc, a.mmax, q0, q1, q2, lane);
@@ -376,6 +379,25 @@ constexpr int oz_pairs_per_block() {
 constexpr int kDmapNeg = 1 << 24, kDmapZero = 1 << 25, kDmapValid = 1 << 26;
 constexpr int kDmapPerThread = 3;
 
+// the LDS places code of half-table entry e (layout above)
+int oz_dtable_place(int mmax, int e) {
+  const int W = 2 * mmax + 1;
+  const int h = e / W, c = e % W;
+  int l = 0;
+  while ((l + 1) * (l + 2) / 2 <= h) ++l;
+  const int mu = h - l * (l + 1) / 2;
+  const int base = (l * l + l) * W;
+  int v = (base + mu * W + c) | kDmapValid;
+  if (c <= 2 * l) {
+    v |= (base - mu * W + 2 * l - c) << 12;
+    if ((mu + c + l) & 1) v |= kDmapNeg;
+  } else {
+    v |= (base - mu * W + c) << 12;
+    v |= kDmapZero;
+  }
+  return v;
+}
+
 std::vector<int> oz_thread_records(const AngularGrid& g, const std::vector<int>& ia_start) {
   const int mmax = g.mmax, np = g.np;
   const int W = 2 * mmax + 1, NE = (mmax + 1) * (mmax + 2) / 2 * W, nu = (NE + np - 1) / np;
@@ -389,24 +411,35 @@ std::vector<int> oz_thread_records(const AngularGrid& g, const std::vector<int>&
   for (int k = 0; k < nu; ++k)
     for (int ip = 0; ip < np; ++ip) {
       const int e = ip + k * np;
-      if (e >= NE) continue;
-      const int h = e / W, c = e % W;
-      int l = 0;
-      while ((l + 1) * (l + 2) / 2 <= h) ++l;
-      const int mu = h - l * (l + 1) / 2;
-      const int base = (l * l + l) * W;
-      int v = (base + mu * W + c) | kDmapValid;
-      if (c <= 2 * l) {
-        v |= (base - mu * W + 2 * l - c) << 12;
-        if ((mu + c + l) & 1) v |= kDmapNeg;
-      } else {
-        v |= (base - mu * W + c) << 12;
-        v |= kDmapZero;
-      }
-      code[static_cast<size_t>(ip) * 4 + k] = v;
+      if (e < NE) code[static_cast<size_t>(ip) * 4 + k] = oz_dtable_place(mmax, e);
     }
   return code;
 }
+
+// the LDS places codes of all NE half-table entries (the star kernel writes
+// the star's one table with one entry per thread)
+std::vector<int> oz_dtable_places(int mmax) {
+  const int NE = (mmax + 1) * (mmax + 2) / 2 * (2 * mmax + 1);
+  std::vector<int> code(NE);
+  for (int e = 0; e < NE; ++e) code[e] = oz_dtable_place(mmax, e);
+  return code;
+}
+
+// LDS of the star kernel for (T, MMAX, MRSO): the lerped c(q) row, the
+// star's d table (a leading zero row, then (mmax + 1)^2 rows), then four pair
+// regions (G, Gm, Dq, Dmq), each rounded up to 16 bytes
+template <typename T, int MMAX, int MRSO>
+struct StarLayout {
+  using S = OzShape<MMAX, MRSO>;
+  static constexpr int NC = oz_terms<MMAX, MRSO>();
+  static constexpr int threads = (4 * S::NP + kWave - 1) / kWave * kWave;
+  static constexpr size_t r16(size_t b) { return (b + 15) / 16 * 16; }
+  static constexpr size_t cbytes = r16(static_cast<size_t>(NC) * sizeof(cplx<T>));
+  static constexpr size_t dbytes = r16(static_cast<size_t>(S::W) * (S::ND + 1) * sizeof(T));
+  static constexpr size_t region = r16((2 * static_cast<size_t>(S::NG) * S::W + 2 * S::NP) * sizeof(cplx<T>));
+  static constexpr size_t dynamic = cbytes + dbytes + 4 * region;
+  static constexpr bool fits = threads <= 1024 && dynamic + 4 * S::W * sizeof(cplx<T>) <= 64 * 1024;
+};
 
 // element idx of a table at a 32-bit byte offset from a block-uniform base
 // (the load then takes a scalar base and a 32-bit vector offset)
@@ -659,10 +692,14 @@ __global__ __launch_bounds__(256) void oz_pairs_ct_kernel(cplx<T>* __restrict__ 
 // the same c(q) row iq, the same interpolation weights and the same lerped
 // row.  One block takes one star: the lerped row goes to LDS once, from loads
 // spread over all the block's threads, and serves every pair of the star
-// (four pairs per c(q) row instead of one).  Each pair keeps its own LDS region
-// and the per-pair arithmetic of oz_pairs_ct_kernel (gathers, phases, half d
-// table, rotation, OZ product, chop, inverse rotation, stores), so both
-// kernels give the same bits.  Stars are ordered by (|ix|, |iy|, |iz|): the
+// (four pairs per c(q) row instead of one).  The designated q of every pair
+// of a star has the same kz >= 0 and |q|, hence the same theta_q, and the real
+// rotation matrices d^l(theta_q) depend on theta_q alone: the star has one
+// half d table (built from its first pair), loaded with one entry per thread
+// and mirrored into one LDS table that all its pairs read.  Each pair keeps
+// its own LDS region for G, Gm, Dq, Dmq and the per-pair arithmetic of
+// oz_pairs_ct_kernel (gathers, phases, rotation, OZ product, chop, inverse
+// rotation, stores).  Stars are ordered by (|ix|, |iy|, |iz|): the
 // pairs of consecutive stars lie on the same z lines at neighbouring +-iz, and
 // a line's cache lines serve the blocks of consecutive stars.  Blocks past the
 // last star (the XCD order pads the grid) request nothing.
@@ -671,15 +708,16 @@ __global__ __launch_bounds__(THREADS) void oz_star_ct_kernel(cplx<T>* __restrict
                                                              const OzPairMeta<T>* __restrict__ meta,
                                                              const cplx<T>* __restrict__ phase,
                                                              const int* __restrict__ dmap,
+                                                             const int* __restrict__ dent,
                                                              const int4* __restrict__ stars, long nstars,
-                                                             OzArgs<T> a, int cbytes, int region) {
+                                                             OzArgs<T> a) {
   using S = OzShape<MMAX, MRSO>;
+  using Lay = StarLayout<T, MMAX, MRSO>;
   constexpr int W = S::W, NP = S::NP, NC = oz_terms<MMAX, MRSO>();
   constexpr int NH = (MMAX + 1) * (MMAX + 2) / 2, NE = NH * W;
-  constexpr int NU = (NE + NP - 1) / NP, NCU = (NC + THREADS - 1) / THREADS;
+  constexpr int NDU = (NE + THREADS - 1) / THREADS, NCU = (NC + THREADS - 1) / THREADS;
   constexpr int PS = 4;  // pairs of a star
-  static_assert(PS * NP <= THREADS, "one thread per (pair, projection)");
-  static_assert(NU <= kDmapPerThread, "d entries per thread of the thread records");
+  static_assert(PS * NP <= THREADS && Lay::threads == THREADS, "one thread per (pair, projection)");
   extern __shared__ unsigned char lds_raw[];
   __shared__ cplx<T> phs[PS][W];  // e^{-i mup phi_q} of each pair
   const int tid = threadIdx.x;
@@ -694,12 +732,12 @@ __global__ __launch_bounds__(THREADS) void oz_star_ct_kernel(cplx<T>* __restrict
   const int pair = ppc == 0 ? st.x : ppc == 1 ? st.y : ppc == 2 ? st.z : st.w;
   const bool active = sval && pp < PS && pair >= 0;  // uniform over the threads of one pair
   cplx<T>* ceff = reinterpret_cast<cplx<T>*>(lds_raw);
-  cplx<T>* G = reinterpret_cast<cplx<T>*>(lds_raw + cbytes + static_cast<long>(ppc) * region);
+  T* dP0 = reinterpret_cast<T*>(lds_raw + Lay::cbytes);  // leading zero row, then ND rows
+  T* dP = dP0 + W;
+  cplx<T>* G = reinterpret_cast<cplx<T>*>(lds_raw + Lay::cbytes + Lay::dbytes + ppc * Lay::region);
   cplx<T>* Gm = G + S::NG * W;
   cplx<T>* Dq = Gm + S::NG * W;
   cplx<T>* Dmq = Dq + NP;
-  T* dP0 = reinterpret_cast<T*>(Dmq + NP);  // leading zero row, then ND rows
-  T* dP = dP0 + W;
   cplx<T>* ph = phs[ppc];
   p += static_cast<long>(job) * a.N;
 
@@ -710,7 +748,17 @@ __global__ __launch_bounds__(THREADS) void oz_star_ct_kernel(cplx<T>* __restrict
   cplx<T> lo[C2 ? 1 : NCU], hi[C2 ? 1 : NCU];
   float4 lo4[C2 ? NCU2 : 1], hi4[C2 ? NCU2 : 1];
   T al = 0, bl = 0;
+  T dreg[NDU];
+  int dcode[NDU];
   if (sval) {
+    // the star's half d table, one entry per thread
+    const T* dstar = dtab + star * NE;  // block-uniform
+#pragma unroll
+    for (int k = 0; k < NDU; ++k) {
+      const int e = min(tid + k * THREADS, NE - 1);
+      dreg[k] = ld_off(dstar, static_cast<unsigned>(e));
+      dcode[k] = tid + k * THREADS < NE ? dent[e] : 0;
+    }
     const OzPairMeta<T> m0 = meta[st.x];  // block-uniform
     al = m0.al;
     bl = m0.bl;
@@ -741,16 +789,8 @@ __global__ __launch_bounds__(THREADS) void oz_star_ct_kernel(cplx<T>* __restrict
   long sq = 0, smq = 0;
   bool singular = false;
   if (active) {
-    // this pair's half d table and phase (they need only the pair index)
-    T dreg[NU];
-    const T* dpair = dtab + static_cast<long>(pair) * NE;
-#pragma unroll
-    for (int k = 0; k < NU; ++k) {
-      const int e = ip + k * NP;
-      dreg[k] = ld_off(dpair, static_cast<unsigned>(min(e, NE - 1)));
-    }
-    const int4 rec = reinterpret_cast<const int4*>(dmap)[ip];
-    const int dcode[kDmapPerThread] = {rec.x, rec.y, rec.z};
+    // this pair's phase and its projection's info
+    const int rec = reinterpret_cast<const int4*>(dmap)[ip].w;
     cplx<T> phv = mkc<T>(0, 0);
     const int amup = ip < MMAX ? MMAX - ip : ip - MMAX;
     if (ip < W) phv = phase[static_cast<long>(pair) * (MMAX + 1) + amup];
@@ -762,15 +802,20 @@ __global__ __launch_bounds__(THREADS) void oz_star_ct_kernel(cplx<T>* __restrict
     singular = (ixm == ix && iym == iy && izm == iz) && (ix == a.nx / 2 || iy == a.ny / 2 || iz == a.nz / 2);
     // gathers (:420-425)
     const cplx<T> u = p[static_cast<long>(ip) * a.pstride + sq], v = p[static_cast<long>(ip) * a.pstride + smq];
-    info = rec.w;
+    info = rec;
     g0x = u.x;
     g0y = u.y;
     g1x = v.x;
     g1y = v.y;
-    // d rows (l, mu >= 0) and their mirrors (l, -mu), at the LDS places
-    // the thread's map entries give; padding stays zero
+    if (ip < W) ph[ip] = ip < MMAX ? cconj<T>(phv) : phv;
+    // zero G and Gm (the padding past 2m+1 must stay zero)
+    for (int i = ip; i < 2 * S::NG * W; i += NP) G[i] = mkc<T>(0, 0);
+  }
+  if (sval) {
+    // d rows (l, mu >= 0) and their mirrors (l, -mu), at the LDS places the
+    // entry codes give; padding stays zero
 #pragma unroll
-    for (int k = 0; k < NU; ++k) {
+    for (int k = 0; k < NDU; ++k) {
       const int code = dcode[k];
       if (code & kDmapValid) {
         const T dv = dreg[k];
@@ -778,15 +823,8 @@ __global__ __launch_bounds__(THREADS) void oz_star_ct_kernel(cplx<T>* __restrict
         dP[(code >> 12) & 0xfff] = (code & kDmapZero) ? T(0) : ((code & kDmapNeg) ? -dv : dv);
       }
     }
-    if (ip < W) {
-      dP0[ip] = T(0);
-      ph[ip] = ip < MMAX ? cconj<T>(phv) : phv;
-    }
-    // zero G and Gm (the padding past 2m+1 must stay zero)
-    for (int i = ip; i < 2 * S::NG * W; i += NP) G[i] = mkc<T>(0, 0);
-  }
-  // c(|q|) rows iq, iq+1 lerped into LDS (the expression of oz_pairs_ct_kernel)
-  if (sval) {
+    if (tid < W) dP0[tid] = T(0);
+    // c(|q|) rows iq, iq+1 lerped into LDS (the expression of oz_pairs_ct_kernel)
     if constexpr (C2) {
 #pragma unroll
       for (int k = 0; k < NCU2; ++k) {
@@ -1230,13 +1268,26 @@ ConvDevice::ConvDevice(int nx, int ny, int nz, double lx, double ly, double lz, 
   coef.insert(coef.end(), &rc.b[0][0][0], &rc.b[0][0][0] + 5 * 13 * 13);
   coef.insert(coef.end(), &rc.c[0][0], &rc.c[0][0] + 5 * 13);
   coef.insert(coef.end(), &rc.d[0][0], &rc.d[0][0] + 5 * 13);
-  if (dispatch_oz(mmax, mrso, [](auto, auto) {})) padw = 2 * mmax + 1;
+  if (dispatch_oz(mmax, mrso, [&](auto mm, auto mr) {
+        constexpr int MMAX = decltype(mm)::value, MRSO = decltype(mr)::value;
+        star32 = StarLayout<float, MMAX, MRSO>::fits;
+        star64 = StarLayout<double, MMAX, MRSO>::fits;
+      }))
+    padw = 2 * mmax + 1;
   {
     const std::vector<int> st = oz_stars(packed, nx, ny, nz);
     nstars = static_cast<long>(st.size() / 4);
     long in_stars = 0;
     for (int v : st) in_stars += v >= 0;
     throw_if(in_stars != npairs, "internal error: the stars do not cover the pair list");
+    // the pairs of a star share one d table: their designated q must have
+    // the same iz (then the same kz and |q|, hence the same theta_q)
+    for (long i = 0; i < nstars; ++i)
+      for (int k = 1; k < 4; ++k) {
+        const int j = st[4 * i + k];
+        throw_if(j >= 0 && (packed[j] >> 20) != (packed[st[4 * i]] >> 20),
+                 "internal error: the pairs of a star differ in kz");
+      }
     stars = upload(st);
   }
   kxyz = upload(k3);
@@ -1252,7 +1303,7 @@ ConvDevice::~ConvDevice() {
   for (void* q : {static_cast<void*>(kxyz), static_cast<void*>(pairs), static_cast<void*>(ia_start),
                   static_cast<void*>(ipinfo), static_cast<void*>(ctab64), static_cast<void*>(ctab32),
                   static_cast<void*>(rcoef), dtab64, dtab32, pmeta32, pphase32, pmeta64, pphase64,
-                  static_cast<void*>(dmap), static_cast<void*>(stars)})
+                  static_cast<void*>(dmap), static_cast<void*>(stars), static_cast<void*>(dent)})
     if (q) (void)hipFree(q);
 }
 
@@ -1292,10 +1343,13 @@ const T* ConvDevice::dtable(hipStream_t s) const {
     const size_t rows = half ? static_cast<size_t>(tab.mmax + 1) * (tab.mmax + 2) / 2
                              : static_cast<size_t>(tab.mmax + 1) * (tab.mmax + 1);
     const size_t row = padw ? rows * padw : nR;
-    MDFT_HIP_CHECK(hipMalloc(&slot, std::max<size_t>(1, npairs * row * sizeof(T))));
+    // one table per star when the star kernel serves T, else one per pair
+    const bool star = std::is_same<T, double>::value ? star64 : star32;
+    const long nrows = star ? nstars : npairs;
+    MDFT_HIP_CHECK(hipMalloc(&slot, std::max<size_t>(1, nrows * row * sizeof(T))));
     OzArgs<T> a = oz_args<T>(*this, 0.0);
-    hipLaunchKernelGGL((oz_dtable_kernel<T>), dim3(static_cast<unsigned>(npairs)), dim3(kWave),
-                       nR * sizeof(double2), s, static_cast<T*>(slot), a, padw, half);
+    hipLaunchKernelGGL((oz_dtable_kernel<T>), dim3(static_cast<unsigned>(nrows)), dim3(kWave),
+                       nR * sizeof(double2), s, static_cast<T*>(slot), a, padw, half, star ? stars : nullptr, nrows);
     MDFT_HIP_CHECK(hipGetLastError());
   }
   return static_cast<const T*>(slot);
@@ -1386,23 +1440,17 @@ void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch,
                               static_cast<size_t>(Sh::W) * (Sh::ND + 1) * sizeof(T);
       const size_t region = (pair_lds + 15) / 16 * 16;
       throw_if(PB * region > 64 * 1024, "angular grid too large for the OZ kernel");
-      constexpr int TH = (4 * Sh::NP + kWave - 1) / kWave * kWave;
-      const size_t cbytes = (static_cast<size_t>(NC) * sizeof(cplx<T>) + 15) / 16 * 16;
-      const size_t star_region =
-          ((2 * static_cast<size_t>(Sh::NG) * Sh::W + 2 * Sh::NP) * sizeof(cplx<T>) +
-           static_cast<size_t>(Sh::W) * (Sh::ND + 1) * sizeof(T) + 15) / 16 * 16;
-      if constexpr (TH <= 1024) {
-        if (cbytes + 4 * star_region + 4 * Sh::W * sizeof(cplx<T>) <= 64 * 1024) {
-          OzArgs<T> ag = a;
-          ag.chunk = (nstars + kXcds - 1) / kXcds;
-          const long nb = ag.chunk * kXcds * nbatch;
-          throw_if(nb > 0x7fffffffL, "too many q pairs");
-          launch_oz(oz_star_ct_kernel<T, MMAX, MRSO, TH>, static_cast<unsigned>(nb), static_cast<unsigned>(TH),
-                    cbytes + 4 * star_region, s, ev0, ev1, p, dt, mt, ph, dmap,
-                    reinterpret_cast<const int4*>(stars), nstars, ag, static_cast<int>(cbytes),
-                    static_cast<int>(star_region));
-          return;
-        }
+      using Lay = StarLayout<T, MMAX, MRSO>;
+      if constexpr (Lay::fits) {
+        OzArgs<T> ag = a;
+        ag.chunk = (nstars + kXcds - 1) / kXcds;
+        const long nb = ag.chunk * kXcds * nbatch;
+        throw_if(nb > 0x7fffffffL, "too many q pairs");
+        if (!dent) dent = upload(oz_dtable_places(MMAX));
+        launch_oz(oz_star_ct_kernel<T, MMAX, MRSO, Lay::threads>, static_cast<unsigned>(nb),
+                  static_cast<unsigned>(Lay::threads), Lay::dynamic, s, ev0, ev1, p, dt, mt, ph, dmap, dent,
+                  reinterpret_cast<const int4*>(stars), nstars, ag);
+        return;
       }
       OzArgs<T> ag = a;
       const long ngroups = (npairs + PB - 1) / PB;

@@ -45,6 +45,10 @@ struct ConvDevice {
   int* stars = nullptr;
   long nstars = 0;
   mutable bool stars_checked32 = false, stars_checked64 = false;
+  // whether the star kernel serves this grid in FP32 / FP64 (then the d
+  // table holds one row per star)
+  bool star32 = false, star64 = false;
+  mutable int* dent = nullptr;  // LDS places codes of the half d table's entries (star kernel)
   // row width of the padded d layout when the compile-time OZ kernel serves
   // this (mmax, mrso) (2 mmax + 1), else 0 (compact rows, generic kernel)
   int padw = 0;
