@@ -8,12 +8,14 @@ synthetic c(q) for all 2002 nmax = 5 Luc tuples, xi = 1 + 0.1 U(-1, 1).
 One step = one Solver::energy_cproj_mrso evaluation (F_exc and df) with xi
 already resident in HBM.
 
-value = evaluations/s of MDFT_MIXED_X6, the mixed form that meets |dF| <= 0.01
-kJ/mol (FFT, rotation, OZ in FP32; angl2proj as a BF16x6 MFMA GEMM; proj2angl
-in FP32; FP64 rotation tables and reductions).  The plain FP64 form and the
-BF16x3 form (projections as BF16x3 MFMA GEMMs both ways) are timed beside it;
-|dF| and the gradient's relative L2 error of each mixed form are stated against
-FP64.  The CPU baseline is the FP64 oracle (C++/OpenMP restatement of the
+value = evaluations/s of MDFT_MIXED_X6F, the fastest mixed form that meets
+|dF| <= 0.01 kJ/mol: FFT, rotation and OZ in FP32; angl2proj as a BF16x6 MFMA
+GEMM; proj2angl in FP32; FP64 rotation tables, r-mean mode and reductions; the
+solvent fields xi and df as FP32 copies (the evaluation reads the FP32 copy of
+the FP64 run's xi, so its dF includes that rounding).  MDFT_MIXED_X6 (the same
+arithmetic on FP64 fields), the plain FP64 form and the BF16x3 form
+(projections as BF16x3 MFMA GEMMs both ways) are timed beside it; |dF| and the
+gradient's relative L2 error of each mixed form are stated against FP64.  The CPU baseline is the FP64 oracle (C++/OpenMP restatement of the
 reference) timed on the host cores.
 
 p99_solve: time-to-converged (V_ext on the grid + L-BFGS from xi = 1 until
@@ -50,7 +52,7 @@ N_SIDE, BOX, MMAX, MRSO = 64, 30.0, 5, 2
 TEMP, N0 = 298.15, 0.0332891
 C_SEED, XI_SEED, XI_AMP = 2024, 12345, 0.1
 OZ_FLOP_PER_PAIR = 52344  # rotations, OZ product, phases (both members of the pair)
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "oz_pairs_64c_mmax5_traffic.md")
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "oz_star_64c_mmax5_traffic.md")
 MFMA_SUMMARY = os.path.join(ROOT, "profiles", "a2p_gemm_64c_mmax5_mfma.md")
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16, vendor figure
 A2P_ALGO_FLOP = 4 * 88 * 330 * 64 ** 3  # dense [N x no] . [no x np] complex contraction, one direction
@@ -67,7 +69,7 @@ def stage_bytes(N, no, np_, form):
     """Algorithmic HBM bytes of each stage of one evaluation: every stage reads
     its input once and writes its output once (delta_rho fused into the
     projection load, delta_f into the proj2angl epilogue)."""
-    O = N * no * 8  # one FP64 orientation field
+    O = N * no * (4 if form == "x6f" else 8)  # one orientation field (FP32 copies in MDFT_MIXED_X6F)
     P = N * np_ * (16 if form == "fp64" else 8)  # the projections
     return {
         "angl2proj": O + P,
@@ -474,7 +476,7 @@ def main():
 
     import torch
 
-    from mini_mdft_amd import FP64, MIXED, MIXED_X6, luc
+    from mini_mdft_amd import FP64, MIXED, MIXED_X6, MIXED_X6F, luc
     from mini_mdft_amd import mdft as M
 
     n, L = (N_SIDE,) * 3, (BOX,) * 3
@@ -498,12 +500,14 @@ def main():
     del calib
 
     results = {}
-    for form, prec in (("mixed_x6", MIXED_X6), ("bf16x3", MIXED), ("fp64", FP64)):
+    xi32 = xi.float()  # the FP32 copy of the field that MDFT_MIXED_X6F evaluates
+    for form, prec in (("x6f", MIXED_X6F), ("mixed_x6", MIXED_X6), ("bf16x3", MIXED), ("fp64", FP64)):
         conv = M.Convolution.from_table(tab, g, ag, mp, prec)
         solver = M.Solver(conv, rho0, pref, prec)
-        df = torch.empty_like(xi)
-        ff = solver.energy_cproj(xi, df)
-        dt, stages, oz_timed = timed(solver, xi, df, ffd, args.steps, args.warmup)
+        x = xi32 if prec == MIXED_X6F else xi
+        df = torch.empty_like(x)
+        ff = solver.energy_cproj(x, df)
+        dt, stages, oz_timed = timed(solver, x, df, ffd, args.steps, args.warmup)
         results[form] = dict(ff=ff, df=df, sec=dt, stages=stages, oz_timed=oz_timed, fft=conv.fft_config())
         print(f"[bench] {form}: {args.steps / dt:.1f} evals/s, F={ff:.10g}, stages(ms)="
               f"{ {k: round(v, 4) for k, v in stages.items()} }", file=sys.stderr, flush=True)
@@ -513,18 +517,22 @@ def main():
     nrm64 = torch.linalg.norm(f64["df"])
 
     def err(r):
-        return r["ff"] - f64["ff"], float(torch.linalg.norm(r["df"] - f64["df"]) / nrm64)
+        return r["ff"] - f64["ff"], float(torch.linalg.norm(r["df"].double() - f64["df"]) / nrm64)
 
-    x6, x3 = results["mixed_x6"], results["bf16x3"]
-    dF, grad_rel = err(x6)
+    xf, x6, x3 = results["x6f"], results["mixed_x6"], results["bf16x3"]
+    dF, grad_rel = err(xf)
+    dF6, grad_rel6 = err(x6)
     dF3, grad_rel3 = err(x3)
-    value = args.steps / x6["sec"]
+    value = args.steps / xf["sec"]
     fp64_rate = args.steps / f64["sec"]
+    x6_rate = args.steps / x6["sec"]
 
-    by = stage_bytes(N, no, np_, "mixed")
-    stages = {k: dict(ms=round(ms, 5), GBs=round(by[k] / (ms * 1e-3) / 1e9, 1)) for k, ms in x6["stages"].items()}
+    by = stage_bytes(N, no, np_, "x6f")
+    stages = {k: dict(ms=round(ms, 5), GBs=round(by[k] / (ms * 1e-3) / 1e9, 1)) for k, ms in xf["stages"].items()}
+    by6 = stage_bytes(N, no, np_, "mixed")
+    stages6 = {k: dict(ms=round(ms, 5), GBs=round(by6[k] / (ms * 1e-3) / 1e9, 1)) for k, ms in x6["stages"].items()}
     # a stage above the HBM peak by its algorithmic bytes would mean a wrong byte model
-    over_peak = [k for k, v in stages.items() if v["GBs"] > HBM_PEAK_GBS]
+    over_peak = [k for st in (stages, stages6) for k, v in st.items() if v["GBs"] > HBM_PEAK_GBS]
     if over_peak:
         print(f"[bench] WARNING: stages above the HBM peak by algorithmic bytes: {over_peak}", file=sys.stderr)
     by64 = stage_bytes(N, no, np_, "fp64")
@@ -534,12 +542,12 @@ def main():
     # dominant kernel: the OZ pass (one launch per evaluation); its time is
     # the interval between the start and end events that the launch records
     # on the solver's stream, averaged over the timed region
-    oz_ms = x6["oz_timed"]
+    oz_ms = xf["oz_timed"]
     oz_bytes = by["oz"]
     npairs = 131076
     achieved = oz_bytes / (oz_ms * 1e-3) / 1e9
     oz_tflops = OZ_FLOP_PER_PAIR * npairs / (oz_ms * 1e-3) / 1e12
-    roofline = dict(kernel="oz_pairs_ct_kernel<float,5,2,2>", bound="hbm", achieved=round(achieved, 1),
+    roofline = dict(kernel="oz_star_ct_kernel<float,5,2,384>", bound="hbm", achieved=round(achieved, 1),
                     peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
                     traffic=traffic_from_summary(), algorithmic_bytes=oz_bytes, launch_ms=round(oz_ms, 5),
                     fp32_tflops=round(oz_tflops, 2), fp32_frac=round(oz_tflops / FP32_PEAK_TFLOPS, 4),
@@ -548,11 +556,11 @@ def main():
     # the projection GEMM on the matrix cores: executed BF16 flops per launch
     # from the counter summary over the live angl2proj time of this run
     mflops, mbusy = mfma_from_summary()
-    a2p_ms = x6["stages"]["angl2proj"]
+    a2p_ms = xf["stages"]["angl2proj"]
     roofline_mfma = None
     if mflops:
         ach = mflops / (a2p_ms * 1e-3) / 1e12
-        roofline_mfma = dict(kernel="a2p_gemm_kernel<3,3,12> (BF16x6 angl2proj)", bound="mfma",
+        roofline_mfma = dict(kernel="a2p_gemm_kernel<float,3,3,12> (BF16x6 angl2proj, FP32 fields)", bound="mfma",
                              achieved=round(ach, 1), peak=MFMA_BF16_PEAK_TFLOPS, unit="TFLOP/s",
                              frac=round(ach / MFMA_BF16_PEAK_TFLOPS, 4), busy_frac_counter=mbusy,
                              executed_flop_per_launch=mflops, algorithmic_flop=A2P_ALGO_FLOP,
@@ -600,26 +608,30 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * x6["sec"] / args.steps, 4),
+        "ms_per_step": round(1e3 * xf["sec"] / args.steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (bf16x6 mfma angl2proj, f64 tables and reductions)",
+        "dtype": "f32 (f32 fields, bf16x6 mfma angl2proj, f64 tables, r-mean mode and reductions)",
         "data": "synthetic (seeded xi and c(q); no external files)",
         "config": {"workload": "c-proj F_exc + gradient, 64^3 grid, 30 A box, mmax=5, mrso=2 (330 orientations, "
                                "88 projections, 131076 q-pairs), spce n0, T=298.15 K, synthetic c(q) nmax=5",
-                   "grid": N_SIDE, "box_A": BOX, "mmax": MMAX, "mrso": MRSO, "form": "MDFT_MIXED_X6"},
+                   "grid": N_SIDE, "box_A": BOX, "mmax": MMAX, "mrso": MRSO, "form": "MDFT_MIXED_X6F"},
         "dF_vs_fp64_kJmol": dF,
         "grad_rel_l2_vs_fp64": grad_rel,
         "fp64_evals_per_s": round(fp64_rate, 2),
         "speedup_vs_fp64": round(value / fp64_rate, 3),
         "F_fp64_kJmol": f64["ff"],
+        "mixed_x6_fp64_fields": {"evals_per_s": round(x6_rate, 2), "dF_vs_fp64_kJmol": dF6,
+                                 "grad_rel_l2_vs_fp64": grad_rel6, "speedup_vs_fp64": round(x6_rate / fp64_rate, 3),
+                                 "stages_ms_GBs": stages6,
+                                 "eval_hbm_frac": round(sum(by6.values()) * x6_rate / 1e9 / HBM_PEAK_GBS, 4)},
         "bf16x3": {"evals_per_s": round(args.steps / x3["sec"], 2), "dF_vs_fp64_kJmol": dF3,
                    "grad_rel_l2_vs_fp64": grad_rel3, "meets_dF_bar": abs(dF3) <= 0.01},
         "stages_ms_GBs": stages,
         "stages_fp64_ms_GBs": stages64,
         "fp64_eval_hbm_frac": round(sum(by64.values()) * fp64_rate / 1e9 / HBM_PEAK_GBS, 4),
-        "fft_autotuned": {"mixed_x6": x6["fft"], "fp64": f64["fft"]},
+        "fft_autotuned": {"x6f": xf["fft"], "mixed_x6": x6["fft"], "fp64": f64["fft"]},
         "p99_solve": p99,
         "served_batch": served,
         "served_stream": streamed,

@@ -49,9 +49,17 @@ struct HIP {
 };
 
 // Precision forms of the transforms (mdft_hip.h): Default follows
-// ScalarType (double -> FP64, float -> FP32).  The mixed forms take double
-// orientation fields and complex<float> projections.
-enum class Precision { Default = -1, FP64 = MDFT_FP64, FP32 = MDFT_FP32, Mixed = MDFT_MIXED, MixedX6 = MDFT_MIXED_X6 };
+// ScalarType (double -> FP64, float -> FP32).  The mixed forms take
+// complex<float> projections and double orientation fields, except MixedX6F
+// (float orientation fields).
+enum class Precision {
+  Default = -1,
+  FP64 = MDFT_FP64,
+  FP32 = MDFT_FP32,
+  Mixed = MDFT_MIXED,
+  MixedX6 = MDFT_MIXED_X6,
+  MixedX6F = MDFT_MIXED_X6F
+};
 
 template <typename ScalarType>
 constexpr int precision_code(Precision p) {

@@ -99,7 +99,7 @@ class OrientationProjectionTransform {
  private:
   template <typename OView, typename PView>
   void check_views(const OView& o, const PView& p) const {
-    const bool dbl_o = m_precision != MDFT_FP32, dbl_p = m_precision == MDFT_FP64;
+    const bool dbl_o = m_precision != MDFT_FP32 && m_precision != MDFT_MIXED_X6F, dbl_p = m_precision == MDFT_FP64;
     Impl::Throw_If(sizeof(typename OView::value_type) != (dbl_o ? sizeof(double) : sizeof(float)),
                    "orientation view element type does not match the precision form");
     Impl::Throw_If(sizeof(typename PView::value_type) != (dbl_p ? 2 * sizeof(double) : 2 * sizeof(float)),

@@ -15,7 +15,7 @@
  *   orientation fields o, xi, vexc, df : (nx*ny*nz, ntheta, nphi, npsi) real
  *   projections p                     : (np, nx, ny, nz) complex (re, im)
  *
- * Precision forms (MDFT_FP64 / MDFT_FP32 / MDFT_MIXED) fix the element types:
+ * Precision forms (MDFT_FP64 / MDFT_FP32 / MDFT_MIXED*) fix the element types:
  *   form        o / xi / df   p               arithmetic
  *   MDFT_FP64   double        complex double  FP64 everywhere
  *   MDFT_FP32   float         complex float   FP32 (reference ScalarType=float)
@@ -27,6 +27,11 @@
  *                  three BF16 parts (BF16x6 MFMA: data exact to FP32,
  *                  weights to ~2^-26) and proj2angl runs in FP32 on the
  *                  vector ALUs; |dF| ~1e-3 kJ/mol at 64^3, mmax = 5
+ *   MDFT_MIXED_X6F the arithmetic of MDFT_MIXED_X6 on float fields: o, xi
+ *                  and df are float (FP32 copies of the solvent fields,
+ *                  half their HBM bytes); delta_rho, the get_delta_f products
+ *                  and every reduction stay FP64.  Evaluation only (the
+ *                  functional and the server take double fields)
  */
 #ifndef MDFT_HIP_H
 #define MDFT_HIP_H
@@ -41,6 +46,7 @@ extern "C" {
 #define MDFT_FP32 1
 #define MDFT_MIXED 2
 #define MDFT_MIXED_X6 3
+#define MDFT_MIXED_X6F 4
 
 /* element type codes of the free functions */
 #define MDFT_DOUBLE 0

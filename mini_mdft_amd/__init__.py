@@ -3,7 +3,7 @@
 The compute path is libmdft_hip.so (HIP kernels for gfx950 behind the C ABI of
 include/mdft_hip.h); this package is its Python face.
 """
-from ._lib import DOUBLE, FLOAT, FP32, FP64, MIXED, MIXED_X6, MDFTError  # noqa: F401
+from ._lib import DOUBLE, FLOAT, FP32, FP64, MIXED, MIXED_X6, MIXED_X6F, MDFTError  # noqa: F401
 from .mdft import (  # noqa: F401
     AngularGrid,
     Convolution,

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libmdft_hip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "mdft_hip.h")
 
-FP64, FP32, MIXED, MIXED_X6 = 0, 1, 2, 3
+FP64, FP32, MIXED, MIXED_X6, MIXED_X6F = 0, 1, 2, 3, 4
 DOUBLE, FLOAT = 0, 1
 
 _lib = None

@@ -39,15 +39,20 @@ int guarded(F&& f) {
 
 void check_precision(int precision) {
   throw_if(precision != MDFT_FP64 && precision != MDFT_FP32 && precision != MDFT_MIXED &&
-               precision != MDFT_MIXED_X6,
+               precision != MDFT_MIXED_X6 && precision != MDFT_MIXED_X6F,
            "unknown precision form");
 }
 
-bool is_mixed(int precision) { return precision == MDFT_MIXED || precision == MDFT_MIXED_X6; }
+bool is_mixed(int precision) {
+  return precision == MDFT_MIXED || precision == MDFT_MIXED_X6 || precision == MDFT_MIXED_X6F;
+}
+
+// the forms whose fields (o, xi, df) are float
+bool float_fields(int precision) { return precision == MDFT_FP32 || precision == MDFT_MIXED_X6F; }
 
 std::unique_ptr<ProjGemm> make_gemm(const AngularGrid& g, int precision) {
   if (!is_mixed(precision)) return nullptr;
-  return precision == MDFT_MIXED_X6 ? std::make_unique<ProjGemm>(g, 3, 3, false) : std::make_unique<ProjGemm>(g, 2, 2, true);
+  return precision == MDFT_MIXED ? std::make_unique<ProjGemm>(g, 2, 2, true) : std::make_unique<ProjGemm>(g, 3, 3, false);
 }
 
 void check_dtype(int dtype) { throw_if(dtype != MDFT_DOUBLE && dtype != MDFT_FLOAT, "unknown element type"); }
@@ -256,6 +261,9 @@ int mdft_angl2proj(mdft_opt_t h, const void* o, void* p, void* stream) {
         angl2proj_factorized<float, float, float>(*h->tab, h->N, static_cast<const float*>(o),
                                                   static_cast<cplx<float>*>(p), nullptr, s);
         break;
+      case MDFT_MIXED_X6F:
+        h->gemm->angl2proj(h->N, static_cast<const float*>(o), static_cast<cplx<float>*>(p), nullptr, s);
+        break;
       default:
         h->gemm->angl2proj(h->N, static_cast<const double*>(o), static_cast<cplx<float>*>(p), nullptr, s);
         break;
@@ -275,6 +283,10 @@ int mdft_proj2angl(mdft_opt_t h, const void* p, void* o, void* stream) {
       case MDFT_FP32:
         proj2angl_factorized<float, float, float>(*h->tab, h->N, static_cast<const cplx<float>*>(p),
                                                   static_cast<float*>(o), nullptr, nullptr, 0.0, nullptr, s);
+        break;
+      case MDFT_MIXED_X6F:
+        h->gemm->proj2angl(h->N, static_cast<const cplx<float>*>(p), static_cast<float*>(o), nullptr, nullptr, 0.0,
+                           nullptr, s);
         break;
       default:
         h->gemm->proj2angl(h->N, static_cast<const cplx<float>*>(p), static_cast<double*>(o), nullptr, nullptr,
@@ -553,7 +565,7 @@ int mdft_solver_create_batch(mdft_conv_t conv, double rho0, double prefactor, in
     MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&h->partials), h->nparts * sizeof(double)));
     MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&h->ffdev), capacity * sizeof(double)));
     h->bytes = pbytes + (h->nparts + capacity) * sizeof(double);
-    if (precision == MDFT_MIXED_X6 && mean_mode_supported(h.get())) {
+    if ((precision == MDFT_MIXED_X6 || precision == MDFT_MIXED_X6F) && mean_mode_supported(h.get())) {
       enable_mean_mode(h.get());
       h->mean_on = true;
     }
@@ -609,6 +621,14 @@ static void energy_impl(mdft_solver_t h, int nb, const void* xi, void* df, doubl
       h->conv->dev->execute<float>(p, eps, s, tm, nb);
       proj2angl_factorized<float, float, float>(*h->tab, Nb, p, nullptr, static_cast<const float*>(xi),
                                                 static_cast<float*>(df), rho0, h->partials, s, colp);
+      break;
+    }
+    case MDFT_MIXED_X6F: {
+      auto* p = static_cast<cplx<float>*>(h->pbuf);
+      h->gemm->angl2proj(Nb, static_cast<const float*>(xi), p, &rho0, s);
+      h->conv->dev->execute<float>(p, eps, s, tm, nb);
+      h->gemm->proj2angl(Nb, p, nullptr, static_cast<const float*>(xi), static_cast<float*>(df), rho0, h->partials,
+                         s, colp);
       break;
     }
     default: {
@@ -725,7 +745,8 @@ int mdft_functional_create(mdft_solver_t solver, const double* vext, double kT, 
   return guarded([&] {
     throw_if(!out || !solver || !vext, "functional_create: null argument");
     *out = nullptr;
-    throw_if(solver->precision == MDFT_FP32, "functional: the FP32 form has float fields; use MDFT_FP64 or a mixed form");
+    throw_if(float_fields(solver->precision),
+             "functional: the MDFT_FP32 and MDFT_MIXED_X6F forms have float fields; use MDFT_FP64, MDFT_MIXED or MDFT_MIXED_X6");
     const ConvTables& t = solver->conv->dev->tab;
     FunctionalParams p{solver->N, t.ang.no, solver->rho0, solver->prefactor, kT,
                        t.lx * t.ly * t.lz / static_cast<double>(solver->N)};
@@ -838,7 +859,7 @@ int mdft_server_create(int np_luc, const int* tuples, const double* c, int nq_ta
     throw_if(!out || !tuples || !c, "server_create: null argument");
     *out = nullptr;
     check_precision(precision);
-    throw_if(precision == MDFT_FP32, "server: the FP32 form has float fields; use MDFT_FP64 or a mixed form");
+    throw_if(float_fields(precision), "server: the MDFT_FP32 and MDFT_MIXED_X6F forms have float fields; use MDFT_FP64, MDFT_MIXED or MDFT_MIXED_X6");
     throw_if(np_luc <= 0 || nq_table <= 0 || !(dq > 0) || mrso < 1 || !(n0 > 0) || !(temperature > 0),
              "server_create: bad arguments");
     throw_if(capacity < 1, "server_create: capacity must be at least 1");

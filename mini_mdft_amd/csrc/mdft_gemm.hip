@@ -160,9 +160,11 @@ struct Stage {
 
 // P^T = Wa . delta_rho^T.  Block: kA2pRowsPerWave kWaves consecutive r; wave
 // w: r0 + kA2pRowsPerWave w + (0 .. kA2pRowsPerWave - 1), in groups u of 16.
-// EVEN: no is even, so a row's K pairs are 16-byte aligned double2 loads.
-template <int PX, int PW, int MT, bool FUSE, bool EVEN>
-__global__ __launch_bounds__(kThreads) void a2p_gemm_kernel(const double* __restrict__ o, float2* __restrict__ p,
+// TI: the element type of o / xi (double, or float for the FP32 fields of
+// MDFT_MIXED_X6F; delta_rho is formed in FP64 either way).
+// EVEN: no is even, so a row's K pairs are aligned two-element loads.
+template <typename TI, int PX, int PW, int MT, bool FUSE, bool EVEN>
+__global__ __launch_bounds__(kThreads) void a2p_gemm_kernel(const TI* __restrict__ o, float2* __restrict__ p,
                                                             const u32x4* __restrict__ wsw, GemmDims d, double rho0) {
   using St = Stage<PW, MT>;
   __shared__ u32x4 lds[St::kPer * kThreads];
@@ -176,7 +178,7 @@ __global__ __launch_bounds__(kThreads) void a2p_gemm_kernel(const double* __rest
     for (int u = 0; u < kA2pGroups; ++u) acc[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
 
   // rows r0 + 16 u + lane%16 (clamped), k = 32 s + 8 (lane/16) + (0..7) (clamped)
-  const double* rowp[kA2pGroups];
+  const TI* rowp[kA2pGroups];
 #pragma unroll
   for (int u = 0; u < kA2pGroups; ++u) {
     const long r = r0 + 16 * u + (lane & 15);
@@ -192,13 +194,15 @@ __global__ __launch_bounds__(kThreads) void a2p_gemm_kernel(const double* __rest
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int k = min(k0 + 8 * q, d.no - 2);
-          const double2 v = *reinterpret_cast<const double2*>(rowp[u] + k);
-          raw[u][2 * q] = v.x;
-          raw[u][2 * q + 1] = v.y;
+          using V2 = typename cvec<TI>::type;
+          const V2 v = *reinterpret_cast<const V2*>(rowp[u] + k);
+          raw[u][2 * q] = static_cast<double>(v.x);
+          raw[u][2 * q + 1] = static_cast<double>(v.y);
         }
       } else {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) raw[u][j] = rowp[u][min(k0 + 8 * (j >> 1) + (j & 1), d.no - 1)];
+        for (int j = 0; j < 8; ++j)
+          raw[u][j] = static_cast<double>(rowp[u][min(k0 + 8 * (j >> 1) + (j & 1), d.no - 1)]);
       }
     }
   };
@@ -505,11 +509,12 @@ ProjGemm::ProjGemm(const AngularGrid& g, int data_parts, int weight_parts, bool 
 
 ProjGemm::~ProjGemm() = default;
 
-void ProjGemm::angl2proj(long N, const double* o, cplx<float>* p, const double* rho0, hipStream_t s) {
+template <typename TI>
+void ProjGemm::angl2proj_t(long N, const TI* o, cplx<float>* p, const double* rho0, hipStream_t s) {
   if (N == 0) return;
   const Impl& I = *impl_;
   if (!I.a2p_gemm()) {
-    angl2proj_factorized<double, float, float>(*I.fb, N, o, p, rho0, s);
+    angl2proj_factorized<TI, float, float>(*I.fb, N, o, p, rho0, s);
     return;
   }
   const GemmDims d{N, I.no, I.np, I.a_kt};
@@ -526,12 +531,23 @@ void ProjGemm::angl2proj(long N, const double* o, cplx<float>* p, const double* 
     dispatch_tiles<kMaxA2pTiles>(I.a_mt, [&](auto mt) {
       constexpr int MT = decltype(mt)::value;
       if (rho0)
-        even ? launch(a2p_gemm_kernel<PX, PW, MT, true, true>) : launch(a2p_gemm_kernel<PX, PW, MT, true, false>);
+        even ? launch(a2p_gemm_kernel<TI, PX, PW, MT, true, true>)
+             : launch(a2p_gemm_kernel<TI, PX, PW, MT, true, false>);
       else
-        even ? launch(a2p_gemm_kernel<PX, PW, MT, false, true>) : launch(a2p_gemm_kernel<PX, PW, MT, false, false>);
+        even ? launch(a2p_gemm_kernel<TI, PX, PW, MT, false, true>)
+             : launch(a2p_gemm_kernel<TI, PX, PW, MT, false, false>);
     });
   });
   MDFT_HIP_CHECK(hipGetLastError());
+}
+
+void ProjGemm::angl2proj(long N, const double* o, cplx<float>* p, const double* rho0, hipStream_t s) {
+  angl2proj_t<double>(N, o, p, rho0, s);
+}
+
+void ProjGemm::angl2proj(long N, const float* o, cplx<float>* p, const double* rho0, hipStream_t s) {
+  throw_if(impl_->px != 3, "angl2proj: float fields take the BF16x6 split (data exact to FP32)");
+  angl2proj_t<float>(N, o, p, rho0, s);
 }
 
 int ProjGemm::p2a_blocks(long N) const {
@@ -559,6 +575,16 @@ int ProjGemm::proj2angl(long N, const cplx<float>* p, double* o, const double* x
   });
   MDFT_HIP_CHECK(hipGetLastError());
   return nblk;
+}
+
+int ProjGemm::proj2angl(long N, const cplx<float>* p, float* o, const float* xi, float* df, double rho0,
+                        double* partials, hipStream_t s, double* colpart) {
+  if (N == 0) return 0;
+  const Impl& I = *impl_;
+  throw_if(I.p2a_gemm(), "proj2angl: float fields take the FP32 VALU kernel");
+  throw_if(colpart != nullptr && !has_column_sums(), "proj2angl: no column sums in this form");
+  proj2angl_factorized<float, float, float>(*I.fb, N, p, o, xi, df, rho0, partials, s, colpart);
+  return proj2angl_blocks(*I.fb, N);
 }
 
 }  // namespace mdft

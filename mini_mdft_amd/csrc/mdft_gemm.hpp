@@ -27,18 +27,25 @@ struct ProjGemm {
   ProjGemm(const ProjGemm&) = delete;
   ProjGemm& operator=(const ProjGemm&) = delete;
 
-  // o (or xi when rho0 != nullptr, delta_rho fused) -> p (np, N) complex float
+  // o (or xi when rho0 != nullptr, delta_rho fused) -> p (np, N) complex float;
+  // float o / xi (MDFT_MIXED_X6F) needs the (3, 3) split
   void angl2proj(long N, const double* o, cplx<float>* p, const double* rho0, hipStream_t s);
+  void angl2proj(long N, const float* o, cplx<float>* p, const double* rho0, hipStream_t s);
   // p -> o; with xi != nullptr get_delta_f runs in the epilogue instead of
   // storing o.  Returns the number of FP64 partials written.
   // colpart: per-row-block column sums (proj2angl_factorized), FP32 VALU
   // proj2angl on compile-time grids only (has_column_sums)
   int proj2angl(long N, const cplx<float>* p, double* o, const double* xi, double* df, double rho0,
                 double* partials, hipStream_t s, double* colpart = nullptr);
+  // float fields (MDFT_MIXED_X6F): the FP32 VALU kernel (p2a_mfma off)
+  int proj2angl(long N, const cplx<float>* p, float* o, const float* xi, float* df, double rho0,
+                double* partials, hipStream_t s, double* colpart = nullptr);
   int p2a_blocks(long N) const;
   bool has_column_sums() const;
 
  private:
+  template <typename TI>
+  void angl2proj_t(long N, const TI* o, cplx<float>* p, const double* rho0, hipStream_t s);
   struct Impl;
   std::unique_ptr<Impl> impl_;
 };

@@ -14,7 +14,7 @@ import math
 import numpy as np
 
 from . import _lib
-from ._lib import DOUBLE, FLOAT, FP32, FP64, MIXED, MIXED_X6, MDFTError, check, lib, ptr, stream_handle
+from ._lib import DOUBLE, FLOAT, FP32, FP64, MIXED, MIXED_X6, MIXED_X6F, MDFTError, check, lib, ptr, stream_handle
 
 # common/MDFT_Constants.hpp:25-56
 BOLTZ = 1.38064852e-23
@@ -57,7 +57,7 @@ def _dtypes(precision):
     torch = _torch()
     if precision == FP64:
         return torch.float64, torch.complex128
-    if precision == FP32:
+    if precision in (FP32, MIXED_X6F):
         return torch.float32, torch.complex64
     if precision in (MIXED, MIXED_X6):
         return torch.float64, torch.complex64

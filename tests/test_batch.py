@@ -3,7 +3,7 @@ every stage, and each field's ff and df are bitwise the ones it gets alone."""
 import numpy as np
 import pytest
 
-from mini_mdft_amd import FP32, FP64, MIXED, MIXED_X6, luc
+from mini_mdft_amd import FP32, FP64, MIXED, MIXED_X6, MIXED_X6F, luc
 from mini_mdft_amd import mdft as M
 
 RHO0, PREF = M.rho0_of(0.0332891, 2), M.prefactor_of(298.15, 2, 0.0332891)
@@ -17,14 +17,14 @@ def _conv(n, L, mmax, prec):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec", [FP64, MIXED_X6, MIXED, FP32])
+@pytest.mark.parametrize("prec", [FP64, MIXED_X6, MIXED, FP32, MIXED_X6F])
 @pytest.mark.parametrize("n,L,mmax", [((8, 8, 8), (9.0,) * 3, 3), ((16, 8, 4), (12.0, 7.0, 5.0), 5),
                                       ((12, 12, 12), (10.0,) * 3, 2)])
 def test_batch_equals_alone_bitwise(prec, n, L, mmax):
     import torch
 
     sg, ag, conv = _conv(n, L, mmax, prec)
-    dt = torch.float32 if prec == FP32 else torch.float64
+    dt = torch.float32 if prec in (FP32, MIXED_X6F) else torch.float64
     nb, shape = 3, (sg.N, ag.m_no)
     xi = torch.stack([torch.from_numpy(1.0 + 0.1 * np.random.default_rng(s).uniform(-1, 1, shape)) for s in range(nb)])
     xi = xi.to(dt).cuda().contiguous()

@@ -25,7 +25,7 @@ import pytest
 
 import configs as C
 from conftest import rel_err
-from mini_mdft_amd import FP64, MIXED, MIXED_X6, luc, rng
+from mini_mdft_amd import FP64, MIXED, MIXED_X6, MIXED_X6F, luc, rng
 from mini_mdft_amd import mdft as M
 
 pytestmark = pytest.mark.gpu
@@ -55,10 +55,12 @@ def _eval(cfg, prec, tab, xi):
     g, ag, mp = _grids(cfg)
     conv = M.Convolution.from_table(tab, g, ag, mp, FP64 if prec == FP64 else M.FP32)
     solver = M.Solver(conv, C.rho0(), C.prefactor(), prec)
-    df = torch.empty_like(xi)
-    ff = solver.energy_cproj(xi, df)
+    # MDFT_MIXED_X6F evaluates the FP32 copy of the same field
+    x = xi.float() if prec == MIXED_X6F else xi
+    df = torch.empty_like(x)
+    ff = solver.energy_cproj(x, df)
     del solver, conv
-    return ff, df
+    return ff, df.double()
 
 
 # ---------------------------------------------------------------------------
@@ -147,11 +149,14 @@ def _mixed_vs_fp64(cfg, prec):
     return ff64, dF, grad
 
 
-def test_config2_mixed_x6_within_the_bar():
+@pytest.mark.parametrize("prec", [MIXED_X6, MIXED_X6F], ids=["x6", "x6f"])
+def test_config2_mixed_x6_within_the_bar(prec):
+    """MDFT_MIXED_X6F evaluates the FP32 copy of the FP64 run's field: its dF
+    includes the rounding of xi to FP32."""
     cfg = C.CONFIG1
-    _say(f"config2 MIXED_X6 vs FP64 evaluation {cfg}")
-    ff64, dF, grad = _mixed_vs_fp64(cfg, MIXED_X6)
-    print(f"config2 MIXED_X6: F_fp64 {ff64!r} dF {dF!r} kJ/mol, grad rel-L2 {grad:.3e}", flush=True)
+    _say(f"config2 form {prec} vs FP64 evaluation {cfg}")
+    ff64, dF, grad = _mixed_vs_fp64(cfg, prec)
+    print(f"config2 form {prec}: F_fp64 {ff64!r} dF {dF!r} kJ/mol, grad rel-L2 {grad:.3e}", flush=True)
     assert abs(dF) <= 0.01
     assert grad <= 1e-5
 
@@ -167,12 +172,13 @@ def test_config2_bf16x3_error_is_stated():
     assert abs(dF) <= 0.5 and grad <= 1e-4
 
 
+@pytest.mark.parametrize("prec", [MIXED_X6, MIXED_X6F], ids=["x6", "x6f"])
 @pytest.mark.parametrize("side", [80, 96])
-def test_large_grid_mixed_x6_within_the_bar(side):
+def test_large_grid_mixed_x6_within_the_bar(side, prec):
     cfg = C.large(side)
-    _say(f"config3/4 MIXED_X6 vs FP64 evaluation {cfg}")
-    ff64, dF, grad = _mixed_vs_fp64(cfg, MIXED_X6)
-    print(f"{side}^3 MIXED_X6: F_fp64 {ff64!r} dF {dF!r} kJ/mol, grad rel-L2 {grad:.3e}", flush=True)
+    _say(f"config3/4 form {prec} vs FP64 evaluation {cfg}")
+    ff64, dF, grad = _mixed_vs_fp64(cfg, prec)
+    print(f"{side}^3 form {prec}: F_fp64 {ff64!r} dF {dF!r} kJ/mol, grad rel-L2 {grad:.3e}", flush=True)
     assert abs(dF) <= 0.01
     assert grad <= 1e-5
 

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from conftest import rel_err
-from mini_mdft_amd import FP32, FP64, MIXED, MIXED_X6, luc, rng
+from mini_mdft_amd import FP32, FP64, MIXED, MIXED_X6, MIXED_X6F, luc, rng
 from mini_mdft_amd import mdft as M
 
 pytestmark = pytest.mark.gpu
@@ -67,7 +67,8 @@ CASES = [((2, 3, 4), 3, 2), ((2, 2, 3), 5, 2), ((3, 3, 3), 1, 2), ((2, 2, 2), 10
 
 
 @pytest.mark.parametrize("n,mmax,mrso", CASES)
-@pytest.mark.parametrize("prec,tol", [(FP64, 1e-13), (FP32, 2e-6), (MIXED, 5e-5), (MIXED_X6, 5e-5)])
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-13), (FP32, 2e-6), (MIXED, 5e-5), (MIXED_X6, 5e-5),
+                                      (MIXED_X6F, 5e-5)])
 def test_angl2proj(oracle, n, mmax, mrso, prec, tol):
     g = M.SpatialGrid(n, (1.0, 1.0, 1.0))
     ag = M.AngularGrid(mmax, mrso)
@@ -82,7 +83,8 @@ def test_angl2proj(oracle, n, mmax, mrso, prec, tol):
 
 
 @pytest.mark.parametrize("n,mmax,mrso", CASES)
-@pytest.mark.parametrize("prec,tol", [(FP64, 1e-13), (FP32, 2e-6), (MIXED, 5e-5), (MIXED_X6, 5e-5)])
+@pytest.mark.parametrize("prec,tol", [(FP64, 1e-13), (FP32, 2e-6), (MIXED, 5e-5), (MIXED_X6, 5e-5),
+                                      (MIXED_X6F, 2e-6)])
 def test_proj2angl(oracle, n, mmax, mrso, prec, tol):
     g = M.SpatialGrid(n, (1.0, 1.0, 1.0))
     ag = M.AngularGrid(mmax, mrso)
@@ -207,7 +209,8 @@ def test_energy_fp64_parity(oracle, n, L, mmax):
     assert solver.energy_cproj(xt, df2) == ff and torch.equal(df, df2)
 
 
-@pytest.mark.parametrize("prec,tol,tol_df", [(FP32, 1e-3, 1e-3), (MIXED, 1e-3, 1e-3), (MIXED_X6, 1e-5, 1e-5)])
+@pytest.mark.parametrize("prec,tol,tol_df", [(FP32, 1e-3, 1e-3), (MIXED, 1e-3, 1e-3), (MIXED_X6, 1e-5, 1e-5),
+                                             (MIXED_X6F, 1e-5, 1e-5)])
 def test_energy_reduced_precision(oracle, prec, tol, tol_df):
     """FP32 chops gamma at FLT_EPSILON like the reference's float build;
     MIXED keeps the FP64 threshold.  MDFT_MIXED_X6 is held to 1e-5 of |ff|
@@ -222,8 +225,7 @@ def test_energy_reduced_precision(oracle, prec, tol, tol_df):
     solver = M.Solver(conv, rho0, pref, prec)
     xi = rng.xi_field(12345, (g.N, ag.m_no))
     ff_ref, df_ref = ocv.energy_cproj(xi, rho0, pref)
-    dt = torch.float32 if prec == FP32 else torch.float64
-    xt = _to(xi, dt)
+    xt = _to(xi, solver.real_dtype)
     df = torch.empty_like(xt)
     ff = solver.energy_cproj(xt, df)
     print(f"{prec}: ff {ff!r} oracle {ff_ref!r} rel {abs(ff - ff_ref) / abs(ff_ref):.2e}", flush=True)
