@@ -45,6 +45,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 #include <vector>
 
 namespace mdft {
@@ -135,6 +136,33 @@ __device__ inline void pack8(const float (&v)[8], u32x4 (&out)[PX]) {
                    b[q][6] | (b[q][7] << 16)};
 }
 
+// pack 8 floats into three BF16 fragments by truncation: hi = the top 16
+// bits of x (sign, exponent, 8 significant bits), mid = the top 16 bits of
+// x - hi, lo = x - hi - mid.  Both subtractions are exact, and an FP32
+// significand has 24 bits, so lo has at most 8 significant bits and
+// x = hi + mid + lo exactly: the same value as the nearest-BF16 split of
+// pack8<3>, for about a third of its VALU work (no rounding arithmetic;
+// v_perm_b32 joins the halves).
+__device__ inline void pack8_exact3(const float (&v)[8], u32x4 (&out)[3]) {
+  uint32_t h[8], m[8], l[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const uint32_t u = __builtin_bit_cast(uint32_t, v[j]);
+    h[j] = u & 0xffff0000u;
+    const float r1 = v[j] - __builtin_bit_cast(float, h[j]);
+    m[j] = __builtin_bit_cast(uint32_t, r1) & 0xffff0000u;
+    l[j] = __builtin_bit_cast(uint32_t, r1 - __builtin_bit_cast(float, m[j]));
+  }
+  // result = (high half of the odd element) << 16 | (high half of the even one)
+  constexpr uint32_t kSel = 0x07060302u;
+  out[0] = u32x4{__builtin_amdgcn_perm(h[1], h[0], kSel), __builtin_amdgcn_perm(h[3], h[2], kSel),
+                 __builtin_amdgcn_perm(h[5], h[4], kSel), __builtin_amdgcn_perm(h[7], h[6], kSel)};
+  out[1] = u32x4{__builtin_amdgcn_perm(m[1], m[0], kSel), __builtin_amdgcn_perm(m[3], m[2], kSel),
+                 __builtin_amdgcn_perm(m[5], m[4], kSel), __builtin_amdgcn_perm(m[7], m[6], kSel)};
+  out[2] = u32x4{__builtin_amdgcn_perm(l[1], l[0], kSel), __builtin_amdgcn_perm(l[3], l[2], kSel),
+                 __builtin_amdgcn_perm(l[5], l[4], kSel), __builtin_amdgcn_perm(l[7], l[6], kSel)};
+}
+
 struct GemmDims {
   long N;
   int no, np, kt;  // kt: k-steps of 32
@@ -184,7 +212,7 @@ __global__ __launch_bounds__(kThreads) void a2p_gemm_kernel(const TI* __restrict
     const long r = r0 + 16 * u + (lane & 15);
     rowp[u] = o + (r < d.N ? r : d.N - 1) * d.no;
   }
-  auto load_data = [&](int s, double (&raw)[kA2pGroups][8]) {
+  auto load_data = [&](int s, TI (&raw)[kA2pGroups][8]) {
     // operand slot j of lane group g = lane/16 holds k = 32 s + 8 (j/2) + 2 g + j%2
     // (a2p_k_of): load q of the four lanes of a row then reads 64 contiguous bytes
     const int k0 = s * kK + 2 * (lane >> 4);
@@ -196,32 +224,52 @@ __global__ __launch_bounds__(kThreads) void a2p_gemm_kernel(const TI* __restrict
           const int k = min(k0 + 8 * q, d.no - 2);
           using V2 = typename cvec<TI>::type;
           const V2 v = *reinterpret_cast<const V2*>(rowp[u] + k);
-          raw[u][2 * q] = static_cast<double>(v.x);
-          raw[u][2 * q + 1] = static_cast<double>(v.y);
+          raw[u][2 * q] = v.x;
+          raw[u][2 * q + 1] = v.y;
         }
       } else {
 #pragma unroll
-        for (int j = 0; j < 8; ++j)
-          raw[u][j] = static_cast<double>(rowp[u][min(k0 + 8 * (j >> 1) + (j & 1), d.no - 1)]);
+        for (int j = 0; j < 8; ++j) raw[u][j] = rowp[u][min(k0 + 8 * (j >> 1) + (j & 1), d.no - 1)];
       }
     }
   };
-  auto pack_data = [&](const double (&raw)[kA2pGroups][8], u32x4 (&x)[kA2pGroups][PX]) {
+  // delta_rho = rho0 (xi^2 - 1): in FP64 for double xi; for float xi as
+  // fma(xi, xi, -1) (one rounding) times rho0 split in two floats (rho0 to
+  // ~2^-48: rounding rho0 to FP32 would scale every projection alike)
+  const float rho0_hi = static_cast<float>(rho0);
+  const float rho0_lo = static_cast<float>(rho0 - static_cast<double>(rho0_hi));
+  auto pack_data = [&](const TI (&raw)[kA2pGroups][8], u32x4 (&x)[kA2pGroups][PX]) {
 #pragma unroll
     for (int u = 0; u < kA2pGroups; ++u) {
       float v[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        double val = raw[u][j];
-        if (FUSE) val = rho0 * (val * val - 1.0);
-        v[j] = static_cast<float>(val);
+        if constexpr (std::is_same<TI, float>::value) {
+          const float xv = raw[u][j];
+          if (FUSE) {
+            const float t = fmaf(xv, xv, -1.0f);
+            v[j] = fmaf(t, rho0_hi, t * rho0_lo);
+          } else {
+            v[j] = xv;
+          }
+        } else {
+          double val = raw[u][j];
+          if (FUSE) val = rho0 * (val * val - 1.0);
+          v[j] = static_cast<float>(val);
+        }
       }
-      pack8<PX>(v, x[u]);
+      // float fields: the truncation split (their delta_rho costs no FP64
+      // work, so the split is most of the VALU work); double fields keep the
+      // nearest-BF16 split (no faster with the other, their FP64 work dominates)
+      if constexpr (PX == 3 && std::is_same<TI, float>::value)
+        pack8_exact3(v, x[u]);
+      else
+        pack8<PX>(v, x[u]);
     }
   };
 
   St st;
-  double raw[kA2pGroups][8];
+  TI raw[kA2pGroups][8];
   u32x4 x[kA2pGroups][PX];
   st.load(wsw, tid);
   load_data(0, raw);
