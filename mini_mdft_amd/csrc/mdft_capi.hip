@@ -752,7 +752,7 @@ int mdft_functional_create(mdft_solver_t solver, const double* vext, double kT, 
                        t.lx * t.ly * t.lz / static_cast<double>(solver->N)};
     auto h = std::make_unique<mdft_functional_s>();
     h->solver = solver;
-    // the mixed forms keep their L-BFGS pairs in FP32
+    // the mixed forms keep their L-BFGS pairs and gradients in FP32
     h->f = std::make_unique<Functional>(
         p, t.ang.w, vext,
         [solver](const double* xi, double* df, double* ffd, hipStream_t s) { energy_impl(solver, 1, xi, df, ffd, 1, s); },
@@ -840,7 +840,7 @@ struct mdft_server_s {
     const AngularGrid ag = make_angular_grid(sh.mmax, mrso);
     const size_t N = static_cast<size_t>(sh.nx) * sh.ny * sh.nz, n = N * ag.no;
     const size_t ts = precision == MDFT_FP64 ? 8 : 4;
-    const size_t job = 4 * n * 8 + 2 * static_cast<size_t>(opt.m + 1) * n * ts;
+    const size_t job = 2 * n * 8 + 2 * static_cast<size_t>(opt.m + 2) * n * ts;  // Engine::job_bytes
     const size_t proj = N * ag.np * capacity * (precision == MDFT_FP64 ? 16 : 8);
     const size_t dtab = (N / 2 + 8) * static_cast<size_t>((sh.mmax + 1) * (sh.mmax + 1)) * (2 * sh.mmax + 1) * ts;
     const size_t slabs = 2 * static_cast<size_t>(capacity) * n * 8 + n * ts;

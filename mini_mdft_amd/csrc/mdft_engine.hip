@@ -58,8 +58,8 @@ struct SlotDesc {
   const double* src;  // current point (start: the initial field)
   double* save;       // copy of the current point, or null
   const double* V;
-  double* g;   // g at the current point (written at the start point)
-  double* gt;  // g at the trial
+  TS* g;   // g at the current point (written at the start point)
+  TS* gt;  // g at the trial
   const TS* s[kLbfgsMaxPairs];
   const TS* y[kLbfgsMaxPairs];
   TS* sc;
@@ -80,7 +80,7 @@ __global__ __launch_bounds__(kFThreads) void slot_direction_kernel(const SlotDes
     return;
   }
   MDFT_ELEMENT_LOOP(e) {
-    double q = a.cg * a.g[i];
+    double q = a.cg * static_cast<double>(a.g[i]);
 #pragma unroll
     for (int j = 0; j < KM; ++j) q = fma(a.cq[j], static_cast<double>(a.y[j][i]), q);
     double r = a.gamma * e.h0[io] * q;
@@ -116,8 +116,9 @@ __global__ __launch_bounds__(kFThreads) void slot_update_kernel(const SlotDesc<T
     MDFT_ELEMENT_LOOP(e) {
       const double xv = xt[i], v = a.V[i], wio = e.w[io], h = e.h0[io];
       local_f(xv, v, wio, acc[5 * KM + 7], acc[5 * KM + 8]);
-      const double gv = local_grad(xv, v, df[i], wio, e);
-      a.g[i] = gv;
+      const TS gs = static_cast<TS>(local_grad(xv, v, df[i], wio, e));
+      a.g[i] = gs;
+      const double gv = static_cast<double>(gs);
       const double hg = h * gv;
       acc[5 * KM + 6] += gv * hg;
       acc[5 * KM + 9] = fmax(acc[5 * KM + 9], fabs(hg));
@@ -127,9 +128,10 @@ __global__ __launch_bounds__(kFThreads) void slot_update_kernel(const SlotDesc<T
       const double wio = e.w[io], h = e.h0[io];
       const double xv = xt[i], v = a.V[i];
       local_f(xv, v, wio, acc[5 * KM + 7], acc[5 * KM + 8]);
-      const double gn = local_grad(xv, v, df[i], wio, e);
-      const TS ys = static_cast<TS>(gn - a.g[i]);
-      a.gt[i] = gn;
+      const TS gs = static_cast<TS>(local_grad(xv, v, df[i], wio, e));
+      const double gn = static_cast<double>(gs);
+      const TS ys = static_cast<TS>(gn - static_cast<double>(a.g[i]));
+      a.gt[i] = gs;
       a.yn[i] = ys;
       const double yv = static_cast<double>(ys);
       const double scv = static_cast<double>(a.sc[i]);
@@ -204,10 +206,10 @@ struct Engine::Slot {
   bool has_trial = false;
   double* x = nullptr;  // current point when not in the slab
   double* V = nullptr;
-  double* g = nullptr;
-  double* gt = nullptr;
+  void* g = nullptr;      // TS
+  void* gt = nullptr;     // TS
   void* pairs = nullptr;  // 2 (m + 1) fields of TS
-  void* mem = nullptr;    // one allocation: x | V | g | gt | pairs
+  void* mem = nullptr;    // one allocation: x | V (FP64) | g | gt | pairs (TS)
   double* out = nullptr;  // where the final point goes (device), or null
   explicit Slot(const LbfgsOptions& o) : st(o) {}
 };
@@ -293,7 +295,7 @@ int Engine::active() const {
 
 size_t Engine::job_bytes() const {
   const size_t ts = fp32_ ? sizeof(float) : sizeof(double);
-  return 4 * static_cast<size_t>(n_) * sizeof(double) + 2 * static_cast<size_t>(o_.m + 1) * n_ * ts;
+  return 2 * static_cast<size_t>(n_) * sizeof(double) + 2 * static_cast<size_t>(o_.m + 2) * n_ * ts;
 }
 
 int Engine::admit(long long id, double* out_xi) {
@@ -305,11 +307,12 @@ int Engine::admit(long long id, double* out_xi) {
   sl->out = out_xi;
   MDFT_HIP_CHECK(hipMalloc(&sl->mem, job_bytes()));
   double* base = static_cast<double*>(sl->mem);
+  const size_t ts = fp32_ ? sizeof(float) : sizeof(double);
   sl->x = base;
   sl->V = base + n_;
   sl->g = base + 2 * n_;
-  sl->gt = base + 3 * n_;
-  sl->pairs = base + 4 * n_;
+  sl->gt = reinterpret_cast<char*>(sl->g) + n_ * ts;
+  sl->pairs = reinterpret_cast<char*>(sl->gt) + n_ * ts;
   fill(sl->x, n_, 1.0, stream_);  // the start point xi = 1 (Solvent::init_density)
   slots_[j] = std::move(sl);
   return j;
@@ -371,7 +374,7 @@ void Engine::enqueue(int nbatch) {
 namespace {
 template <typename TS>
 void fill_desc(SlotDesc<TS>& d, int mode, const LbfgsState::Trial* tr, const std::vector<int>* order,
-               const double* src, double* save, const double* V, double* g, double* gt, TS* pairs, long n,
+               const double* src, double* save, const double* V, TS* g, TS* gt, TS* pairs, long n,
                const TS* zeros, int m) {
   d = SlotDesc<TS>();
   d.mode = mode;
@@ -427,11 +430,13 @@ void Engine::begin_tick() {
     const std::vector<int>* order = sl ? &sl->st.order() : nullptr;
     if (fp32_)
       fill_desc(static_cast<SlotDesc<float>*>(host_desc_)[j], mode, tr, order, src, save, sl ? sl->V : nullptr,
-                sl ? sl->g : nullptr, sl ? sl->gt : nullptr, sl ? static_cast<float*>(sl->pairs) : nullptr, n_,
+                sl ? static_cast<float*>(sl->g) : nullptr, sl ? static_cast<float*>(sl->gt) : nullptr,
+                sl ? static_cast<float*>(sl->pairs) : nullptr, n_,
                 static_cast<const float*>(zeros_), o_.m);
     else
       fill_desc(static_cast<SlotDesc<double>*>(host_desc_)[j], mode, tr, order, src, save, sl ? sl->V : nullptr,
-                sl ? sl->g : nullptr, sl ? sl->gt : nullptr, sl ? static_cast<double*>(sl->pairs) : nullptr, n_,
+                sl ? static_cast<double*>(sl->g) : nullptr, sl ? static_cast<double*>(sl->gt) : nullptr,
+                sl ? static_cast<double*>(sl->pairs) : nullptr, n_,
                 static_cast<const double*>(zeros_), o_.m);
   }
   const size_t dsz = fp32_ ? sizeof(SlotDesc<float>) : sizeof(SlotDesc<double>);

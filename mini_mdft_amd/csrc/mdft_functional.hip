@@ -38,16 +38,19 @@ namespace {
 
 using namespace elem;
 
-// g, F_id and F_ext sums, g'H g and max |h0 g| at x
+// g, F_id and F_ext sums, g'H g and max |h0 g| at x; g is stored as TG
+// (the sums take the stored value)
+template <typename TG>
 __global__ __launch_bounds__(kFThreads) void local_kernel(const double* __restrict__ x, const double* __restrict__ V,
-                                                          const double* __restrict__ df, double* __restrict__ g,
+                                                          const double* __restrict__ df, TG* __restrict__ g,
                                                           Elem e, double* __restrict__ part, int nb) {
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   MDFT_ELEMENT_LOOP(e) {
     const double xv = x[i], v = V[i], wio = e.w[io], h = e.h0[io];
     local_f(xv, v, wio, acc[0], acc[1]);
-    const double gv = local_grad(xv, v, df[i], wio, e);
-    g[i] = gv;
+    const TG gs = static_cast<TG>(local_grad(xv, v, df[i], wio, e));
+    g[i] = gs;
+    const double gv = static_cast<double>(gs);
     const double hg = h * gv;
     acc[2] += gv * hg;
     acc[3] = fmax(acc[3], fabs(hg));
@@ -55,13 +58,14 @@ __global__ __launch_bounds__(kFThreads) void local_kernel(const double* __restri
   store_partials<4, true>(acc, part, nb);
 }
 
-// The stored pairs s_j, y_j and the candidate pair are TS (double, or float
-// for the mixed forms, whose pairs then take half the traffic); every dot
-// product and the direction are formed in FP64 from the stored values.
+// The stored pairs s_j, y_j, the candidate pair and the gradients g, g_t are
+// TS (double, or float for the mixed forms, which then move half their
+// bytes); every dot product and the direction are formed in FP64 from the
+// stored values.
 template <typename TS>
 struct DirArgs {
   const double* x;
-  const double* g;
+  const TS* g;
   const TS* s[kLbfgsMaxPairs];
   const TS* y[kLbfgsMaxPairs];
   double cs[kLbfgsMaxPairs], cq[kLbfgsMaxPairs];
@@ -74,7 +78,7 @@ struct DirArgs {
 template <typename TS, int K>
 __global__ __launch_bounds__(kFThreads) void direction_kernel(DirArgs<TS> a, Elem e) {
   MDFT_ELEMENT_LOOP(e) {
-    double q = a.cg * a.g[i];
+    double q = a.cg * static_cast<double>(a.g[i]);
 #pragma unroll
     for (int j = 0; j < K; ++j) q = fma(a.cq[j], static_cast<double>(a.y[j][i]), q);
     double r = a.gamma * e.h0[io] * q;
@@ -92,8 +96,8 @@ struct UpdArgs {
   const double* xt;
   const double* V;
   const double* df;
-  const double* g;  // g at x
-  double* gt;       // out: g at xt
+  const TS* g;  // g at x
+  TS* gt;       // out: g at xt
   const TS* sc;
   TS* yn;  // out: g at xt - g at x
   const TS* s[kLbfgsMaxPairs];
@@ -106,7 +110,7 @@ struct UpdArgs {
 //   [2K+3, 3K+3) s_j.g_t | 3K+3 sc.g_t | [3K+4, 4K+4) yn'H y_j | 4K+4 yn'H yn |
 //   [4K+5, 5K+5) y_j'H g_t | 5K+5 yn'H g_t | 5K+6 g_t'H g_t | 5K+7 F_id sum |
 //   5K+8 F_ext sum | 5K+9 max |h0 g_t|
-// yn enters the sums as stored (rounded to TS).
+// g_t and yn enter the sums as stored (rounded to TS).
 template <typename TS, int K>
 __global__ __launch_bounds__(kFThreads) void update_kernel(UpdArgs<TS> a, Elem e, double* __restrict__ part,
                                                            int nb) {
@@ -119,9 +123,10 @@ __global__ __launch_bounds__(kFThreads) void update_kernel(UpdArgs<TS> a, Elem e
     const double wio = e.w[io], h = e.h0[io];
     const double xv = a.xt[i], v = a.V[i];
     local_f(xv, v, wio, acc[5 * K + 7], acc[5 * K + 8]);
-    const double gn = local_grad(xv, v, a.df[i], wio, e);
-    const TS ys = static_cast<TS>(gn - a.g[i]);
-    a.gt[i] = gn;
+    const TS gs = static_cast<TS>(local_grad(xv, v, a.df[i], wio, e));
+    const double gn = static_cast<double>(gs);
+    const TS ys = static_cast<TS>(gn - static_cast<double>(a.g[i]));
+    a.gt[i] = gs;
     a.yn[i] = ys;
     const double yv = static_cast<double>(ys);
     const double scv = static_cast<double>(a.sc[i]);
@@ -369,9 +374,9 @@ size_t Functional::bytes() const {
 }
 
 size_t Functional::minimize_bytes(int m) const {
-  // the trial point and two gradients in FP64, m + 1 slots of s and y
+  // the trial point in FP64, two gradients and m + 1 slots of s and y in TS
   const size_t ts = fp32_pairs_ ? sizeof(float) : sizeof(double);
-  return 3 * static_cast<size_t>(n_) * sizeof(double) + 2 * static_cast<size_t>(m + 1) * n_ * ts;
+  return static_cast<size_t>(n_) * sizeof(double) + 2 * static_cast<size_t>(m + 2) * n_ * ts;
 }
 
 void Functional::reserve(int m) {
@@ -400,7 +405,7 @@ void Functional::eval(const double* xi, double* g, double F[4], hipStream_t s) {
   throw_if(!xi || !g || !F, "functional eval: null argument");
   excess_(xi, df_, res_ + kFfSlot, s);
   const Elem e{tab_, tab_ + p_.no, p_.rho0, p_.kT, p_.dv, 2.0 * p_.prefactor / p_.N, n_, p_.no, step_};
-  hipLaunchKernelGGL(local_kernel, dim3(nb_), dim3(kFThreads), 0, s, xi, vext_, df_, g, e, part_, nb_);
+  hipLaunchKernelGGL(local_kernel<double>, dim3(nb_), dim3(kFThreads), 0, s, xi, vext_, df_, g, e, part_, nb_);
   MDFT_HIP_CHECK(hipGetLastError());
   reduce_rows(part_, nb_, 0, 4, 3, res_, s);
   const double* h = read_results(s);
@@ -570,9 +575,9 @@ LbfgsResult Functional::minimize_t(double* xi, const LbfgsOptions& o, hipStream_
   const Elem e{tab_, tab_ + p_.no, p_.rho0, p_.kT, p_.dv, 2.0 * p_.prefactor / p_.N, n_, p_.no, step_};
   double* x = xi;
   double* xt = static_cast<double*>(work_);
-  double* g = xt + n_;
-  double* gt = g + n_;
-  TS* pairs = reinterpret_cast<TS*>(gt + n_);
+  TS* g = reinterpret_cast<TS*>(xt + n_);
+  TS* gt = g + n_;
+  TS* pairs = gt + n_;
   std::vector<TS*> S(nslot), Y(nslot);
   for (int j = 0; j < nslot; ++j) {
     S[j] = pairs + 2 * static_cast<long>(j) * n_;
@@ -581,7 +586,7 @@ LbfgsResult Functional::minimize_t(double* xi, const LbfgsOptions& o, hipStream_
 
   // the start point
   excess_(x, df_, res_ + kFfSlot, s);
-  hipLaunchKernelGGL(local_kernel, dim3(nb_), dim3(kFThreads), 0, s, x, vext_, df_, g, e, part_, nb_);
+  hipLaunchKernelGGL(local_kernel<TS>, dim3(nb_), dim3(kFThreads), 0, s, x, vext_, df_, g, e, part_, nb_);
   MDFT_HIP_CHECK(hipGetLastError());
   reduce_rows(part_, nb_, 0, 4, 3, res_, s);
   const double* h = read_results(s);

@@ -93,7 +93,8 @@ class Functional {
   using ExcessEval = std::function<void(const double* xi, double* df, double* ff_dev, hipStream_t s)>;
 
   // w: quadrature weights (no); vext: device, N x no, borrowed (kept alive by
-  // the caller); fp32_pairs: store the L-BFGS pairs in FP32 (dot products stay FP64)
+  // the caller); fp32_pairs: store the L-BFGS pairs and gradients in FP32
+  // (dot products stay FP64)
   Functional(const FunctionalParams& p, const std::vector<double>& w, const double* vext, ExcessEval excess,
              bool fp32_pairs = false);
   ~Functional();
