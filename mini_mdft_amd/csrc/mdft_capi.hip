@@ -47,9 +47,6 @@ bool is_mixed(int precision) {
   return precision == MDFT_MIXED || precision == MDFT_MIXED_X6 || precision == MDFT_MIXED_X6F;
 }
 
-// the forms whose fields (o, xi, df) are float
-bool float_fields(int precision) { return precision == MDFT_FP32 || precision == MDFT_MIXED_X6F; }
-
 std::unique_ptr<ProjGemm> make_gemm(const AngularGrid& g, int precision) {
   if (!is_mixed(precision)) return nullptr;
   return precision == MDFT_MIXED ? std::make_unique<ProjGemm>(g, 2, 2, true) : std::make_unique<ProjGemm>(g, 3, 3, false);
@@ -745,18 +742,19 @@ int mdft_functional_create(mdft_solver_t solver, const double* vext, double kT, 
   return guarded([&] {
     throw_if(!out || !solver || !vext, "functional_create: null argument");
     *out = nullptr;
-    throw_if(float_fields(solver->precision),
-             "functional: the MDFT_FP32 and MDFT_MIXED_X6F forms have float fields; use MDFT_FP64, MDFT_MIXED or MDFT_MIXED_X6");
+    throw_if(solver->precision == MDFT_FP32,
+             "functional: the MDFT_FP32 form has no FP64 reductions; use MDFT_FP64 or a mixed form");
     const ConvTables& t = solver->conv->dev->tab;
     FunctionalParams p{solver->N, t.ang.no, solver->rho0, solver->prefactor, kT,
                        t.lx * t.ly * t.lz / static_cast<double>(solver->N)};
     auto h = std::make_unique<mdft_functional_s>();
     h->solver = solver;
-    // the mixed forms keep their L-BFGS pairs and gradients in FP32
+    // the mixed forms keep their L-BFGS pairs and gradients in FP32; MDFT_MIXED_X6F
+    // evaluates the excess on an FP32 copy of the FP64 iterate and returns FP32 df
     h->f = std::make_unique<Functional>(
         p, t.ang.w, vext,
-        [solver](const double* xi, double* df, double* ffd, hipStream_t s) { energy_impl(solver, 1, xi, df, ffd, 1, s); },
-        solver->precision != MDFT_FP64);
+        [solver](const void* xi, void* df, double* ffd, hipStream_t s) { energy_impl(solver, 1, xi, df, ffd, 1, s); },
+        solver->precision != MDFT_FP64, solver->precision == MDFT_MIXED_X6F);
     *out = h.release();
   });
 }
@@ -844,8 +842,9 @@ struct mdft_server_s {
     const size_t proj = N * ag.np * capacity * (precision == MDFT_FP64 ? 16 : 8);
     const size_t dtab = (N / 2 + 8) * static_cast<size_t>((sh.mmax + 1) * (sh.mmax + 1)) * (2 * sh.mmax + 1) * ts;
     const size_t slabs = 2 * static_cast<size_t>(capacity) * n * 8 + n * ts;
-    // the column sums of the FP64 mean mode (MDFT_MIXED_X6)
-    const size_t colp = precision == MDFT_MIXED_X6 ? (N * capacity / 64 + 1) * 2 * ag.no * 8 : 0;
+    // the column sums of the FP64 mean mode (MDFT_MIXED_X6, MDFT_MIXED_X6F)
+    const size_t colp =
+        precision == MDFT_MIXED_X6 || precision == MDFT_MIXED_X6F ? (N * capacity / 64 + 1) * 2 * ag.no * 8 : 0;
     return {job, proj + dtab + slabs + colp + (size_t(64) << 20)};
   }
 };
@@ -859,7 +858,7 @@ int mdft_server_create(int np_luc, const int* tuples, const double* c, int nq_ta
     throw_if(!out || !tuples || !c, "server_create: null argument");
     *out = nullptr;
     check_precision(precision);
-    throw_if(float_fields(precision), "server: the MDFT_FP32 and MDFT_MIXED_X6F forms have float fields; use MDFT_FP64, MDFT_MIXED or MDFT_MIXED_X6");
+    throw_if(precision == MDFT_FP32, "server: the MDFT_FP32 form has no FP64 reductions; use MDFT_FP64 or a mixed form");
     throw_if(np_luc <= 0 || nq_table <= 0 || !(dq > 0) || mrso < 1 || !(n0 > 0) || !(temperature > 0),
              "server_create: bad arguments");
     throw_if(capacity < 1, "server_create: capacity must be at least 1");
@@ -896,9 +895,10 @@ int mdft_server_create(int np_luc, const int* tuples, const double* c, int nq_ta
                          t.lx * t.ly * t.lz / static_cast<double>(sv->N)};
       auto eng = std::make_unique<Engine>(
           p, t.ang.w, sp->capacity, sp->opt, sp->precision != MDFT_FP64,
-          [hold, sv](int nb, const double* x, double* df, double* ff, long st, hipStream_t s) {
+          [hold, sv](int nb, const void* x, void* df, double* ff, long st, hipStream_t s) {
             energy_impl(sv, nb, x, df, ff, st, s);
-          });
+          },
+          sp->precision == MDFT_MIXED_X6F);
       eng->use_graphs(sp->graphs);
       table_bytes = sp->footprint(sh).second;
       return eng;

@@ -66,17 +66,24 @@ struct SlotDesc {
   TS* yn;
 };
 
-// grid (nb, slots): x_t of slot blockIdx.y into xslab + slot n
-template <typename TS, int KM>
+// grid (nb, slots): x_t of slot blockIdx.y into xslab + slot n, and with
+// F32F its FP32 copy into xfslab + slot n (the excess evaluation's input)
+template <typename TS, int KM, bool F32F>
 // (xslab is not restrict: a slot's current point may be its slab field, read
 // and overwritten in place element by element)
 __global__ __launch_bounds__(kFThreads) void slot_direction_kernel(const SlotDesc<TS>* __restrict__ desc,
-                                                                   double* xslab, Elem e) {
+                                                                   double* xslab, float* __restrict__ xfslab,
+                                                                   Elem e) {
   const SlotDesc<TS>& a = desc[blockIdx.y];
   if (a.mode == kModeIdle) return;
   double* xt = xslab + static_cast<long>(blockIdx.y) * e.n;
+  float* xf = F32F ? xfslab + static_cast<long>(blockIdx.y) * e.n : nullptr;
   if (a.mode == kModeStart) {
-    MDFT_ELEMENT_LOOP(e) { xt[i] = a.src[i]; }
+    MDFT_ELEMENT_LOOP(e) {
+      const double xv = a.src[i];
+      xt[i] = xv;
+      if (F32F) xf[i] = static_cast<float>(xv);
+    }
     return;
   }
   MDFT_ELEMENT_LOOP(e) {
@@ -90,6 +97,7 @@ __global__ __launch_bounds__(kFThreads) void slot_direction_kernel(const SlotDes
     if (a.save) a.save[i] = xv;
     const double xn = xv + a.t * (-r);
     xt[i] = xn;
+    if (F32F) xf[i] = static_cast<float>(xn);
     a.sc[i] = static_cast<TS>(xn - xv);
   }
 }
@@ -97,17 +105,17 @@ __global__ __launch_bounds__(kFThreads) void slot_direction_kernel(const SlotDes
 // grid (nb, slots): the update pass of Functional's L-BFGS for slot
 // blockIdx.y (rows as in update_kernel with K = KM); at the start point, g
 // and its sums into the same rows (5K+6 g'H g, 5K+7 F_id, 5K+8 F_ext, 5K+9 max)
-template <typename TS, int KM>
+template <typename TS, int KM, typename TF>
 __global__ __launch_bounds__(kFThreads) void slot_update_kernel(const SlotDesc<TS>* __restrict__ desc,
                                                                 const double* __restrict__ xslab,
-                                                                const double* __restrict__ dfslab, Elem e,
+                                                                const TF* __restrict__ dfslab, Elem e,
                                                                 double* __restrict__ part, int nb) {
   constexpr int NA = 5 * KM + 10;
   constexpr int KA = KM > 0 ? KM : 1;
   const SlotDesc<TS>& a = desc[blockIdx.y];
   if (a.mode == kModeIdle) return;
   const double* xt = xslab + static_cast<long>(blockIdx.y) * e.n;
-  const double* df = dfslab + static_cast<long>(blockIdx.y) * e.n;
+  const TF* df = dfslab + static_cast<long>(blockIdx.y) * e.n;
   part += static_cast<long>(blockIdx.y) * kSlotRows * nb;
   double acc[NA];
 #pragma unroll
@@ -116,7 +124,7 @@ __global__ __launch_bounds__(kFThreads) void slot_update_kernel(const SlotDesc<T
     MDFT_ELEMENT_LOOP(e) {
       const double xv = xt[i], v = a.V[i], wio = e.w[io], h = e.h0[io];
       local_f(xv, v, wio, acc[5 * KM + 7], acc[5 * KM + 8]);
-      const TS gs = static_cast<TS>(local_grad(xv, v, df[i], wio, e));
+      const TS gs = static_cast<TS>(local_grad(xv, v, static_cast<double>(df[i]), wio, e));
       a.g[i] = gs;
       const double gv = static_cast<double>(gs);
       const double hg = h * gv;
@@ -128,7 +136,7 @@ __global__ __launch_bounds__(kFThreads) void slot_update_kernel(const SlotDesc<T
       const double wio = e.w[io], h = e.h0[io];
       const double xv = xt[i], v = a.V[i];
       local_f(xv, v, wio, acc[5 * KM + 7], acc[5 * KM + 8]);
-      const TS gs = static_cast<TS>(local_grad(xv, v, df[i], wio, e));
+      const TS gs = static_cast<TS>(local_grad(xv, v, static_cast<double>(df[i]), wio, e));
       const double gn = static_cast<double>(gs);
       const TS ys = static_cast<TS>(gn - static_cast<double>(a.g[i]));
       a.gt[i] = gs;
@@ -183,14 +191,16 @@ __global__ __launch_bounds__(kFThreads) void slot_reduce_kernel(const double* __
   if (threadIdx.x == 0) res[static_cast<long>(blockIdx.y) * kSlotRes + row] = red[0];
 }
 
-__global__ __launch_bounds__(kFThreads) void fill_kernel(double* __restrict__ x, long n, double v) {
+template <typename T>
+__global__ __launch_bounds__(kFThreads) void fill_kernel(T* __restrict__ x, long n, T v) {
   for (long i = static_cast<long>(blockIdx.x) * kFThreads + threadIdx.x; i < n;
        i += static_cast<long>(gridDim.x) * kFThreads)
     x[i] = v;
 }
 
-void fill(double* x, long n, double v, hipStream_t s) {
-  hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n, kFThreads, 65536)), dim3(kFThreads), 0, s, x, n, v);
+template <typename T>
+void fill(T* x, long n, T v, hipStream_t s) {
+  hipLaunchKernelGGL(fill_kernel<T>, dim3(grid_for(n, kFThreads, 65536)), dim3(kFThreads), 0, s, x, n, v);
   MDFT_HIP_CHECK(hipGetLastError());
 }
 
@@ -215,9 +225,10 @@ struct Engine::Slot {
 };
 
 Engine::Engine(const FunctionalParams& p, const std::vector<double>& w, int capacity, const LbfgsOptions& o,
-               bool fp32_pairs, ExcessBatch excess)
-    : p_(p), o_(o), excess_(std::move(excess)), capacity_(capacity), fp32_(fp32_pairs) {
+               bool fp32_pairs, ExcessBatch excess, bool fp32_fields)
+    : p_(p), o_(o), excess_(std::move(excess)), capacity_(capacity), fp32_(fp32_pairs), f32f_(fp32_fields) {
   throw_if(capacity < 1, "engine: capacity must be at least 1");
+  throw_if(fp32_fields && !fp32_pairs, "engine: FP32 fields take FP32 pairs");
   throw_if(o.m < 1 || o.m > kLbfgsMaxPairs, "lbfgs: m must be in 1 .. 8");
   throw_if(o.max_iter < 0 || o.max_backtrack < 1 || !(o.gtol >= 0) || !(o.c1 > 0 && o.c1 < 0.5) ||
                !(o.eps_f >= 0),
@@ -240,7 +251,8 @@ Engine::Engine(const FunctionalParams& p, const std::vector<double>& w, int capa
     MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&tab_), tab.size() * sizeof(double)));
     MDFT_HIP_CHECK(hipMemcpy(tab_, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
     MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&xslab_), capacity * n_ * sizeof(double)));
-    MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&dfslab_), capacity * n_ * sizeof(double)));
+    MDFT_HIP_CHECK(hipMalloc(&dfslab_, capacity * n_ * (f32f_ ? sizeof(float) : sizeof(double))));
+    if (f32f_) MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&xfslab_), capacity * n_ * sizeof(float)));
     MDFT_HIP_CHECK(hipMalloc(&zeros_, n_ * ts));
     MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&part_),
                              static_cast<size_t>(capacity) * kSlotRows * nb_ * sizeof(double)));
@@ -252,13 +264,19 @@ Engine::Engine(const FunctionalParams& p, const std::vector<double>& w, int capa
     std::memset(host_desc_, 0, desc_bytes_);
     // finite contents everywhere the batched passes may read
     fill(xslab_, capacity * n_, 1.0, stream_);
-    fill(dfslab_, capacity * n_, 0.0, stream_);
+    if (f32f_) {
+      fill(xfslab_, capacity * n_, 1.0f, stream_);
+      fill(static_cast<float*>(dfslab_), capacity * n_, 0.0f, stream_);
+    } else {
+      fill(static_cast<double*>(dfslab_), capacity * n_, 0.0, stream_);
+    }
     MDFT_HIP_CHECK(hipMemsetAsync(zeros_, 0, n_ * ts, stream_));
     MDFT_HIP_CHECK(hipStreamSynchronize(stream_));
   } catch (...) {
     release();
     throw;
   }
+  // the slabs: 16 bytes per value either way (x, df: 8 + 8; x, x32, df32: 8 + 4 + 4)
   shared_bytes_ = (2 * static_cast<size_t>(capacity) * n_ + 2 * p.no) * sizeof(double) + n_ * ts +
                   static_cast<size_t>(capacity) * (kSlotRows * nb_ + kSlotRes) * sizeof(double) + desc_bytes_;
   for (int j = 0; j < capacity; ++j) slots_.push_back(nullptr);
@@ -275,6 +293,7 @@ void Engine::release() {
   if (tab_) (void)hipFree(tab_);
   if (xslab_) (void)hipFree(xslab_);
   if (dfslab_) (void)hipFree(dfslab_);
+  if (xfslab_) (void)hipFree(xfslab_);
   if (zeros_) (void)hipFree(zeros_);
   if (part_) (void)hipFree(part_);
   if (res_) (void)hipFree(res_);
@@ -282,8 +301,9 @@ void Engine::release() {
   if (desc_) (void)hipFree(desc_);
   if (host_desc_) (void)hipHostFree(host_desc_);
   if (stream_) (void)hipStreamDestroy(stream_);
-  tab_ = xslab_ = dfslab_ = part_ = res_ = host_res_ = nullptr;
-  zeros_ = desc_ = host_desc_ = nullptr;
+  tab_ = xslab_ = part_ = res_ = host_res_ = nullptr;
+  xfslab_ = nullptr;
+  zeros_ = dfslab_ = desc_ = host_desc_ = nullptr;
   stream_ = nullptr;
 }
 
@@ -345,24 +365,36 @@ void Engine::enqueue(int nbatch) {
   const int KM = o_.m, NA = 5 * KM + 10;
   dispatch_k(KM, [&](auto kc) {
     constexpr int K = decltype(kc)::value;
-    if (fp32_) {
-      hipLaunchKernelGGL((slot_direction_kernel<float, K>), dim3(nb_, nbatch), dim3(kFThreads), 0, stream_,
-                         static_cast<const SlotDesc<float>*>(desc_), xslab_, e);
+    if (f32f_) {
+      hipLaunchKernelGGL((slot_direction_kernel<float, K, true>), dim3(nb_, nbatch), dim3(kFThreads), 0, stream_,
+                         static_cast<const SlotDesc<float>*>(desc_), xslab_, xfslab_, e);
+    } else if (fp32_) {
+      hipLaunchKernelGGL((slot_direction_kernel<float, K, false>), dim3(nb_, nbatch), dim3(kFThreads), 0, stream_,
+                         static_cast<const SlotDesc<float>*>(desc_), xslab_, nullptr, e);
     } else {
-      hipLaunchKernelGGL((slot_direction_kernel<double, K>), dim3(nb_, nbatch), dim3(kFThreads), 0, stream_,
-                         static_cast<const SlotDesc<double>*>(desc_), xslab_, e);
+      hipLaunchKernelGGL((slot_direction_kernel<double, K, false>), dim3(nb_, nbatch), dim3(kFThreads), 0,
+                         stream_, static_cast<const SlotDesc<double>*>(desc_), xslab_, nullptr, e);
     }
   });
   MDFT_HIP_CHECK(hipGetLastError());
-  excess_(nbatch, xslab_, dfslab_, res_ + kSlotFf, kSlotRes, stream_);
+  if (f32f_)
+    excess_(nbatch, xfslab_, dfslab_, res_ + kSlotFf, kSlotRes, stream_);
+  else
+    excess_(nbatch, xslab_, dfslab_, res_ + kSlotFf, kSlotRes, stream_);
   dispatch_k(KM, [&](auto kc) {
     constexpr int K = decltype(kc)::value;
-    if (fp32_) {
-      hipLaunchKernelGGL((slot_update_kernel<float, K>), dim3(nb_, nbatch), dim3(kFThreads), 0, stream_,
-                         static_cast<const SlotDesc<float>*>(desc_), xslab_, dfslab_, e, part_, nb_);
+    if (f32f_) {
+      hipLaunchKernelGGL((slot_update_kernel<float, K, float>), dim3(nb_, nbatch), dim3(kFThreads), 0, stream_,
+                         static_cast<const SlotDesc<float>*>(desc_), xslab_, static_cast<const float*>(dfslab_), e,
+                         part_, nb_);
+    } else if (fp32_) {
+      hipLaunchKernelGGL((slot_update_kernel<float, K, double>), dim3(nb_, nbatch), dim3(kFThreads), 0, stream_,
+                         static_cast<const SlotDesc<float>*>(desc_), xslab_, static_cast<const double*>(dfslab_),
+                         e, part_, nb_);
     } else {
-      hipLaunchKernelGGL((slot_update_kernel<double, K>), dim3(nb_, nbatch), dim3(kFThreads), 0, stream_,
-                         static_cast<const SlotDesc<double>*>(desc_), xslab_, dfslab_, e, part_, nb_);
+      hipLaunchKernelGGL((slot_update_kernel<double, K, double>), dim3(nb_, nbatch), dim3(kFThreads), 0, stream_,
+                         static_cast<const SlotDesc<double>*>(desc_), xslab_, static_cast<const double*>(dfslab_),
+                         e, part_, nb_);
     }
   });
   MDFT_HIP_CHECK(hipGetLastError());

@@ -22,12 +22,15 @@ namespace mdft {
 // replayed; per-slot scalars travel in a device table uploaded each tick.
 class Engine {
  public:
-  // nb fields back to back in x / df; ff of field j into ff[j ffstride]
-  using ExcessBatch = std::function<void(int nb, const double* x, double* df, double* ff, long ffstride,
+  // nb fields back to back in x / df (double, or float with fp32_fields);
+  // ff of field j into ff[j ffstride]
+  using ExcessBatch = std::function<void(int nb, const void* x, void* df, double* ff, long ffstride,
                                          hipStream_t s)>;
 
+  // fp32_fields (needs fp32_pairs): the excess sees FP32 copies of the FP64
+  // trial points and returns FP32 df
   Engine(const FunctionalParams& p, const std::vector<double>& w, int capacity, const LbfgsOptions& o,
-         bool fp32_pairs, ExcessBatch excess);
+         bool fp32_pairs, ExcessBatch excess, bool fp32_fields = false);
   ~Engine();
   Engine(const Engine&) = delete;
   Engine& operator=(const Engine&) = delete;
@@ -75,13 +78,15 @@ class Engine {
   ExcessBatch excess_;
   int capacity_;
   bool fp32_;
+  bool f32f_;
   bool graphs_ = true;
   long n_ = 0;
   int nb_ = 0, step_ = 0;
   hipStream_t stream_ = nullptr;
   double* tab_ = nullptr;    // w | h0 (device)
   double* xslab_ = nullptr;  // capacity trial points, back to back
-  double* dfslab_ = nullptr;
+  float* xfslab_ = nullptr;  // their FP32 copies (fp32_fields)
+  void* dfslab_ = nullptr;   // double, or float with fp32_fields
   void* zeros_ = nullptr;    // one zero field of pair values (unused pair columns)
   double* part_ = nullptr;   // capacity x rows x nb partial sums
   double* res_ = nullptr;    // capacity x results

@@ -40,15 +40,15 @@ using namespace elem;
 
 // g, F_id and F_ext sums, g'H g and max |h0 g| at x; g is stored as TG
 // (the sums take the stored value)
-template <typename TG>
+template <typename TG, typename TF>
 __global__ __launch_bounds__(kFThreads) void local_kernel(const double* __restrict__ x, const double* __restrict__ V,
-                                                          const double* __restrict__ df, TG* __restrict__ g,
+                                                          const TF* __restrict__ df, TG* __restrict__ g,
                                                           Elem e, double* __restrict__ part, int nb) {
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   MDFT_ELEMENT_LOOP(e) {
     const double xv = x[i], v = V[i], wio = e.w[io], h = e.h0[io];
     local_f(xv, v, wio, acc[0], acc[1]);
-    const TG gs = static_cast<TG>(local_grad(xv, v, df[i], wio, e));
+    const TG gs = static_cast<TG>(local_grad(xv, v, static_cast<double>(df[i]), wio, e));
     g[i] = gs;
     const double gv = static_cast<double>(gs);
     const double hg = h * gv;
@@ -71,8 +71,17 @@ struct DirArgs {
   double cs[kLbfgsMaxPairs], cq[kLbfgsMaxPairs];
   double cg, gamma, t;
   double* xt;  // x + t d
+  float* xf;   // its FP32 copy for the excess evaluation, or null
   TS* sc;      // xt - x (the candidate s)
 };
+
+// xf = x rounded to FP32 (the evaluation's copy of the start point)
+__global__ __launch_bounds__(kFThreads) void to_float_kernel(const double* __restrict__ x, float* __restrict__ xf,
+                                                             long n) {
+  for (long i = static_cast<long>(blockIdx.x) * kFThreads + threadIdx.x; i < n;
+       i += static_cast<long>(gridDim.x) * kFThreads)
+    xf[i] = static_cast<float>(x[i]);
+}
 
 // d = -(gamma h0 (cg g + sum cq_j y_j) + sum cs_j s_j); xt = x + t d; sc = xt - x
 template <typename TS, int K>
@@ -87,15 +96,16 @@ __global__ __launch_bounds__(kFThreads) void direction_kernel(DirArgs<TS> a, Ele
     const double xv = a.x[i];
     const double xn = xv + a.t * (-r);
     a.xt[i] = xn;
+    if (a.xf) a.xf[i] = static_cast<float>(xn);
     a.sc[i] = static_cast<TS>(xn - xv);
   }
 }
 
-template <typename TS>
+template <typename TS, typename TF>
 struct UpdArgs {
   const double* xt;
   const double* V;
-  const double* df;
+  const TF* df;
   const TS* g;  // g at x
   TS* gt;       // out: g at xt
   const TS* sc;
@@ -111,8 +121,8 @@ struct UpdArgs {
 //   [4K+5, 5K+5) y_j'H g_t | 5K+5 yn'H g_t | 5K+6 g_t'H g_t | 5K+7 F_id sum |
 //   5K+8 F_ext sum | 5K+9 max |h0 g_t|
 // g_t and yn enter the sums as stored (rounded to TS).
-template <typename TS, int K>
-__global__ __launch_bounds__(kFThreads) void update_kernel(UpdArgs<TS> a, Elem e, double* __restrict__ part,
+template <typename TS, typename TF, int K>
+__global__ __launch_bounds__(kFThreads) void update_kernel(UpdArgs<TS, TF> a, Elem e, double* __restrict__ part,
                                                            int nb) {
   constexpr int NA = 5 * K + 10;
   constexpr int KA = K > 0 ? K : 1;
@@ -123,7 +133,7 @@ __global__ __launch_bounds__(kFThreads) void update_kernel(UpdArgs<TS> a, Elem e
     const double wio = e.w[io], h = e.h0[io];
     const double xv = a.xt[i], v = a.V[i];
     local_f(xv, v, wio, acc[5 * K + 7], acc[5 * K + 8]);
-    const TS gs = static_cast<TS>(local_grad(xv, v, a.df[i], wio, e));
+    const TS gs = static_cast<TS>(local_grad(xv, v, static_cast<double>(a.df[i]), wio, e));
     const double gn = static_cast<double>(gs);
     const TS ys = static_cast<TS>(gn - static_cast<double>(a.g[i]));
     a.gt[i] = gs;
@@ -327,8 +337,8 @@ void reduce_rows(const double* part, int nb, int first, int rows, int max_row, d
 }  // namespace
 
 Functional::Functional(const FunctionalParams& p, const std::vector<double>& w, const double* vext, ExcessEval excess,
-                       bool fp32_pairs)
-    : p_(p), excess_(std::move(excess)), vext_(vext), fp32_pairs_(fp32_pairs) {
+                       bool fp32_pairs, bool fp32_fields)
+    : p_(p), excess_(std::move(excess)), vext_(vext), fp32_pairs_(fp32_pairs), fp32_fields_(fp32_fields) {
   throw_if(p.N <= 0 || p.no <= 0 || static_cast<int>(w.size()) != p.no, "functional: bad sizes");
   throw_if(!vext, "functional: null vext");
   throw_if(!(p.kT > 0) || !(p.dv > 0) || !(p.rho0 > 0), "functional: kT, dv and rho0 must be positive");
@@ -344,7 +354,8 @@ Functional::Functional(const FunctionalParams& p, const std::vector<double>& w, 
   try {
     MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&tab_), tab.size() * sizeof(double)));
     MDFT_HIP_CHECK(hipMemcpy(tab_, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
-    MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&df_), n_ * sizeof(double)));
+    MDFT_HIP_CHECK(hipMalloc(&df_, n_ * (fp32_fields_ ? sizeof(float) : sizeof(double))));
+    if (fp32_fields_) MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&xf_), n_ * sizeof(float)));
     MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&part_), static_cast<size_t>(kRows) * nb_ * sizeof(double)));
     MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&res_), kResults * sizeof(double)));
     MDFT_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&host_), kResults * sizeof(double)));
@@ -359,18 +370,21 @@ Functional::~Functional() { release(); }
 void Functional::release() {
   if (tab_) (void)hipFree(tab_);
   if (df_) (void)hipFree(df_);
+  if (xf_) (void)hipFree(xf_);
+  xf_ = nullptr;
   if (part_) (void)hipFree(part_);
   if (res_) (void)hipFree(res_);
   if (host_) (void)hipHostFree(host_);
   if (work_) (void)hipFree(work_);
-  tab_ = df_ = part_ = res_ = host_ = nullptr;
+  tab_ = part_ = res_ = host_ = nullptr;
+  df_ = nullptr;
   work_ = nullptr;
   m_alloc_ = 0;
 }
 
 size_t Functional::bytes() const {
-  return (2 * static_cast<size_t>(p_.no) + n_ + static_cast<size_t>(kRows) * nb_ + kResults) * sizeof(double) +
-         (m_alloc_ ? minimize_bytes(m_alloc_) : 0);
+  return (2 * static_cast<size_t>(p_.no) + static_cast<size_t>(kRows) * nb_ + kResults) * sizeof(double) +
+         n_ * (fp32_fields_ ? 2 * sizeof(float) : sizeof(double)) + (m_alloc_ ? minimize_bytes(m_alloc_) : 0);
 }
 
 size_t Functional::minimize_bytes(int m) const {
@@ -403,9 +417,17 @@ void Functional::fparts(double sid, double sext, double ff, double F[4]) const {
 
 void Functional::eval(const double* xi, double* g, double F[4], hipStream_t s) {
   throw_if(!xi || !g || !F, "functional eval: null argument");
-  excess_(xi, df_, res_ + kFfSlot, s);
   const Elem e{tab_, tab_ + p_.no, p_.rho0, p_.kT, p_.dv, 2.0 * p_.prefactor / p_.N, n_, p_.no, step_};
-  hipLaunchKernelGGL(local_kernel<double>, dim3(nb_), dim3(kFThreads), 0, s, xi, vext_, df_, g, e, part_, nb_);
+  if (fp32_fields_) {
+    hipLaunchKernelGGL(to_float_kernel, dim3(nb_), dim3(kFThreads), 0, s, xi, xf_, n_);
+    excess_(xf_, df_, res_ + kFfSlot, s);
+    hipLaunchKernelGGL((local_kernel<double, float>), dim3(nb_), dim3(kFThreads), 0, s, xi, vext_,
+                       static_cast<const float*>(df_), g, e, part_, nb_);
+  } else {
+    excess_(xi, df_, res_ + kFfSlot, s);
+    hipLaunchKernelGGL((local_kernel<double, double>), dim3(nb_), dim3(kFThreads), 0, s, xi, vext_,
+                       static_cast<const double*>(df_), g, e, part_, nb_);
+  }
   MDFT_HIP_CHECK(hipGetLastError());
   reduce_rows(part_, nb_, 0, 4, 3, res_, s);
   const double* h = read_results(s);
@@ -419,7 +441,11 @@ LbfgsResult Functional::minimize(double* xi, const LbfgsOptions& o, hipStream_t 
                !(o.eps_f >= 0),
            "lbfgs: bad options");
   reserve(o.m);
-  return fp32_pairs_ ? minimize_t<float>(xi, o, s) : minimize_t<double>(xi, o, s);
+  if (fp32_fields_) {
+    throw_if(!fp32_pairs_, "functional: FP32 fields take FP32 pairs");
+    return minimize_t<float, float>(xi, o, s);
+  }
+  return fp32_pairs_ ? minimize_t<float, double>(xi, o, s) : minimize_t<double, double>(xi, o, s);
 }
 
 LbfgsState::LbfgsState(const LbfgsOptions& o)
@@ -569,8 +595,10 @@ bool LbfgsState::judge(const double Fn[4], const double* h, int K) {
   return true;
 }
 
-template <typename TS>
+template <typename TS, typename TF>
 LbfgsResult Functional::minimize_t(double* xi, const LbfgsOptions& o, hipStream_t s) {
+  constexpr bool F32 = std::is_same<TF, float>::value;
+  TF* df = static_cast<TF*>(df_);
   const int nslot = o.m + 1;
   const Elem e{tab_, tab_ + p_.no, p_.rho0, p_.kT, p_.dv, 2.0 * p_.prefactor / p_.N, n_, p_.no, step_};
   double* x = xi;
@@ -585,8 +613,14 @@ LbfgsResult Functional::minimize_t(double* xi, const LbfgsOptions& o, hipStream_
   }
 
   // the start point
-  excess_(x, df_, res_ + kFfSlot, s);
-  hipLaunchKernelGGL(local_kernel<TS>, dim3(nb_), dim3(kFThreads), 0, s, x, vext_, df_, g, e, part_, nb_);
+  if (F32) {
+    hipLaunchKernelGGL(to_float_kernel, dim3(nb_), dim3(kFThreads), 0, s, x, xf_, n_);
+    MDFT_HIP_CHECK(hipGetLastError());
+    excess_(xf_, df, res_ + kFfSlot, s);
+  } else {
+    excess_(x, df, res_ + kFfSlot, s);
+  }
+  hipLaunchKernelGGL((local_kernel<TS, TF>), dim3(nb_), dim3(kFThreads), 0, s, x, vext_, df, g, e, part_, nb_);
   MDFT_HIP_CHECK(hipGetLastError());
   reduce_rows(part_, nb_, 0, 4, 3, res_, s);
   const double* h = read_results(s);
@@ -601,7 +635,7 @@ LbfgsResult Functional::minimize_t(double* xi, const LbfgsOptions& o, hipStream_
   while (st.next(tr)) {
     const int K = tr.k;
     DirArgs<TS> da{};
-    UpdArgs<TS> ua{};
+    UpdArgs<TS, TF> ua{};
     for (int j = 0; j < K; ++j) {
       da.s[j] = ua.s[j] = S[st.order()[j]];
       da.y[j] = ua.y[j] = Y[st.order()[j]];
@@ -614,10 +648,11 @@ LbfgsResult Functional::minimize_t(double* xi, const LbfgsOptions& o, hipStream_
     da.gamma = tr.gamma;
     da.t = tr.t;
     da.xt = xt;
+    da.xf = F32 ? xf_ : nullptr;
     da.sc = S[tr.cand];
     ua.xt = xt;
     ua.V = vext_;
-    ua.df = df_;
+    ua.df = df;
     ua.g = g;
     ua.gt = gt;
     ua.sc = S[tr.cand];
@@ -626,10 +661,13 @@ LbfgsResult Functional::minimize_t(double* xi, const LbfgsOptions& o, hipStream_
       hipLaunchKernelGGL((direction_kernel<TS, decltype(kc)::value>), dim3(nb_), dim3(kFThreads), 0, s, da, e);
     });
     MDFT_HIP_CHECK(hipGetLastError());
-    excess_(xt, df_, res_ + kFfSlot, s);
+    if (F32)
+      excess_(xf_, df, res_ + kFfSlot, s);
+    else
+      excess_(xt, df, res_ + kFfSlot, s);
     dispatch_k(K, [&](auto kc) {
-      hipLaunchKernelGGL((update_kernel<TS, decltype(kc)::value>), dim3(nb_), dim3(kFThreads), 0, s, ua, e, part_,
-                         nb_);
+      hipLaunchKernelGGL((update_kernel<TS, TF, decltype(kc)::value>), dim3(nb_), dim3(kFThreads), 0, s, ua, e,
+                         part_, nb_);
     });
     MDFT_HIP_CHECK(hipGetLastError());
     reduce_rows(part_, nb_, 0, 5 * K + 10, 5 * K + 9, res_, s);
@@ -646,7 +684,8 @@ LbfgsResult Functional::minimize_t(double* xi, const LbfgsOptions& o, hipStream_
   return st.result();
 }
 
-template LbfgsResult Functional::minimize_t<double>(double*, const LbfgsOptions&, hipStream_t);
-template LbfgsResult Functional::minimize_t<float>(double*, const LbfgsOptions&, hipStream_t);
+template LbfgsResult Functional::minimize_t<double, double>(double*, const LbfgsOptions&, hipStream_t);
+template LbfgsResult Functional::minimize_t<float, double>(double*, const LbfgsOptions&, hipStream_t);
+template LbfgsResult Functional::minimize_t<float, float>(double*, const LbfgsOptions&, hipStream_t);
 
 }  // namespace mdft

@@ -88,15 +88,18 @@ class LbfgsState {
 
 class Functional {
  public:
-  // the excess (c-proj) evaluation: fills df (device, N x no doubles) for xi
-  // and writes ff to ff_dev (device), stream ordered, without synchronising
-  using ExcessEval = std::function<void(const double* xi, double* df, double* ff_dev, hipStream_t s)>;
+  // the excess (c-proj) evaluation: fills df (device, N x no) for xi and
+  // writes ff to ff_dev (device), stream ordered, without synchronising; xi
+  // and df are doubles, or floats with fp32_fields
+  using ExcessEval = std::function<void(const void* xi, void* df, double* ff_dev, hipStream_t s)>;
 
   // w: quadrature weights (no); vext: device, N x no, borrowed (kept alive by
   // the caller); fp32_pairs: store the L-BFGS pairs and gradients in FP32
-  // (dot products stay FP64)
+  // (dot products stay FP64); fp32_fields: the excess evaluation takes an
+  // FP32 copy of the point (written by the trial pass) and gives df in FP32
+  // (MDFT_MIXED_X6F); the point itself and the functional's sums stay FP64
   Functional(const FunctionalParams& p, const std::vector<double>& w, const double* vext, ExcessEval excess,
-             bool fp32_pairs = false);
+             bool fp32_pairs = false, bool fp32_fields = false);
   ~Functional();
   Functional(const Functional&) = delete;
   Functional& operator=(const Functional&) = delete;
@@ -116,18 +119,20 @@ class Functional {
   void release();
   const double* read_results(hipStream_t s);
   void fparts(double sid, double sext, double ff, double F[4]) const;
-  template <typename TS>
+  template <typename TS, typename TF>
   LbfgsResult minimize_t(double* xi, const LbfgsOptions& o, hipStream_t s);
 
   FunctionalParams p_;
   ExcessEval excess_;
   const double* vext_;       // borrowed
   bool fp32_pairs_ = false;
+  bool fp32_fields_ = false;
   long n_ = 0;               // N * no
   int nb_ = 0;               // blocks (= partials per sum) of the element passes
   int step_ = 0;             // (blocks * threads) mod no
   double* tab_ = nullptr;    // device: w (no), h0 (no)
-  double* df_ = nullptr;     // device, N x no
+  void* df_ = nullptr;       // device, N x no (double, or float with fp32_fields)
+  float* xf_ = nullptr;      // device, N x no: the FP32 copy of the evaluated point (fp32_fields)
   double* part_ = nullptr;   // device partials, kMaxSums x nb
   double* res_ = nullptr;    // device results, kMaxSums + 1
   double* host_ = nullptr;   // pinned copy of res_

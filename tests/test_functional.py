@@ -366,7 +366,7 @@ def test_gpu_lbfgs_c0_limit(oracle):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec", [M.MIXED_X6, M.MIXED])
+@pytest.mark.parametrize("prec", [M.MIXED_X6, M.MIXED, M.MIXED_X6F])
 def test_gpu_lbfgs_mixed_vs_fp64(oracle, prec):
     """A mixed-form solve reaches the FP64 minimum: |dF| <= 0.01 kJ/mol, and
     the FP64 gradient at the mixed minimiser is small (its relative L2 against

@@ -5,7 +5,7 @@ admission keeps the device footprint under the budget."""
 import numpy as np
 import pytest
 
-from mini_mdft_amd import FP32, FP64, MIXED_X6, luc, solute
+from mini_mdft_amd import FP32, FP64, MIXED_X6, MIXED_X6F, luc, solute
 from mini_mdft_amd import mdft as M
 
 N3, L3, MMAX, MRSO = (8, 8, 8), (10.0,) * 3, 3, 2
@@ -50,7 +50,7 @@ def _run(srv, subs, stagger=0):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec", [FP64, MIXED_X6])
+@pytest.mark.parametrize("prec", [FP64, MIXED_X6, MIXED_X6F])
 def test_alone_equals_in_the_mix_bitwise(prec):
     import torch
 
@@ -73,7 +73,7 @@ def test_alone_equals_in_the_mix_bitwise(prec):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec", [FP64, MIXED_X6])
+@pytest.mark.parametrize("prec", [FP64, MIXED_X6, MIXED_X6F])
 def test_server_matches_functional_minimize(prec):
     """A solute through the server and through the plain Functional.minimize
     on the same inputs: the same decisions and the same numbers."""
