@@ -386,7 +386,11 @@ __global__ __launch_bounds__(64) void proj2angl_wave_kernel(const cplx<TP>* __re
                                                             double* __restrict__ colpart) {
   using A = AngSizes<MMAX, MRSO>;
   constexpr int NP = A::np();
-  __shared__ TC stage[kWaveRows * A::kNjk];
+  // FP64 outputs are staged half the phi values at a time (the full stage of
+  // 64 x nphi npsi doubles would hold a CU to five blocks)
+  constexpr int kChunks = sizeof(TC) == 8 ? 2 : 1;
+  constexpr int kJc = (A::kNphi + kChunks - 1) / kChunks;
+  __shared__ TC stage[kWaveRows * kJc * A::kNpsi];
   const int lane = threadIdx.x;
   const int q = blockIdx.x / kXcds;
   const int th = q % A::kNth;
@@ -430,129 +434,135 @@ __global__ __launch_bounds__(64) void proj2angl_wave_kernel(const cplx<TP>* __re
       si[MMAX - mup][b] = pi - ni;
     }
   }
-  TC* out = stage + lane * A::kNjk;
-#pragma unroll
-  for (int j = 0; j < A::kNphi; ++j) {
-    TC zr[A::kNmu2], zi[A::kNmu2];
-#pragma unroll
-    for (int b = 0; b < A::kNmu2; ++b) {
-      zr[b] = sr[MMAX][b];
-      zi[b] = si[MMAX][b];
-#pragma unroll
-      for (int mup = 1; mup <= MMAX; ++mup) {
-        const cplx<TC> e = tb.ephi[(mup * j) % A::kNphi];
-        zr[b] = fma(sr[MMAX + mup][b], e.x, fma(-si[MMAX - mup][b], e.y, zr[b]));
-        zi[b] = fma(si[MMAX + mup][b], e.x, fma(sr[MMAX - mup][b], e.y, zi[b]));
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < A::kNpsi; ++k) {
-      TC v = zr[0];
-#pragma unroll
-      for (int b = 1; b < A::kNmu2; ++b) {
-        const cplx<TC> e = tb.epsi[b * A::kNpsi + k];
-        v += static_cast<TC>(2) * (zr[b] * e.x - zi[b] * e.y);
-      }
-      out[j * A::kNpsi + k] = v;
-    }
-  }
-  __syncthreads();
-
   double acc_ff = 0.0;
   const long base = r0 * A::kNo + th * A::kNjk;
-  constexpr int kE = kWaveRows * A::kNjk / 64;  // elements per lane
-  if (colpart != nullptr && xi != nullptr) {
-    // get_delta_f with the column sums: lane jk owns orientation th * nphi
-    // npsi + jk and walks the block's rows, so that its sums over r of
-    // xi^2 - 1 and of vexc build up in registers (each row's nphi npsi values
-    // are one contiguous run of xi and df; for the mixed forms this walk also
-    // measured faster than the row-major one below, 0.358 against 0.371 ms
-    // at 64^3, and in FP64 the same, 1.11 against 1.10 ms)
-    constexpr int kRB = 32;  // rows per batch of loads in flight
-    const int nrow = static_cast<int>(min(static_cast<long>(kWaveRows), N - r0));
-    const bool on = lane < A::kNjk;
-    const int jk = on ? lane : 0;
-    double cs = 0.0, cv = 0.0;
-    for (int rl0 = 0; rl0 < kWaveRows; rl0 += kRB) {
-      double xv[kRB];
+  // the (phi, psi) outputs of chunk c: phi in [c kJc, c kJc + jn), their nphi
+  // npsi columns [c0, c0 + W) of the row's run, staged as [64 rows][W]
+  const auto chunk = [&](auto cc) {
+    constexpr int c = decltype(cc)::value;
+    constexpr int j0 = c * kJc, jn = A::kNphi - j0 < kJc ? A::kNphi - j0 : kJc;
+    constexpr int W = jn * A::kNpsi, c0 = j0 * A::kNpsi;
+    if (c > 0) __syncthreads();  // the previous chunk's epilogue has read the stage
+    TC* out = stage + lane * W;
 #pragma unroll
-      for (int b = 0; b < kRB; ++b) {
-        const int rl = min(rl0 + b, nrow - 1);
-        xv[b] = static_cast<double>(xi[base + static_cast<long>(rl) * A::kNo + jk]);
-      }
+    for (int jj = 0; jj < jn; ++jj) {
+      const int j = j0 + jj;
+      TC zr[A::kNmu2], zi[A::kNmu2];
 #pragma unroll
-      for (int b = 0; b < kRB; ++b) {
-        const int rl = rl0 + b;
-        if (on && rl < nrow) {
-          const double x = xv[b];
-          const double vx = static_cast<double>(static_cast<TO>(stage[rl * A::kNjk + jk]));
-          const double x2m1 = x * x - 1.0;
-          acc_ff += vx * wth * x2m1;
-          df[base + static_cast<long>(rl) * A::kNo + jk] = static_cast<TO>(2.0 * rho0 * x * vx);
-          cs += x2m1;
-          cv += vx;
+      for (int b = 0; b < A::kNmu2; ++b) {
+        zr[b] = sr[MMAX][b];
+        zi[b] = si[MMAX][b];
+#pragma unroll
+        for (int mup = 1; mup <= MMAX; ++mup) {
+          const cplx<TC> e = tb.ephi[(mup * j) % A::kNphi];
+          zr[b] = fma(sr[MMAX + mup][b], e.x, fma(-si[MMAX - mup][b], e.y, zr[b]));
+          zi[b] = fma(si[MMAX + mup][b], e.x, fma(sr[MMAX - mup][b], e.y, zi[b]));
         }
       }
-    }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) acc_ff += __shfl_xor(acc_ff, off);
-    if (lane == 0) partials[rb * A::kNth + th] = acc_ff;
-    if (on) {
-      // colpart[rb][0][io] = sum_r (xi^2 - 1), colpart[rb][1][io] = sum_r vexc
-      double* cp = colpart + rb * (2 * A::kNo) + th * A::kNjk + jk;
-      cp[0] = cs;
-      cp[A::kNo] = cv;
-    }
-    return;
-  }
-  if (xi != nullptr && r0 + kWaveRows <= N) {
-    // full block: batches of kB elements per lane with all their xi loads in
-    // flight before the arithmetic.  The mixed forms (FP32 arithmetic, FP64
-    // fields) take all kE at once: at mmax 5 that waits out one HBM latency
-    // instead of five and measured 3% faster, at the same three waves per
-    // SIMD.  The FP64 and FP32 forms keep batches of 11 (one batch would
-    // nearly double their registers at mmax 5).
-    constexpr bool kMixed = std::is_same<TC, float>::value && std::is_same<TO, double>::value;
-    constexpr int kB = kMixed && kE <= 64 ? kE : 11;
-    for (int i0 = 0; i0 < kE; i0 += kB) {
-      double xv[kB];
+      for (int k = 0; k < A::kNpsi; ++k) {
+        TC v = zr[0];
 #pragma unroll
-      for (int b = 0; b < kB; ++b) {
-        const int e = lane + 64 * min(i0 + b, kE - 1);
-        const int rl = e / A::kNjk, jk = e - rl * A::kNjk;
-        xv[b] = static_cast<double>(xi[base + static_cast<long>(rl) * A::kNo + jk]);
+        for (int b = 1; b < A::kNmu2; ++b) {
+          const cplx<TC> e = tb.epsi[b * A::kNpsi + k];
+          v += static_cast<TC>(2) * (zr[b] * e.x - zi[b] * e.y);
+        }
+        out[jj * A::kNpsi + k] = v;
       }
+    }
+    __syncthreads();
+
+    constexpr int kE = W;  // elements per lane (64 rows x W columns)
+    if (colpart != nullptr && xi != nullptr) {
+      // get_delta_f with the column sums: lane jk owns orientation th * nphi
+      // npsi + c0 + jk and walks the block's rows, so that its sums over r of
+      // xi^2 - 1 and of vexc build up in registers (each row's W values are
+      // one contiguous run of xi and df; for the mixed forms this walk also
+      // measured faster than the row-major one below, 0.358 against 0.371 ms
+      // at 64^3, and in FP64 the same, 1.11 against 1.10 ms)
+      constexpr int kRB = 32;  // rows per batch of loads in flight
+      const int nrow = static_cast<int>(min(static_cast<long>(kWaveRows), N - r0));
+      const bool on = lane < W;
+      const int jk = on ? lane : 0;
+      double cs = 0.0, cv = 0.0;
+      for (int rl0 = 0; rl0 < kWaveRows; rl0 += kRB) {
+        double xv[kRB];
 #pragma unroll
-      for (int b = 0; b < kB; ++b) {
-        if (i0 + b < kE) {
-          const int e = lane + 64 * (i0 + b);
-          const int rl = e / A::kNjk, jk = e - rl * A::kNjk;
-          const long g = base + static_cast<long>(rl) * A::kNo + jk;
-          const double x = xv[b];
-          const double vx = static_cast<double>(static_cast<TO>(stage[e]));
-          acc_ff += vx * wth * (x * x - 1.0);
-          df[g] = static_cast<TO>(2.0 * rho0 * x * vx);
+        for (int b = 0; b < kRB; ++b) {
+          const int rl = min(rl0 + b, nrow - 1);
+          xv[b] = static_cast<double>(xi[base + static_cast<long>(rl) * A::kNo + c0 + jk]);
+        }
+#pragma unroll
+        for (int b = 0; b < kRB; ++b) {
+          const int rl = rl0 + b;
+          if (on && rl < nrow) {
+            const double x = xv[b];
+            const double vx = static_cast<double>(static_cast<TO>(stage[rl * W + jk]));
+            const double x2m1 = x * x - 1.0;
+            acc_ff += vx * wth * x2m1;
+            df[base + static_cast<long>(rl) * A::kNo + c0 + jk] = static_cast<TO>(2.0 * rho0 * x * vx);
+            cs += x2m1;
+            cv += vx;
+          }
         }
       }
-    }
-  } else {
+      if (on) {
+        // colpart[rb][0][io] = sum_r (xi^2 - 1), colpart[rb][1][io] = sum_r vexc
+        double* cp = colpart + rb * (2 * A::kNo) + th * A::kNjk + c0 + jk;
+        cp[0] = cs;
+        cp[A::kNo] = cv;
+      }
+    } else if (xi != nullptr && r0 + kWaveRows <= N) {
+      // full block: batches of kB elements per lane with all their xi loads
+      // in flight before the arithmetic.  The mixed forms (FP32 arithmetic,
+      // FP64 fields) take all kE at once: at mmax 5 that waits out one HBM
+      // latency instead of five and measured 3% faster, at the same three
+      // waves per SIMD.  The FP64 and FP32 forms keep batches of 11 (one
+      // batch would nearly double their registers at mmax 5).
+      constexpr bool kMixed = std::is_same<TC, float>::value && std::is_same<TO, double>::value;
+      constexpr int kB = kMixed && kE <= 64 ? kE : 11;
+      for (int i0 = 0; i0 < kE; i0 += kB) {
+        double xv[kB];
+#pragma unroll
+        for (int b = 0; b < kB; ++b) {
+          const int e = lane + 64 * min(i0 + b, kE - 1);
+          const int rl = e / W, jk = e - rl * W;
+          xv[b] = static_cast<double>(xi[base + static_cast<long>(rl) * A::kNo + c0 + jk]);
+        }
+#pragma unroll
+        for (int b = 0; b < kB; ++b) {
+          if (i0 + b < kE) {
+            const int e = lane + 64 * (i0 + b);
+            const int rl = e / W, jk = e - rl * W;
+            const long g = base + static_cast<long>(rl) * A::kNo + c0 + jk;
+            const double x = xv[b];
+            const double vx = static_cast<double>(static_cast<TO>(stage[e]));
+            acc_ff += vx * wth * (x * x - 1.0);
+            df[g] = static_cast<TO>(2.0 * rho0 * x * vx);
+          }
+        }
+      }
+    } else {
 #pragma unroll 5
-    for (int e = lane; e < kWaveRows * A::kNjk; e += 64) {
-      const int rl = e / A::kNjk, jk = e - rl * A::kNjk;
-      if (r0 + rl < N) {
-        const long g = base + static_cast<long>(rl) * A::kNo + jk;
-        const TC v = stage[e];
-        if (xi == nullptr) {
-          o[g] = static_cast<TO>(v);
-        } else {
-          const double x = static_cast<double>(xi[g]);
-          const double vx = static_cast<double>(static_cast<TO>(v));
-          acc_ff += vx * wth * (x * x - 1.0);
-          df[g] = static_cast<TO>(2.0 * rho0 * x * vx);
+      for (int e = lane; e < kWaveRows * W; e += 64) {
+        const int rl = e / W, jk = e - rl * W;
+        if (r0 + rl < N) {
+          const long g = base + static_cast<long>(rl) * A::kNo + c0 + jk;
+          const TC v = stage[e];
+          if (xi == nullptr) {
+            o[g] = static_cast<TO>(v);
+          } else {
+            const double x = static_cast<double>(xi[g]);
+            const double vx = static_cast<double>(static_cast<TO>(v));
+            acc_ff += vx * wth * (x * x - 1.0);
+            df[g] = static_cast<TO>(2.0 * rho0 * x * vx);
+          }
         }
       }
     }
-  }
+  };
+  chunk(std::integral_constant<int, 0>{});
+  if constexpr (kChunks > 1) chunk(std::integral_constant<int, 1>{});
   if (xi != nullptr) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) acc_ff += __shfl_xor(acc_ff, off);
