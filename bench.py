@@ -58,10 +58,11 @@ MFMA_SUMMARY = os.path.join(ROOT, "profiles", "a2p_gemm_64c_mmax5_mfma.md")
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16, vendor figure
 A2P_ALGO_FLOP = 4 * 88 * 330 * 64 ** 3  # dense [N x no] . [no x np] complex contraction, one direction
 # the solve leg: 20 sites within 5 A of the centre, charge scale 0.1, V_ext
-# capped at 100 kT, L-BFGS with 3 pairs to max |h0 g| < 1e-5 (3 pairs take
-# as many iterations as 5 here, 80-91, and move 4 fewer pair fields per pass)
+# capped at 100 kT, L-BFGS with 1 pair to max |h0 g| < 1e-5 (with the h0
+# preconditioner one pair takes as many iterations as 3 or 5 here, 81-89, and
+# each pass moves 4 fewer pair fields than with 3)
 SOLVE_SITES, SOLVE_QSCALE, SOLVE_VMAX_KT = 20, 0.1, 100.0
-SOLVE_M, SOLVE_GTOL, SOLVE_MAX_ITER = 3, 1e-5, 400
+SOLVE_M, SOLVE_GTOL, SOLVE_MAX_ITER = 1, 1e-5, 400
 # the mixed-size batch: slots per shape and the device-memory budget
 SERVE_CAPACITY, SERVE_BUDGET = 4, 200e9
 

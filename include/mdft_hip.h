@@ -30,8 +30,9 @@
  *   MDFT_MIXED_X6F the arithmetic of MDFT_MIXED_X6 on float fields: o, xi
  *                  and df are float (FP32 copies of the solvent fields,
  *                  half their HBM bytes); delta_rho, the get_delta_f products
- *                  and every reduction stay FP64.  Evaluation only (the
- *                  functional and the server take double fields)
+ *                  and every reduction stay FP64.  The functional and the
+ *                  server keep a double iterate and evaluate the excess on
+ *                  its FP32 copy
  */
 #ifndef MDFT_HIP_H
 #define MDFT_HIP_H
@@ -262,9 +263,10 @@ int mdft_vext(int nx, int ny, int nz, double lx, double ly, double lz, int mmax,
  *   F_exc = (dv / N) ff              (ff of mdft_energy_cproj)
  *   g     = dv w (2 rho0 xi (kT ln xi^2 + V) + (2 prefactor / N) df)
  * dv = lx ly lz / N.  The functional takes its grid, rho0, prefactor and
- * precision form from the solver (MDFT_FP64, MDFT_MIXED or MDFT_MIXED_X6: xi
- * and g are doubles); vext (device, N x no) is borrowed and must outlive the
- * handle.  F (host) receives F, F_id, F_ext, F_exc.  Both calls synchronise. */
+ * precision form from the solver (any form but MDFT_FP32; xi and g are
+ * doubles, and MDFT_MIXED_X6F evaluates the excess on an FP32 copy of xi and
+ * reads its df in FP32); vext (device, N x no) is borrowed and must outlive
+ * the handle.  F (host) receives F, F_id, F_ext, F_exc.  Both calls synchronise. */
 typedef struct mdft_functional_s* mdft_functional_t;
 int mdft_functional_create(mdft_solver_t solver, const double* vext, double kT, mdft_functional_t* out);
 int mdft_functional_destroy(mdft_functional_t h);
@@ -285,7 +287,9 @@ int mdft_functional_eval(mdft_functional_t h, const double* xi, double* g, doubl
  * functional evaluations, status (0 converged, 1 max_iter reached, 2 no
  * acceptable point within 30 trials, 4 F or g not finite at the start point:
  * NaN or Inf in xi or V_ext) and max |h0 g|.  The handle keeps its
- * work space (3 N no doubles and 2 (m + 1) N no pair values) between calls. */
+ * work space between calls: N no doubles (x_t) and 2 (m + 2) N no values of
+ * the pair type (g, g_t and the pairs), besides the evaluation's df (N no
+ * doubles, or for MDFT_MIXED_X6F N no floats and the FP32 copy of xi). */
 int mdft_lbfgs_minimize(mdft_functional_t h, double* xi, int m, int max_iter, double gtol, double* F,
                         int* iterations, int* evaluations, int* status, double* residual, double* history,
                         void* stream);
