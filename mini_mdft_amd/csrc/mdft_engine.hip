@@ -80,25 +80,25 @@ __global__ __launch_bounds__(kFThreads) void slot_direction_kernel(const SlotDes
   float* xf = F32F ? xfslab + static_cast<long>(blockIdx.y) * e.n : nullptr;
   if (a.mode == kModeStart) {
     MDFT_ELEMENT_LOOP(e) {
-      const double xv = a.src[i];
-      xt[i] = xv;
-      if (F32F) xf[i] = static_cast<float>(xv);
+      const double xv = ld_nt(&a.src[i]);
+      st_nt(&xt[i], xv);
+      if (F32F) st_nt(&xf[i], static_cast<float>(xv));
     }
     return;
   }
   MDFT_ELEMENT_LOOP(e) {
-    double q = a.cg * static_cast<double>(a.g[i]);
+    double q = a.cg * static_cast<double>(ld_nt(&a.g[i]));
 #pragma unroll
-    for (int j = 0; j < KM; ++j) q = fma(a.cq[j], static_cast<double>(a.y[j][i]), q);
+    for (int j = 0; j < KM; ++j) q = fma(a.cq[j], static_cast<double>(ld_nt(&a.y[j][i])), q);
     double r = a.gamma * e.h0[io] * q;
 #pragma unroll
-    for (int j = 0; j < KM; ++j) r = fma(a.cs[j], static_cast<double>(a.s[j][i]), r);
-    const double xv = a.src[i];
-    if (a.save) a.save[i] = xv;
+    for (int j = 0; j < KM; ++j) r = fma(a.cs[j], static_cast<double>(ld_nt(&a.s[j][i])), r);
+    const double xv = ld_nt(&a.src[i]);
+    if (a.save) st_nt(&a.save[i], xv);
     const double xn = xv + a.t * (-r);
-    xt[i] = xn;
-    if (F32F) xf[i] = static_cast<float>(xn);
-    a.sc[i] = static_cast<TS>(xn - xv);
+    st_nt(&xt[i], xn);
+    if (F32F) st_nt(&xf[i], static_cast<float>(xn));
+    st_nt(&a.sc[i], static_cast<TS>(xn - xv));
   }
 }
 
@@ -122,10 +122,10 @@ __global__ __launch_bounds__(kFThreads) void slot_update_kernel(const SlotDesc<T
   for (int k = 0; k < NA; ++k) acc[k] = 0.0;
   if (a.mode == kModeStart) {
     MDFT_ELEMENT_LOOP(e) {
-      const double xv = xt[i], v = a.V[i], wio = e.w[io], h = e.h0[io];
+      const double xv = ld_nt(&xt[i]), v = ld_nt(&a.V[i]), wio = e.w[io], h = e.h0[io];
       local_f(xv, v, wio, acc[5 * KM + 7], acc[5 * KM + 8]);
-      const TS gs = static_cast<TS>(local_grad(xv, v, static_cast<double>(df[i]), wio, e));
-      a.g[i] = gs;
+      const TS gs = static_cast<TS>(local_grad(xv, v, static_cast<double>(ld_nt(&df[i])), wio, e));
+      st_nt(&a.g[i], gs);
       const double gv = static_cast<double>(gs);
       const double hg = h * gv;
       acc[5 * KM + 6] += gv * hg;
@@ -134,20 +134,20 @@ __global__ __launch_bounds__(kFThreads) void slot_update_kernel(const SlotDesc<T
   } else {
     MDFT_ELEMENT_LOOP(e) {
       const double wio = e.w[io], h = e.h0[io];
-      const double xv = xt[i], v = a.V[i];
+      const double xv = ld_nt(&xt[i]), v = ld_nt(&a.V[i]);
       local_f(xv, v, wio, acc[5 * KM + 7], acc[5 * KM + 8]);
-      const TS gs = static_cast<TS>(local_grad(xv, v, static_cast<double>(df[i]), wio, e));
+      const TS gs = static_cast<TS>(local_grad(xv, v, static_cast<double>(ld_nt(&df[i])), wio, e));
       const double gn = static_cast<double>(gs);
-      const TS ys = static_cast<TS>(gn - static_cast<double>(a.g[i]));
-      a.gt[i] = gs;
-      a.yn[i] = ys;
+      const TS ys = static_cast<TS>(gn - static_cast<double>(ld_nt(&a.g[i])));
+      st_nt(&a.gt[i], gs);
+      st_nt(&a.yn[i], ys);
       const double yv = static_cast<double>(ys);
-      const double scv = static_cast<double>(a.sc[i]);
+      const double scv = static_cast<double>(ld_nt(&a.sc[i]));
       double sj[KA], yj[KA];
 #pragma unroll
       for (int j = 0; j < KM; ++j) {
-        sj[j] = static_cast<double>(a.s[j][i]);
-        yj[j] = static_cast<double>(a.y[j][i]);
+        sj[j] = static_cast<double>(ld_nt(&a.s[j][i]));
+        yj[j] = static_cast<double>(ld_nt(&a.y[j][i]));
       }
       const double hy = h * yv, hg = h * gn;
 #pragma unroll

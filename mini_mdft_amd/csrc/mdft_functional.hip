@@ -46,10 +46,10 @@ __global__ __launch_bounds__(kFThreads) void local_kernel(const double* __restri
                                                           Elem e, double* __restrict__ part, int nb) {
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   MDFT_ELEMENT_LOOP(e) {
-    const double xv = x[i], v = V[i], wio = e.w[io], h = e.h0[io];
+    const double xv = ld_nt(&x[i]), v = ld_nt(&V[i]), wio = e.w[io], h = e.h0[io];
     local_f(xv, v, wio, acc[0], acc[1]);
-    const TG gs = static_cast<TG>(local_grad(xv, v, static_cast<double>(df[i]), wio, e));
-    g[i] = gs;
+    const TG gs = static_cast<TG>(local_grad(xv, v, static_cast<double>(ld_nt(&df[i])), wio, e));
+    st_nt(&g[i], gs);
     const double gv = static_cast<double>(gs);
     const double hg = h * gv;
     acc[2] += gv * hg;
@@ -87,17 +87,17 @@ __global__ __launch_bounds__(kFThreads) void to_float_kernel(const double* __res
 template <typename TS, int K>
 __global__ __launch_bounds__(kFThreads) void direction_kernel(DirArgs<TS> a, Elem e) {
   MDFT_ELEMENT_LOOP(e) {
-    double q = a.cg * static_cast<double>(a.g[i]);
+    double q = a.cg * static_cast<double>(ld_nt(&a.g[i]));
 #pragma unroll
-    for (int j = 0; j < K; ++j) q = fma(a.cq[j], static_cast<double>(a.y[j][i]), q);
+    for (int j = 0; j < K; ++j) q = fma(a.cq[j], static_cast<double>(ld_nt(&a.y[j][i])), q);
     double r = a.gamma * e.h0[io] * q;
 #pragma unroll
-    for (int j = 0; j < K; ++j) r = fma(a.cs[j], static_cast<double>(a.s[j][i]), r);
-    const double xv = a.x[i];
+    for (int j = 0; j < K; ++j) r = fma(a.cs[j], static_cast<double>(ld_nt(&a.s[j][i])), r);
+    const double xv = ld_nt(&a.x[i]);
     const double xn = xv + a.t * (-r);
-    a.xt[i] = xn;
-    if (a.xf) a.xf[i] = static_cast<float>(xn);
-    a.sc[i] = static_cast<TS>(xn - xv);
+    st_nt(&a.xt[i], xn);
+    if (a.xf) st_nt(&a.xf[i], static_cast<float>(xn));
+    st_nt(&a.sc[i], static_cast<TS>(xn - xv));
   }
 }
 
@@ -131,20 +131,20 @@ __global__ __launch_bounds__(kFThreads) void update_kernel(UpdArgs<TS, TF> a, El
   for (int k = 0; k < NA; ++k) acc[k] = 0.0;
   MDFT_ELEMENT_LOOP(e) {
     const double wio = e.w[io], h = e.h0[io];
-    const double xv = a.xt[i], v = a.V[i];
+    const double xv = ld_nt(&a.xt[i]), v = ld_nt(&a.V[i]);
     local_f(xv, v, wio, acc[5 * K + 7], acc[5 * K + 8]);
-    const TS gs = static_cast<TS>(local_grad(xv, v, static_cast<double>(a.df[i]), wio, e));
+    const TS gs = static_cast<TS>(local_grad(xv, v, static_cast<double>(ld_nt(&a.df[i])), wio, e));
     const double gn = static_cast<double>(gs);
-    const TS ys = static_cast<TS>(gn - static_cast<double>(a.g[i]));
-    a.gt[i] = gs;
-    a.yn[i] = ys;
+    const TS ys = static_cast<TS>(gn - static_cast<double>(ld_nt(&a.g[i])));
+    st_nt(&a.gt[i], gs);
+    st_nt(&a.yn[i], ys);
     const double yv = static_cast<double>(ys);
-    const double scv = static_cast<double>(a.sc[i]);
+    const double scv = static_cast<double>(ld_nt(&a.sc[i]));
     double sj[KA], yj[KA];
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-      sj[j] = static_cast<double>(a.s[j][i]);
-      yj[j] = static_cast<double>(a.y[j][i]);
+      sj[j] = static_cast<double>(ld_nt(&a.s[j][i]));
+      yj[j] = static_cast<double>(ld_nt(&a.y[j][i]));
     }
     const double hy = h * yv, hg = h * gn;
 #pragma unroll
