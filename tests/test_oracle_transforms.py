@@ -72,6 +72,27 @@ def test_angl2proj(oracle, mmax, mrso, n):
     assert np.abs(pc[0] - 2.5).max() < 1e-14 and np.abs(pc[1:]).max() < 1e-14
 
 
+@pytest.mark.parametrize("mmax,mrso", [(3, 2), (5, 2), (4, 1), (6, 3), (10, 10)])
+def test_angl2proj_real_field_symmetry(oracle, mmax, mrso):
+    """A real field's mu = 0 projections pair up: R^m_{-mup,-mu} = (-1)^(mup-mu)
+    conj(R^m_{mup,mu}) gives p^m_{-mup,0} = (-1)^mup conj(p^m_{mup,0}) (at mmax 5,
+    mrso 2, 15 of the 88 channels).  A dropped sign, a swapped mup or a wrong
+    conjugation in the phi quadrature breaks it; the identity without the sign
+    must fail, so the check discriminates."""
+    T = oracle.angular_tables(mmax, mrso)
+    n = (2, 3, 2)
+    N = int(np.prod(n))
+    o = np.random.default_rng(11).uniform(-1, 1, (N, T["ntheta"], T["nphi"], T["npsi"]))
+    p = oracle.angl2proj(o, n, mmax, mrso).reshape(T["np"], N)
+    idx = {(int(a), int(b), int(c)): i for i, (a, b, c) in enumerate(zip(T["m"], T["mup"], T["mu"]))}
+    pairs = [(i, idx[(m, -mup, 0)], mup) for (m, mup, mu), i in idx.items() if mu == 0 and mup < 0]
+    assert len(pairs) == mmax * (mmax + 1) // 2
+    scale = np.abs(p).max()
+    for i, j, mup in pairs:
+        assert np.abs(p[i] - (-1) ** mup * np.conj(p[j])).max() < 1e-14 * scale
+    assert max(np.abs(p[i] - np.conj(p[j])).max() for i, j, mup in pairs if mup % 2) > 1e-3 * scale
+
+
 @pytest.mark.parametrize("mmax,mrso,n", [(3, 2, (2, 3, 4)), (5, 2, (2, 2, 3)), (4, 1, (3, 1, 2)), (6, 4, (2, 1, 2))])
 def test_proj2angl(oracle, mmax, mrso, n):
     T = oracle.angular_tables(mmax, mrso)
