@@ -228,6 +228,21 @@ int mdft_solver_set_timing(mdft_solver_t h, int enable);
 int mdft_solver_set_mean_fp64(mdft_solver_t h, int on);
 int mdft_solver_stage_ms(mdft_solver_t h, double* ms, long* count);
 
+/* No counterpart in the reference (its path stores every projection).  The
+ * compact channel order of the fused evaluation: the fields are real, so a
+ * mu = 0 projection of delta_rho with mup < 0 equals (-1)^mup conj of its
+ * (m, -mup, 0) partner, before the FFT and (at -q) after it.  In this order
+ * angl2proj writes only the other projections (73 of 88 at mmax 5, mrso 2)
+ * and the forward FFT transforms those; the OZ pass reads an implied
+ * projection from its partner at the mirrored point and writes all outputs
+ * (the OZ output obeys the symmetry only for a c(q) that does, which is not
+ * assumed), which the inverse FFT and proj2angl take in full.  On by default
+ * for MDFT_MIXED_X6 and MDFT_MIXED_X6F where the grid has the compile-time
+ * proj2angl and OZ kernels; results change by rounding only.
+ * mdft_solver_compact reports whether it is on. */
+int mdft_solver_set_compact(mdft_solver_t h, int on);
+int mdft_solver_compact(mdft_solver_t h, int* on);
+
 /* ---- utilities --------------------------------------------------------------
  * Counter-based uniforms: x_i = lo + (hi - lo) u_i with u_i the splitmix64
  * stream of (seed, offset + i) (same definition as mini_mdft_amd/rng.py). */

@@ -62,6 +62,8 @@ _SIGS = {
     "mdft_energy_cproj_async": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "mdft_solver_set_timing": (_i, [_vp, _i]),
     "mdft_solver_set_mean_fp64": (_i, [_vp, _i]),
+    "mdft_solver_set_compact": (_i, [_vp, _i]),
+    "mdft_solver_compact": (_i, [_vp, ctypes.POINTER(_i)]),
     "mdft_solver_stage_ms": (_i, [_vp, _pd, ctypes.POINTER(ctypes.c_long)]),
     "mdft_fill_uniform": (_i, [_vp, _sz, _ull, _ull, _d, _d, _i, _vp]),
     "mdft_solver_create_batch": (_i, [_vp, _d, _d, _i, _i, ctypes.POINTER(_vp)]),

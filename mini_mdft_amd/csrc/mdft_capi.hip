@@ -47,9 +47,10 @@ bool is_mixed(int precision) {
   return precision == MDFT_MIXED || precision == MDFT_MIXED_X6 || precision == MDFT_MIXED_X6F;
 }
 
-std::unique_ptr<ProjGemm> make_gemm(const AngularGrid& g, int precision) {
+std::unique_ptr<ProjGemm> make_gemm(const AngularGrid& g, int precision, bool compact = false) {
   if (!is_mixed(precision)) return nullptr;
-  return precision == MDFT_MIXED ? std::make_unique<ProjGemm>(g, 2, 2, true) : std::make_unique<ProjGemm>(g, 3, 3, false);
+  return precision == MDFT_MIXED ? std::make_unique<ProjGemm>(g, 2, 2, true)
+                                 : std::make_unique<ProjGemm>(g, 3, 3, false, compact);
 }
 
 void check_dtype(int dtype) { throw_if(dtype != MDFT_DOUBLE && dtype != MDFT_FLOAT, "unknown element type"); }
@@ -84,7 +85,12 @@ struct mdft_solver_s {
   int field_parts = 0;    // partial sums per field
   std::unique_ptr<ProjTables> tab;
   std::unique_ptr<ProjGemm> gemm;
-  void* pbuf = nullptr;  // projections, np x N complex
+  // the fused evaluation in the compact channel order (the implied mu = 0,
+  // mup < 0 projections of delta_rho neither written nor transformed
+  // forward; MDFT_MIXED_X6 and MDFT_MIXED_X6F where the grid allows, on by
+  // default)
+  bool compact = false;
+  void* pbuf = nullptr;  // projections, np x N complex (compact: the first channels)
   double* partials = nullptr;
   int nparts = 0;
   double* ffdev = nullptr;
@@ -108,6 +114,14 @@ struct mdft_solver_s {
 };
 
 namespace {
+
+// whether the fused evaluation of this form can take the compact channel
+// order: the BF16x6 GEMM angl2proj, the compile-time FP32 proj2angl and the
+// FP32 star OZ kernel
+bool compact_available(int precision, const mdft_conv_s* conv) {
+  if (precision != MDFT_MIXED_X6 && precision != MDFT_MIXED_X6F) return false;
+  return ProjGemm::compact_possible(conv->dev->tab.ang, false) && conv->dev->compact_supported();
+}
 
 // whether this solver's proj2angl can write the column sums of the mean mode
 bool mean_mode_supported(const mdft_solver_s* h) {
@@ -554,7 +568,8 @@ int mdft_solver_create_batch(mdft_conv_t conv, double rho0, double prefactor, in
     h->N = static_cast<long>(t.nx) * t.ny * t.nz;
     throw_if(capacity > 1 && h->N % 64 != 0, "batched evaluation needs nx ny nz to be a multiple of 64");
     h->tab = std::make_unique<ProjTables>(t.ang);
-    h->gemm = make_gemm(t.ang, precision);
+    h->compact = compact_available(precision, conv);
+    h->gemm = make_gemm(t.ang, precision, h->compact);
     h->field_parts = h->gemm ? h->gemm->p2a_blocks(h->N) : proj2angl_blocks(*h->tab, h->N);
     const size_t pbytes = static_cast<size_t>(h->N) * t.ang.np * capacity * (precision == MDFT_FP64 ? 16 : 8);
     h->nparts = h->field_parts * capacity;
@@ -623,7 +638,7 @@ static void energy_impl(mdft_solver_t h, int nb, const void* xi, void* df, doubl
     case MDFT_MIXED_X6F: {
       auto* p = static_cast<cplx<float>*>(h->pbuf);
       h->gemm->angl2proj(Nb, static_cast<const float*>(xi), p, &rho0, s);
-      h->conv->dev->execute<float>(p, eps, s, tm, nb);
+      h->conv->dev->execute<float>(p, eps, s, tm, nb, h->compact);
       h->gemm->proj2angl(Nb, p, nullptr, static_cast<const float*>(xi), static_cast<float*>(df), rho0, h->partials,
                          s, colp);
       break;
@@ -631,7 +646,7 @@ static void energy_impl(mdft_solver_t h, int nb, const void* xi, void* df, doubl
     default: {
       auto* p = static_cast<cplx<float>*>(h->pbuf);
       h->gemm->angl2proj(Nb, static_cast<const double*>(xi), p, &rho0, s);
-      h->conv->dev->execute<float>(p, eps, s, tm, nb);
+      h->conv->dev->execute<float>(p, eps, s, tm, nb, h->compact);
       h->gemm->proj2angl(Nb, p, nullptr, static_cast<const double*>(xi), static_cast<double*>(df), rho0, h->partials,
                          s, colp);
       break;
@@ -641,6 +656,23 @@ static void energy_impl(mdft_solver_t h, int nb, const void* xi, void* df, doubl
   reduce_partials(h->partials, h->field_parts, rho0 * h->prefactor, ffd, s, nb, ffstride, colp ? h->extra : nullptr,
                   colp ? h->mean->extra_per_field() : 0);
   if (tm) tm->mark(s, kStageEnd);
+}
+
+int mdft_solver_set_compact(mdft_solver_t h, int on) {
+  return guarded([&] {
+    throw_if(!h, "null handle");
+    if (on) throw_if(!compact_available(h->precision, h->conv), "solver: no compact channel order for this form and grid");
+    if ((on != 0) == h->compact) return;
+    h->compact = on != 0;
+    h->gemm = make_gemm(h->conv->dev->tab.ang, h->precision, h->compact);
+  });
+}
+
+int mdft_solver_compact(mdft_solver_t h, int* on) {
+  return guarded([&] {
+    throw_if(!h || !on, "null argument");
+    *on = h->compact ? 1 : 0;
+  });
 }
 
 int mdft_solver_set_mean_fp64(mdft_solver_t h, int on) {

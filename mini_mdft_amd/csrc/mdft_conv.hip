@@ -22,6 +22,7 @@
 // the block -> pair map gives every XCD a contiguous range of pairs so that
 // neighbouring pairs share L2 lines.
 #include "mdft_conv.hpp"
+#include "mdft_proj.hpp"
 
 #include <hip/hip_ext.h>
 
@@ -703,14 +704,14 @@ __global__ __launch_bounds__(256) void oz_pairs_ct_kernel(cplx<T>* __restrict__ 
 // pairs of consecutive stars lie on the same z lines at neighbouring +-iz, and
 // a line's cache lines serve the blocks of consecutive stars.  Blocks past the
 // last star (the XCD order pads the grid) request nothing.
-template <typename T, int MMAX, int MRSO, int THREADS>
+template <typename T, int MMAX, int MRSO, int THREADS, bool CP>
 __global__ __launch_bounds__(THREADS) void oz_star_ct_kernel(cplx<T>* __restrict__ p, const T* __restrict__ dtab,
                                                              const OzPairMeta<T>* __restrict__ meta,
                                                              const cplx<T>* __restrict__ phase,
                                                              const int* __restrict__ dmap,
                                                              const int* __restrict__ dent,
                                                              const int4* __restrict__ stars, long nstars,
-                                                             OzArgs<T> a) {
+                                                             OzArgs<T> a, const int* __restrict__ ccode) {
   using S = OzShape<MMAX, MRSO>;
   using Lay = StarLayout<T, MMAX, MRSO>;
   constexpr int W = S::W, NP = S::NP, NC = oz_terms<MMAX, MRSO>();
@@ -788,6 +789,11 @@ __global__ __launch_bounds__(THREADS) void oz_star_ct_kernel(cplx<T>* __restrict
   T g0x = 0, g0y = 0, g1x = 0, g1y = 0;
   long sq = 0, smq = 0;
   bool singular = false;
+  // the projection's channels in p; in the compact order an implied
+  // projection reads its partner's channel at the mirrored point, conjugated
+  // and times (-1)^mup, and writes its own output channel after the stored ones
+  long chan = ip, ochan = ip;
+  bool implied = false;
   if (active) {
     // this pair's phase and its projection's info
     const int rec = reinterpret_cast<const int4*>(dmap)[ip].w;
@@ -801,12 +807,27 @@ __global__ __launch_bounds__(THREADS) void oz_star_ct_kernel(cplx<T>* __restrict
     smq = (static_cast<long>(ixm) * a.ny + iym) * a.nz + izm;
     singular = (ixm == ix && iym == iy && izm == iz) && (ix == a.nx / 2 || iy == a.ny / 2 || iz == a.nz / 2);
     // gathers (:420-425)
-    const cplx<T> u = p[static_cast<long>(ip) * a.pstride + sq], v = p[static_cast<long>(ip) * a.pstride + smq];
+    bool odd = false;
+    if constexpr (CP) {
+      const int cc = ccode[ip];
+      chan = cc & 0xff;
+      ochan = (cc >> 8) & 0xff;
+      implied = (cc & kCompactImplied) != 0;
+      odd = (cc & kCompactOdd) != 0;
+    }
+    const long c0 = chan * a.pstride;
+    const cplx<T> u = p[c0 + (implied ? smq : sq)], v = p[c0 + (implied ? sq : smq)];
     info = rec;
     g0x = u.x;
     g0y = u.y;
     g1x = v.x;
     g1y = v.y;
+    if (implied) {
+      g0x = odd ? -g0x : g0x;
+      g0y = odd ? g0y : -g0y;
+      g1x = odd ? -g1x : g1x;
+      g1y = odd ? g1y : -g1y;
+    }
     if (ip < W) ph[ip] = ip < MMAX ? cconj<T>(phv) : phv;
     // zero G and Gm (the padding past 2m+1 must stay zero)
     for (int i = ip; i < 2 * S::NG * W; i += NP) G[i] = mkc<T>(0, 0);
@@ -936,8 +957,8 @@ __global__ __launch_bounds__(THREADS) void oz_star_ct_kernel(cplx<T>* __restrict
     const cplx<T> o0 = cmul<T>(mkc<T>(ar, ai), cconj<T>(e));
     cplx<T> o1 = cmul<T>(mkc<T>(br, bi), cconj<T>(e));
     if (m & 1) o1 = mkc<T>(-o1.x, -o1.y);
-    p[static_cast<long>(ip) * a.pstride + sq] = o0;
-    p[static_cast<long>(ip) * a.pstride + smq] = singular ? cconj<T>(o1) : o1;
+    p[ochan * a.pstride + sq] = o0;
+    p[ochan * a.pstride + smq] = singular ? cconj<T>(o1) : o1;
   }
 }
 
@@ -1303,7 +1324,8 @@ ConvDevice::~ConvDevice() {
   for (void* q : {static_cast<void*>(kxyz), static_cast<void*>(pairs), static_cast<void*>(ia_start),
                   static_cast<void*>(ipinfo), static_cast<void*>(ctab64), static_cast<void*>(ctab32),
                   static_cast<void*>(rcoef), dtab64, dtab32, pmeta32, pphase32, pmeta64, pphase64,
-                  static_cast<void*>(dmap), static_cast<void*>(stars), static_cast<void*>(dent)})
+                  static_cast<void*>(dmap), static_cast<void*>(stars), static_cast<void*>(dent),
+                  static_cast<void*>(ccode)})
     if (q) (void)hipFree(q);
 }
 
@@ -1412,7 +1434,9 @@ void launch_oz(K kernel, unsigned nb, unsigned threads, size_t lds, hipStream_t 
 
 template <typename T>
 void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch, hipEvent_t ev0,
-                         hipEvent_t ev1) const {
+                         hipEvent_t ev1, bool compact) const {
+  throw_if(compact && (!std::is_same<T, float>::value || !compact_supported()),
+           "OZ pass: the compact channel order needs FP32 projections and the star kernel");
   if (npairs == 0) {
     if (ev0) MDFT_HIP_CHECK(hipEventRecord(ev0, s));
     if (ev1) MDFT_HIP_CHECK(hipEventRecord(ev1, s));
@@ -1447,9 +1471,18 @@ void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch,
         const long nb = ag.chunk * kXcds * nbatch;
         throw_if(nb > 0x7fffffffL, "too many q pairs");
         if (!dent) dent = upload(oz_dtable_places(MMAX));
-        launch_oz(oz_star_ct_kernel<T, MMAX, MRSO, Lay::threads>, static_cast<unsigned>(nb),
+        if constexpr (std::is_same<T, float>::value) {
+          if (compact) {
+            if (!ccode) ccode = upload(compact_codes(tab.ang));
+            launch_oz(oz_star_ct_kernel<T, MMAX, MRSO, Lay::threads, true>, static_cast<unsigned>(nb),
+                      static_cast<unsigned>(Lay::threads), Lay::dynamic, s, ev0, ev1, p, dt, mt, ph, dmap, dent,
+                      reinterpret_cast<const int4*>(stars), nstars, ag, static_cast<const int*>(ccode));
+            return;
+          }
+        }
+        launch_oz(oz_star_ct_kernel<T, MMAX, MRSO, Lay::threads, false>, static_cast<unsigned>(nb),
                   static_cast<unsigned>(Lay::threads), Lay::dynamic, s, ev0, ev1, p, dt, mt, ph, dmap, dent,
-                  reinterpret_cast<const int4*>(stars), nstars, ag);
+                  reinterpret_cast<const int4*>(stars), nstars, ag, static_cast<const int*>(nullptr));
         return;
       }
       OzArgs<T> ag = a;
@@ -1470,24 +1503,38 @@ void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch,
 }
 
 template <typename T>
-void ConvDevice::execute(cplx<T>* p, double chop_eps, hipStream_t s, StageTimer* timer, int nbatch) const {
+void ConvDevice::execute(cplx<T>* p, double chop_eps, hipStream_t s, StageTimer* timer, int nbatch,
+                         bool compact) const {
+  // the compact order transforms the stored projections forward, and every
+  // projection of the OZ output backward
   const long batch = static_cast<long>(tab.ang.np) * nbatch;
+  const long fbatch = static_cast<long>(compact ? compact_projections(tab.ang) : tab.ang.np) * nbatch;
   if (!fft.tuned<T>()) fft.autotune<T>(batch, s);  // once per element type, before any timing mark
   if (timer) timer->mark(s, kStageConvStart);
-  fft.execute<T>(p, batch, -1, s);
+  fft.execute<T>(p, fbatch, -1, s);
   if (timer) timer->mark(s, kStageFftFwd);
   if (timer && timer->oz_only())
-    oz_pass<T>(p, chop_eps, s, nbatch, timer->event(kStageFftFwd), timer->event(kStageOz));
+    oz_pass<T>(p, chop_eps, s, nbatch, timer->event(kStageFftFwd), timer->event(kStageOz), compact);
   else
-    oz_pass<T>(p, chop_eps, s, nbatch);
+    oz_pass<T>(p, chop_eps, s, nbatch, nullptr, nullptr, compact);
   if (timer) timer->mark(s, kStageOz);
   fft.execute<T>(p, batch, +1, s);
   if (timer) timer->mark(s, kStageFftBwd);
 }
 
-template void ConvDevice::oz_pass<double>(cplx<double>*, double, hipStream_t, int, hipEvent_t, hipEvent_t) const;
-template void ConvDevice::oz_pass<float>(cplx<float>*, double, hipStream_t, int, hipEvent_t, hipEvent_t) const;
-template void ConvDevice::execute<double>(cplx<double>*, double, hipStream_t, StageTimer*, int) const;
-template void ConvDevice::execute<float>(cplx<float>*, double, hipStream_t, StageTimer*, int) const;
+template void ConvDevice::oz_pass<double>(cplx<double>*, double, hipStream_t, int, hipEvent_t, hipEvent_t,
+                                          bool) const;
+template void ConvDevice::oz_pass<float>(cplx<float>*, double, hipStream_t, int, hipEvent_t, hipEvent_t, bool) const;
+template void ConvDevice::execute<double>(cplx<double>*, double, hipStream_t, StageTimer*, int, bool) const;
+template void ConvDevice::execute<float>(cplx<float>*, double, hipStream_t, StageTimer*, int, bool) const;
+
+bool ConvDevice::compact_supported() const {
+  if (npairs == 0 || tab.ang.np > 4 * kWave || !padw) return false;
+  bool ok = false;
+  dispatch_oz(tab.mmax, tab.mrso, [&](auto mm, auto mr) {
+    ok = StarLayout<float, decltype(mm)::value, decltype(mr)::value>::fits;
+  });
+  return ok;
+}
 
 }  // namespace mdft

@@ -49,6 +49,7 @@ struct ConvDevice {
   // table holds one row per star)
   bool star32 = false, star64 = false;
   mutable int* dent = nullptr;  // LDS places codes of the half d table's entries (star kernel)
+  mutable int* ccode = nullptr;  // per projection: its compact channel (compact_codes)
   // row width of the padded d layout when the compile-time OZ kernel serves
   // this (mmax, mrso) (2 mmax + 1), else 0 (compact rows, generic kernel)
   int padw = 0;
@@ -57,13 +58,18 @@ struct ConvDevice {
   // gamma components below chop_eps in magnitude are set to 0.  With
   // nbatch > 1, p holds nbatch fields as (np, nbatch, nx, ny, nz).  With
   // events, the launch records its own start and end times in ev0 and ev1.
+  // With compact, p holds the compact channel order (compact_projections
+  // channels; FP32 and the star kernel only, compact_supported).
   template <typename T>
   void oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch = 1, hipEvent_t ev0 = nullptr,
-               hipEvent_t ev1 = nullptr) const;
+               hipEvent_t ev1 = nullptr, bool compact = false) const;
   // FFT, oz_pass, inverse FFT: in place on (np, nx, ny, nz) (or (np, nbatch,
   // nx, ny, nz)); marks the FFT / OZ stage boundaries on `timer` when given
   template <typename T>
-  void execute(cplx<T>* p, double chop_eps, hipStream_t s, StageTimer* timer = nullptr, int nbatch = 1) const;
+  void execute(cplx<T>* p, double chop_eps, hipStream_t s, StageTimer* timer = nullptr, int nbatch = 1,
+               bool compact = false) const;
+  // whether oz_pass takes the compact channel order for FP32 projections
+  bool compact_supported() const;
 };
 
 // (ix, iy, iz) triples of the reference's (q, -q) list with the same q / -q

@@ -508,7 +508,8 @@ void projection_matrices(const AngularGrid& g, std::vector<double>& wa, std::vec
 struct ProjGemm::Impl {
   int px, pw;
   bool p2a_mfma;  // proj2angl on the matrix cores (BF16x3) or the FP32 VALU kernel
-  int no, np;
+  bool compact = false;
+  int no, np;  // np: stored projections (compact: the compact count)
   int a_mt, a_kt, p_mt, p_kt;      // tile buckets and k-steps
   bool a2p_fits, p2a_fits;         // grid within the MFMA register tiles
   std::unique_ptr<ProjTables> fb;  // factorised FP32 projections
@@ -524,7 +525,7 @@ struct ProjGemm::Impl {
   }
 };
 
-ProjGemm::ProjGemm(const AngularGrid& g, int data_parts, int weight_parts, bool p2a_mfma)
+ProjGemm::ProjGemm(const AngularGrid& g, int data_parts, int weight_parts, bool p2a_mfma, bool compact)
     : impl_(std::make_unique<Impl>()) {
   dispatch_split(data_parts, weight_parts, [](auto, auto) {});  // rejects pairs without kernels
   throw_if(p2a_mfma && (data_parts != 2 || weight_parts != 2), "proj2angl on MFMA is compiled for BF16x3 only");
@@ -533,8 +534,9 @@ ProjGemm::ProjGemm(const AngularGrid& g, int data_parts, int weight_parts, bool 
   I.pw = weight_parts;
   I.p2a_mfma = p2a_mfma;
   I.no = g.no;
-  I.np = g.np;
-  const int nps = g.npsi, np = g.np, no = g.no;
+  I.compact = compact;
+  I.np = compact ? compact_projections(g) : g.np;
+  const int nps = g.npsi, np = I.np, no = g.no;
   throw_if(nps % 2 == 0, "npsi must be odd");
   I.a_mt = tile_bucket((2 * np + 15) / 16);
   I.a_kt = (no + kK - 1) / kK;
@@ -543,8 +545,20 @@ ProjGemm::ProjGemm(const AngularGrid& g, int data_parts, int weight_parts, bool 
   I.a2p_fits = I.a_mt <= kMaxA2pTiles && no >= 2;
   I.p2a_fits = I.p_mt <= kMaxP2aTiles;
   if (!I.a2p_gemm() || !I.p2a_gemm()) I.fb = std::make_unique<ProjTables>(g);
+  throw_if(compact && !compact_possible(g, p2a_mfma), "projections: no compact channel order for this grid and form");
   std::vector<double> wa, wp;
   projection_matrices(g, wa, wp);
+  if (compact) {
+    // the angl2proj rows (2 ip + re/im) of the stored projections, in order
+    const std::vector<int> code = compact_codes(g);
+    std::vector<double> wc;
+    wc.reserve(static_cast<size_t>(2 * np) * no);
+    for (int ip = 0; ip < g.np; ++ip)
+      if (!(code[ip] & kCompactImplied))
+        wc.insert(wc.end(), wa.begin() + static_cast<size_t>(2 * ip) * no, wa.begin() + static_cast<size_t>(2 * ip + 2) * no);
+    throw_if(wc.size() != static_cast<size_t>(2 * np) * no, "internal error: compact angl2proj rows");
+    wa.swap(wc);
+  }
   if (I.a2p_gemm())
     dispatch_split(I.px, I.pw, [&](auto, auto pwc) {
       I.wa = upload(swizzle<decltype(pwc)::value>(wa, 2 * np, no, true, I.a_mt));
@@ -606,13 +620,23 @@ int ProjGemm::p2a_blocks(long N) const {
 
 bool ProjGemm::has_column_sums() const { return !impl_->p2a_gemm() && proj2angl_has_column_sums(*impl_->fb); }
 
+bool ProjGemm::compact() const { return impl_->compact; }
+
+bool ProjGemm::compact_possible(const AngularGrid& g, bool p2a_mfma) {
+  // the GEMM angl2proj (register tiles) and the compile-time FP32 proj2angl
+  const int nc = compact_projections(g);
+  if (p2a_mfma || nc == g.np) return false;
+  if (tile_bucket((2 * nc + 15) / 16) > kMaxA2pTiles || g.no < 2) return false;
+  return proj2angl_compiled(g.mmax, g.mrso);
+}
+
 int ProjGemm::proj2angl(long N, const cplx<float>* p, double* o, const double* xi, double* df, double rho0,
                         double* partials, hipStream_t s, double* colpart) {
   if (N == 0) return 0;
   const Impl& I = *impl_;
   throw_if(colpart != nullptr && !has_column_sums(), "proj2angl: no column sums in this form");
   if (!I.p2a_gemm()) {
-    proj2angl_factorized<float, double, float>(*I.fb, N, p, o, xi, df, rho0, partials, s, colpart);
+    proj2angl_factorized<float, double, float>(*I.fb, N, p, o, xi, df, rho0, partials, s, colpart, I.compact);
     return proj2angl_blocks(*I.fb, N);
   }
   const GemmDims d{N, I.no, I.np, I.p_kt};
@@ -631,7 +655,7 @@ int ProjGemm::proj2angl(long N, const cplx<float>* p, float* o, const float* xi,
   const Impl& I = *impl_;
   throw_if(I.p2a_gemm(), "proj2angl: float fields take the FP32 VALU kernel");
   throw_if(colpart != nullptr && !has_column_sums(), "proj2angl: no column sums in this form");
-  proj2angl_factorized<float, float, float>(*I.fb, N, p, o, xi, df, rho0, partials, s, colpart);
+  proj2angl_factorized<float, float, float>(*I.fb, N, p, o, xi, df, rho0, partials, s, colpart, I.compact);
   return proj2angl_blocks(*I.fb, N);
 }
 

@@ -22,7 +22,11 @@ struct ProjGemm {
   // ~2^-26.  p2a_mfma: proj2angl as a BF16x3 GEMM (needs (2, 2)), else the
   // factorised FP32 VALU kernel.  Grids beyond the MFMA register tiles
   // (2 np > 256 or no > 384) run the factorised FP32 projections instead.
-  ProjGemm(const AngularGrid& g, int data_parts, int weight_parts, bool p2a_mfma);
+  // compact: angl2proj writes the stored projections of the compact channel
+  // order (the implied mu = 0, mup < 0 ones left out) and proj2angl reads the
+  // OZ output order (compact_codes); needs the GEMM angl2proj and the
+  // compile-time FP32 proj2angl
+  ProjGemm(const AngularGrid& g, int data_parts, int weight_parts, bool p2a_mfma, bool compact = false);
   ~ProjGemm();
   ProjGemm(const ProjGemm&) = delete;
   ProjGemm& operator=(const ProjGemm&) = delete;
@@ -42,6 +46,9 @@ struct ProjGemm {
                 double* partials, hipStream_t s, double* colpart = nullptr);
   int p2a_blocks(long N) const;
   bool has_column_sums() const;
+  bool compact() const;
+  // whether a ProjGemm of this grid and split can take the compact order
+  static bool compact_possible(const AngularGrid& g, bool p2a_mfma);
 
  private:
   template <typename TI>

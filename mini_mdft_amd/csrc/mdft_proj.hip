@@ -377,7 +377,7 @@ struct AngSizes {
 constexpr int kWaveRows = 64;  // r values per block of the compile-time kernel
 constexpr int kXcds = 8;
 
-template <int MMAX, int MRSO, typename TP, typename TO, typename TC>
+template <int MMAX, int MRSO, typename TP, typename TO, typename TC, bool CP>
 __global__ __launch_bounds__(64) void proj2angl_wave_kernel(const cplx<TP>* __restrict__ p, TO* __restrict__ o,
                                                             const TO* __restrict__ xi, TO* __restrict__ df,
                                                             const double* __restrict__ wq, long N, long nrb,
@@ -407,6 +407,7 @@ __global__ __launch_bounds__(64) void proj2angl_wave_kernel(const cplx<TP>* __re
 #pragma unroll
     for (int b = 0; b < A::kNmu2; ++b) sr[a][b] = si[a][b] = TC(0);
   {
+    constexpr OutputChannels<MMAX, MRSO> kOut{};
     int ip = 0;
 #pragma unroll
     for (int m = 0; m <= MMAX; ++m)
@@ -414,7 +415,8 @@ __global__ __launch_bounds__(64) void proj2angl_wave_kernel(const cplx<TP>* __re
       for (int mup = -m; mup <= m; ++mup)
 #pragma unroll
         for (int mu = 0; mu <= m; mu += MRSO, ++ip) {
-          const cplx<TP> v = p[static_cast<long>(ip) * N + r];
+          // compact order: the OZ output channel of this projection
+          const cplx<TP> v = p[static_cast<long>(CP ? kOut.v[ip] : ip) * N + r];
           const TC wt = w[ip];
           sr[mup + MMAX][mu / MRSO] = fma(static_cast<TC>(v.x), wt, sr[mup + MMAX][mu / MRSO]);
           si[mup + MMAX][mu / MRSO] = fma(static_cast<TC>(v.y), wt, si[mup + MMAX][mu / MRSO]);
@@ -705,6 +707,32 @@ T* upload(const std::vector<T>& h) {
 
 }  // namespace
 
+int compact_projections(const AngularGrid& g) {
+  int c = 0;
+  for (int ip = 0; ip < g.np; ++ip) c += implied_projection(g.mup[ip], g.mu[ip]) ? 0 : 1;
+  return c;
+}
+
+std::vector<int> compact_codes(const AngularGrid& g) {
+  throw_if(g.np > 256, "compact channel order: at most 256 projections");
+  std::vector<int> idx(g.np, -1), code(g.np, 0);
+  int c = 0;
+  for (int ip = 0; ip < g.np; ++ip)
+    if (!implied_projection(g.mup[ip], g.mu[ip])) idx[ip] = c++;
+  int k = c;  // output channels of the implied projections
+  for (int ip = 0; ip < g.np; ++ip) {
+    if (idx[ip] >= 0) {
+      code[ip] = idx[ip] | (idx[ip] << 8);
+      continue;
+    }
+    const int partner = g.P(g.m[ip], -g.mup[ip], 0);
+    throw_if(partner < 0 || idx[partner] < 0 || g.mu[partner] != 0, "internal error: compact projection partner");
+    code[ip] = idx[partner] | (k++ << 8) | kCompactImplied | ((g.mup[ip] & 1) ? kCompactOdd : 0);
+  }
+  throw_if(k != g.np, "internal error: compact output channels");
+  return code;
+}
+
 ProjTables::ProjTables(const AngularGrid& g)
     : mmax(g.mmax), mrso(g.mrso), ntheta(g.ntheta), nphi(g.nphi), npsi(g.npsi), nmu2(g.nmu2), no(g.no), np(g.np) {
   throw_if(npsi % 2 == 0, "npsi must be odd");
@@ -831,23 +859,35 @@ int proj2angl_blocks(const ProjTables& t, long N) {
 
 bool proj2angl_has_column_sums(const ProjTables& t) { return dispatch_angular(t.mmax, t.mrso, [](auto, auto) {}); }
 
+bool proj2angl_compiled(int mmax, int mrso) { return dispatch_angular(mmax, mrso, [](auto, auto) {}); }
+
 long proj2angl_row_blocks(long N) { return wave_row_blocks(N); }
 
 template <typename TP, typename TO, typename TC>
 void proj2angl_factorized(const ProjTables& t, long N, const cplx<TP>* p, TO* o, const TO* xi, TO* df,
-                          double rho0, double* partials, hipStream_t s, double* colpart) {
+                          double rho0, double* partials, hipStream_t s, double* colpart, bool compact) {
   if (N == 0) return;
   const long nwb = wave_blocks(t, N);
   throw_if(nwb > 0x7fffffffL, "grid too large");
+  // the compact channel order is compiled for FP32 projections (the mixed forms)
+  constexpr bool kCompactOk = std::is_same<TP, float>::value && std::is_same<TC, float>::value;
+  throw_if(compact && !kCompactOk, "proj2angl: the compact channel order takes FP32 projections");
   const bool compiled = dispatch_angular(t.mmax, t.mrso, [&](auto mm, auto mr) {
-    hipLaunchKernelGGL((proj2angl_wave_kernel<decltype(mm)::value, decltype(mr)::value, TP, TO, TC>),
-                       dim3(static_cast<unsigned>(nwb)), dim3(64), 0, s, p, o, xi, df, t.w, N, wave_row_blocks(N),
-                       tabs_of<TC>(t, false), rho0, partials, colpart);
+    constexpr int MM = decltype(mm)::value, MR = decltype(mr)::value;
+    if (kCompactOk && compact)
+      hipLaunchKernelGGL((proj2angl_wave_kernel<MM, MR, TP, TO, TC, kCompactOk>), dim3(static_cast<unsigned>(nwb)),
+                         dim3(64), 0, s, p, o, xi, df, t.w, N, wave_row_blocks(N), tabs_of<TC>(t, false), rho0,
+                         partials, colpart);
+    else
+      hipLaunchKernelGGL((proj2angl_wave_kernel<MM, MR, TP, TO, TC, false>), dim3(static_cast<unsigned>(nwb)),
+                         dim3(64), 0, s, p, o, xi, df, t.w, N, wave_row_blocks(N), tabs_of<TC>(t, false), rho0,
+                         partials, colpart);
   });
   if (compiled) {
     MDFT_HIP_CHECK(hipGetLastError());
     return;
   }
+  throw_if(compact, "proj2angl: the compact channel order needs a compile-time angular grid");
   throw_if(colpart != nullptr, "proj2angl: column sums need a compile-time angular grid");
   const long nblk = (N + kRT - 1) / kRT;
   throw_if(nblk > 0x7fffffffL, "grid too large");
@@ -904,13 +944,13 @@ template void angl2proj_factorized<double, float, float>(const ProjTables&, long
                                                          const double*, hipStream_t);
 template void proj2angl_factorized<double, double, double>(const ProjTables&, long, const cplx<double>*, double*,
                                                            const double*, double*, double, double*, hipStream_t,
-                                                           double*);
+                                                           double*, bool);
 template void proj2angl_factorized<float, float, float>(const ProjTables&, long, const cplx<float>*, float*,
                                                         const float*, float*, double, double*, hipStream_t,
-                                                        double*);
+                                                        double*, bool);
 template void proj2angl_factorized<float, double, float>(const ProjTables&, long, const cplx<float>*, double*,
                                                          const double*, double*, double, double*, hipStream_t,
-                                                         double*);
+                                                         double*, bool);
 template void delta_rho<double>(const double*, double*, size_t, double, hipStream_t);
 template void delta_rho<float>(const float*, float*, size_t, double, hipStream_t);
 template int delta_f<double>(const double*, const double*, const double*, double*, long, int, double, double*, int,

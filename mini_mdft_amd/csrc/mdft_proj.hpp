@@ -2,6 +2,8 @@
 // mdft_proj.hip).
 #pragma once
 
+#include <vector>
+
 #include "mdft_device.hpp"
 
 namespace mdft {
@@ -29,6 +31,56 @@ struct ProjTables {
   ProjTables& operator=(const ProjTables&) = delete;
 };
 
+// The compact channel order of the fused evaluation.  The fields of the path
+// are real, so a mu = 0 projection with mup < 0 is implied by its (m, -mup, 0)
+// partner: p^m_{-mup,0} = (-1)^mup conj(p^m_{mup,0}) in r space, and after the
+// forward FFT p^m_{-mup,0}(q) = (-1)^mup conj(p^m_{mup,0}(-q)).  angl2proj and
+// the forward FFT hold the other projections only ("stored", in their
+// canonical order: m, mup, mu ascending; at mmax 5, mrso 2, 73 of 88); the OZ
+// pass reads an implied projection from its partner and writes every
+// output, the implied ones after the stored ones (their canonical order).
+// The OZ output keeps all projections because it is real only for a c(q)
+// with the matching symmetry, which the path does not assume.
+__host__ __device__ constexpr bool implied_projection(int mup, int mu) { return mu == 0 && mup < 0; }
+// stored projections, and the channel of canonical projection ip in the OZ
+// output order (stored ones first), for compile-time grids
+template <int MMAX, int MRSO>
+__host__ __device__ constexpr int compact_count() {
+  int c = 0;
+  for (int m = 0; m <= MMAX; ++m)
+    for (int mup = -m; mup <= m; ++mup)
+      for (int mu = 0; mu <= m; mu += MRSO) c += implied_projection(mup, mu) ? 0 : 1;
+  return c;
+}
+template <int MMAX, int MRSO>
+__host__ __device__ constexpr int output_channel(int ip) {
+  int c = 0, k = compact_count<MMAX, MRSO>(), i = 0;
+  for (int m = 0; m <= MMAX; ++m)
+    for (int mup = -m; mup <= m; ++mup)
+      for (int mu = 0; mu <= m; mu += MRSO, ++i) {
+        const bool imp = implied_projection(mup, mu);
+        if (i == ip) return imp ? k : c;
+        c += imp ? 0 : 1;
+        k += imp ? 1 : 0;
+      }
+  return -1;
+}
+// the same as a table, for kernels that index it with unrolled loop counters
+template <int MMAX, int MRSO>
+struct OutputChannels {
+  int v[(MMAX + 1) * (MMAX + 1) * (MMAX + 2)];
+  __host__ __device__ constexpr OutputChannels() : v() {
+    for (int i = 0; i < (MMAX + 1) * (MMAX + 1) * (MMAX + 2); ++i) v[i] = output_channel<MMAX, MRSO>(i);
+  }
+};
+// host: the number of stored projections of a grid, and per canonical
+// projection a code: bits 0-7 the stored channel it is read from (an implied
+// projection: its partner's), bits 8-15 its OZ output channel, bit 16
+// implied, bit 17 mup odd
+constexpr int kCompactImplied = 1 << 16, kCompactOdd = 1 << 17;
+int compact_projections(const AngularGrid& g);
+std::vector<int> compact_codes(const AngularGrid& g);
+
 // angl2proj (common/MDFT_OrientationProjectionTransform.hpp:267-380).  With
 // drho_rho0 != nullptr the input is xi and delta_rho = rho0 (xi^2 - 1) is
 // formed on load (common/MDFT_Solvent.hpp:575-596 fused).
@@ -42,12 +94,17 @@ void angl2proj_factorized(const ProjTables& t, long N, const TI* o, cplx<TP>* p,
 // With colpart != nullptr (and xi, compile-time angular grids only) the
 // epilogue also writes, per block of 64 rows rb, the column sums over its
 // rows: colpart[rb][0][io] = sum_r (xi^2 - 1), colpart[rb][1][io] = sum_r vexc.
+// With compact, p holds the OZ output order of the compact channel order
+// (compile-time grids only).
 template <typename TP, typename TO, typename TC>
 void proj2angl_factorized(const ProjTables& t, long N, const cplx<TP>* p, TO* o, const TO* xi, TO* df,
-                          double rho0, double* partials, hipStream_t s, double* colpart = nullptr);
+                          double rho0, double* partials, hipStream_t s, double* colpart = nullptr,
+                          bool compact = false);
 // whether proj2angl_factorized can write column sums for this grid, and the
 // row blocks (64 rows each) it writes them for
 bool proj2angl_has_column_sums(const ProjTables& t);
+// whether proj2angl has a compile-time kernel for (mmax, mrso)
+bool proj2angl_compiled(int mmax, int mrso);
 long proj2angl_row_blocks(long N);
 // partial sums written by the fused proj2angl over N points (one per row
 // block, or per (row block, theta)); for N a multiple of 64 the partials of

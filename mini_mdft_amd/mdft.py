@@ -339,6 +339,20 @@ class Solver:
         available for MDFT_FP64 / MDFT_FP32, where it changes ff by rounding only)."""
         check(lib().mdft_solver_set_mean_fp64(self._h, 1 if on else 0))
 
+    def set_compact(self, on: bool) -> None:
+        """The compact channel order of the fused evaluation (default on for
+        MDFT_MIXED_X6 / MDFT_MIXED_X6F where the grid allows): the mu = 0,
+        mup < 0 projections of delta_rho, implied by their partners for real
+        fields, are not written by angl2proj nor transformed by the forward
+        FFT; results change by rounding only."""
+        check(lib().mdft_solver_set_compact(self._h, 1 if on else 0))
+
+    @property
+    def compact(self) -> bool:
+        v = ctypes.c_int(0)
+        check(lib().mdft_solver_compact(self._h, ctypes.byref(v)))
+        return bool(v.value)
+
     def set_timing(self, enable: bool, oz_only: bool = False) -> None:
         """Per-stage hipEvent timers (the reference Solver's stage timers); with
         oz_only, only the OZ launch is timed, by the start and end events that
