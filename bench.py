@@ -56,7 +56,16 @@ OZ_FLOP_PER_PAIR = 52344  # rotations, OZ product, phases (both members of the p
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "oz_star_64c_mmax5_traffic.md")
 MFMA_SUMMARY = os.path.join(ROOT, "profiles", "a2p_gemm_64c_mmax5_mfma.md")
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16, vendor figure
-A2P_ALGO_FLOP = 4 * 88 * 330 * 64 ** 3  # dense [N x no] . [no x np] complex contraction, one direction
+# dense [N x no] . [no x np] complex contraction, one direction (np: the
+# projections angl2proj writes, 73 of 88 in the compact channel order)
+def a2p_algo_flop(np_written):
+    return 4 * np_written * 330 * 64 ** 3
+
+
+def implied_projections(mmax):
+    """mu = 0, mup < 0 projections: implied by their partners for real fields
+    (sum over m of m)."""
+    return mmax * (mmax + 1) // 2
 # the solve leg: 20 sites within 5 A of the centre, charge scale 0.1, V_ext
 # capped at 100 kT, L-BFGS with 1 pair to max |h0 g| < 1e-5 (with the h0
 # preconditioner one pair takes as many iterations as 3 or 5 here, 81-89, and
@@ -67,16 +76,21 @@ SOLVE_M, SOLVE_GTOL, SOLVE_MAX_ITER = 1, 1e-5, 400
 SERVE_CAPACITY, SERVE_BUDGET = 4, 200e9
 
 
-def stage_bytes(N, no, np_, form):
+def stage_bytes(N, no, np_, form, np_stored=None):
     """Algorithmic HBM bytes of each stage of one evaluation: every stage reads
     its input once and writes its output once (delta_rho fused into the
-    projection load, delta_f into the proj2angl epilogue)."""
+    projection load, delta_f into the proj2angl epilogue).  With the compact
+    channel order (np_stored < np_) angl2proj writes and the forward FFT
+    transforms the stored projections only, and the OZ pass reads those and
+    writes all."""
     O = N * no * (4 if form == "x6f" else 8)  # one orientation field (FP32 copies in MDFT_MIXED_X6F)
-    P = N * np_ * (16 if form == "fp64" else 8)  # the projections
+    pb = 16 if form == "fp64" else 8
+    P = N * np_ * pb  # the projections
+    Ps = N * (np_stored or np_) * pb  # the stored projections
     return {
-        "angl2proj": O + P,
-        "fft_forward": 2 * P,
-        "oz": 2 * P,
+        "angl2proj": O + Ps,
+        "fft_forward": 2 * Ps,
+        "oz": Ps + P,
         "fft_backward": 2 * P,
         "proj2angl_delta_f": P + 2 * O,
     }
@@ -510,7 +524,9 @@ def main():
         df = torch.empty_like(x)
         ff = solver.energy_cproj(x, df)
         dt, stages, oz_timed = timed(solver, x, df, ffd, args.steps, args.warmup)
-        results[form] = dict(ff=ff, df=df, sec=dt, stages=stages, oz_timed=oz_timed, fft=conv.fft_config())
+        np_stored = np_ - implied_projections(MMAX) if solver.compact else np_
+        results[form] = dict(ff=ff, df=df, sec=dt, stages=stages, oz_timed=oz_timed, fft=conv.fft_config(),
+                             np_stored=np_stored)
         print(f"[bench] {form}: {args.steps / dt:.1f} evals/s, F={ff:.10g}, stages(ms)="
               f"{ {k: round(v, 4) for k, v in stages.items()} }", file=sys.stderr, flush=True)
         del solver, conv
@@ -529,9 +545,9 @@ def main():
     fp64_rate = args.steps / f64["sec"]
     x6_rate = args.steps / x6["sec"]
 
-    by = stage_bytes(N, no, np_, "x6f")
+    by = stage_bytes(N, no, np_, "x6f", xf["np_stored"])
     stages = {k: dict(ms=round(ms, 5), GBs=round(by[k] / (ms * 1e-3) / 1e9, 1)) for k, ms in xf["stages"].items()}
-    by6 = stage_bytes(N, no, np_, "mixed")
+    by6 = stage_bytes(N, no, np_, "mixed", x6["np_stored"])
     stages6 = {k: dict(ms=round(ms, 5), GBs=round(by6[k] / (ms * 1e-3) / 1e9, 1)) for k, ms in x6["stages"].items()}
     # a stage above the HBM peak by its algorithmic bytes would mean a wrong byte model
     over_peak = [k for st in (stages, stages6) for k, v in st.items() if v["GBs"] > HBM_PEAK_GBS]
@@ -562,11 +578,13 @@ def main():
     roofline_mfma = None
     if mflops:
         ach = mflops / (a2p_ms * 1e-3) / 1e12
-        roofline_mfma = dict(kernel="a2p_gemm_kernel<float,3,3,12> (BF16x6 angl2proj, FP32 fields)", bound="mfma",
+        mt = ((2 * xf["np_stored"] + 15) // 16 + 1) & ~1  # output tiles of 16 (even bucket)
+        algo = a2p_algo_flop(xf["np_stored"])
+        roofline_mfma = dict(kernel=f"a2p_gemm_kernel<float,3,3,{mt}> (BF16x6 angl2proj, FP32 fields)", bound="mfma",
                              achieved=round(ach, 1), peak=MFMA_BF16_PEAK_TFLOPS, unit="TFLOP/s",
                              frac=round(ach / MFMA_BF16_PEAK_TFLOPS, 4), busy_frac_counter=mbusy,
-                             executed_flop_per_launch=mflops, algorithmic_flop=A2P_ALGO_FLOP,
-                             algorithmic_tflops=round(A2P_ALGO_FLOP / (a2p_ms * 1e-3) / 1e12, 1),
+                             executed_flop_per_launch=mflops, algorithmic_flop=algo,
+                             algorithmic_tflops=round(algo / (a2p_ms * 1e-3) / 1e12, 1),
                              launch_ms=round(a2p_ms, 5), hbm_GBs=stages["angl2proj"]["GBs"],
                              source=os.path.relpath(MFMA_SUMMARY, ROOT))
 
@@ -634,6 +652,11 @@ def main():
         "stages_fp64_ms_GBs": stages64,
         "fp64_eval_hbm_frac": round(sum(by64.values()) * fp64_rate / 1e9 / HBM_PEAK_GBS, 4),
         "fft_autotuned": {"x6f": xf["fft"], "mixed_x6": x6["fft"], "fp64": f64["fft"]},
+        # projections angl2proj writes and the forward FFT transforms (the
+        # compact channel order leaves out the mu = 0, mup < 0 ones, implied
+        # by their partners for the real delta_rho)
+        "projections_stored": {"x6f": xf["np_stored"], "mixed_x6": x6["np_stored"], "fp64": f64["np_stored"],
+                               "of": np_},
         "p99_solve": p99,
         "served_batch": served,
         "served_stream": streamed,
