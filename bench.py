@@ -565,7 +565,8 @@ def main():
     npairs = 131076
     achieved = oz_bytes / (oz_ms * 1e-3) / 1e9
     oz_tflops = OZ_FLOP_PER_PAIR * npairs / (oz_ms * 1e-3) / 1e12
-    roofline = dict(kernel="oz_star_ct_kernel<float,5,2,384>", bound="hbm", achieved=round(achieved, 1),
+    compact = "true" if xf["np_stored"] < np_ else "false"
+    roofline = dict(kernel=f"oz_star_ct_kernel<float,5,2,384,{compact}>", bound="hbm", achieved=round(achieved, 1),
                     peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
                     traffic=traffic_from_summary(), algorithmic_bytes=oz_bytes, launch_ms=round(oz_ms, 5),
                     fp32_tflops=round(oz_tflops, 2), fp32_frac=round(oz_tflops / FP32_PEAK_TFLOPS, 4),
