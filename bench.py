@@ -26,8 +26,8 @@ whose independent draws make the excess term non-self-adjoint and unbounded
 below; MDFT_MIXED_X6F, MDFT_MIXED_X6 and MDFT_FP64 solves of the same solutes,
 with |dF| of each mixed form against FP64.
 
-served_batch: 64 solutes of mixed grids and mmax submitted at once to a server
-(same-shape engines, HBM-budget admission), solutes/hour and p50/p99 latency to
+served_batch: 64 solutes of mixed grids and mmax submitted at once to an
+MDFT_MIXED_X6F server (same-shape engines, HBM-budget admission), solutes/hour and p50/p99 latency to
 max |h0 g| < 1e-5.  served_stream: Poisson streams of 48^3 - 96^3 / mmax 5
 requests, backlog drained: at 2 req/s for 20 s (over what the GPU sustains for
 this mix; its p99 grows with the window) which also gives the capacity, then
@@ -240,13 +240,13 @@ def solve_leg(g, ag, mp, rho0, pref, nsolves):
 def serve_leg(nsolutes, seed=7):
     """The mixed-size batch: nsolutes solutes with grids drawn from
     {32, 48, 64, 80}^3 (box = 0.47 A x grid side), mmax from {3, 5} and 5-40
-    sites, all submitted at once to one MDFT_MIXED_X6 server (engines of
+    sites, all submitted at once to one MDFT_MIXED_X6F server (engines of
     SERVE_CAPACITY slots per shape, HBM budget SERVE_BUDGET); each solve runs to
     max |h0 g| < 1e-5.  Latency is submit -> converged."""
     import numpy as np
     import torch
 
-    from mini_mdft_amd import MIXED_X6, luc, solute
+    from mini_mdft_amd import MIXED_X6F, luc, solute
     from mini_mdft_amd import mdft as M
 
     kT = M.kT(TEMP)
@@ -260,7 +260,7 @@ def serve_leg(nsolutes, seed=7):
         reqs.append((side, box, mm, solute.synthetic_solute(ns, 1000 + i, (box,) * 3, charge_scale=SOLVE_QSCALE)))
     nq = max(luc.nq_for(luc.qmax_of_grid((r[0],) * 3, (r[1],) * 3), luc.DQ_TIP3P) for r in reqs) + 4
     tab = luc.model_c_symmetric(5, MRSO, nq, seed=C_SEED)
-    srv = M.Server(tab, MRSO, N0, TEMP, MIXED_X6, capacity=SERVE_CAPACITY, m=SOLVE_M, max_iter=SOLVE_MAX_ITER,
+    srv = M.Server(tab, MRSO, N0, TEMP, MIXED_X6F, capacity=SERVE_CAPACITY, m=SOLVE_M, max_iter=SOLVE_MAX_ITER,
                    gtol=SOLVE_GTOL, budget_bytes=SERVE_BUDGET)
     water = solute.SOLVENT_SITES["tip3p"]
     torch.cuda.synchronize()
@@ -285,7 +285,7 @@ def serve_leg(nsolutes, seed=7):
     for i, (side, mm) in ids.items():
         shapes.setdefault(f"{side}^3/mmax{mm}", 0)
         shapes[f"{side}^3/mmax{mm}"] += 1
-    return {"solutes": nsolutes, "form": "MDFT_MIXED_X6", "capacity_per_shape": SERVE_CAPACITY,
+    return {"solutes": nsolutes, "form": "MDFT_MIXED_X6F", "capacity_per_shape": SERVE_CAPACITY,
             "hbm_budget_GB": SERVE_BUDGET / 1e9, "makespan_s": round(makespan, 3),
             "solutes_per_hour": round(3600.0 * nsolutes / makespan, 1),
             "p50_s": round(percentile(lat, 50), 4), "p99_s": round(percentile(lat, 99), 4),
@@ -309,13 +309,13 @@ def stream_tab():
 def stream_leg(seconds, rate, seed):
     """A Poisson stream of solvation requests at `rate` per second for
     `seconds`: grids 48^3 - 96^3 at mmax = 5 (box 0.47 A x side), 5-60 sites,
-    one MDFT_MIXED_X6 server; requests arriving while others run are admitted
+    one MDFT_MIXED_X6F server; requests arriving while others run are admitted
     as slots and memory free up.  After the window the backlog is drained.
     Latency is arrival -> converged.  Returns the summary and, per request,
     (arrival s, latency s, side, solute, F, status)."""
     import numpy as np
 
-    from mini_mdft_amd import MIXED_X6, solute
+    from mini_mdft_amd import MIXED_X6F, solute
     from mini_mdft_amd import mdft as M
 
     kT = M.kT(TEMP)
@@ -330,7 +330,7 @@ def stream_leg(seconds, rate, seed):
         sol = solute.synthetic_solute(int(rng.integers(5, 61)), 5000 + 1000 * seed + len(arrivals), (box,) * 3,
                                       charge_scale=SOLVE_QSCALE)
         arrivals.append((t, side, box, sol))
-    srv = M.Server(stream_tab(), MRSO, N0, TEMP, MIXED_X6, capacity=SERVE_CAPACITY, m=SOLVE_M,
+    srv = M.Server(stream_tab(), MRSO, N0, TEMP, MIXED_X6F, capacity=SERVE_CAPACITY, m=SOLVE_M,
                    max_iter=SOLVE_MAX_ITER, gtol=SOLVE_GTOL, budget_bytes=SERVE_BUDGET)
     water = solute.SOLVENT_SITES["tip3p"]
     t0 = time.perf_counter()
@@ -360,7 +360,7 @@ def stream_leg(seconds, rate, seed):
     lat = [x[1] for x in recs]
     half = [x[1] for x in recs if x[0] < seconds / 2], [x[1] for x in recs if x[0] >= seconds / 2]
     busy = end - recs[0][0]  # first arrival -> last completion
-    return {"requests": len(arrivals), "rate_per_s": rate, "window_s": seconds, "form": "MDFT_MIXED_X6",
+    return {"requests": len(arrivals), "rate_per_s": rate, "window_s": seconds, "form": "MDFT_MIXED_X6F",
             "finished_s": round(end, 3), "solutes_per_hour": round(3600.0 * len(arrivals) / end, 1),
             "completed_per_busy_s": round(len(recs) / busy, 3),
             "p50_s": round(percentile(lat, 50), 4), "p99_s": round(percentile(lat, 99), 4),
@@ -427,7 +427,7 @@ def serving_leg(window_over, window_sustained, every=8, seed=11):
     rows = fp64_resolve(recs, every)
     dfs = [abs(r["dF"]) for r in rows]
     return {
-        "form": "MDFT_MIXED_X6",
+        "form": "MDFT_MIXED_X6F",
         "capacity_per_s": cap,
         "sustainable_rate_per_s": sustained["rate_per_s"] if flat else None,
         "sustainable_criterion": "p99 of the second half of the arrivals <= 1.5 x p99 of the first half",
