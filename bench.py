@@ -80,9 +80,8 @@ def stage_bytes(N, no, np_, form, np_stored=None):
     """Algorithmic HBM bytes of each stage of one evaluation: every stage reads
     its input once and writes its output once (delta_rho fused into the
     projection load, delta_f into the proj2angl epilogue).  With the compact
-    channel order (np_stored < np_) angl2proj writes and the forward FFT
-    transforms the stored projections only, and the OZ pass reads those and
-    writes all."""
+    channel order (np_stored < np_) every stage moves the stored projections
+    only."""
     O = N * no * (4 if form == "x6f" else 8)  # one orientation field (FP32 copies in MDFT_MIXED_X6F)
     pb = 16 if form == "fp64" else 8
     P = N * np_ * pb  # the projections
@@ -90,9 +89,9 @@ def stage_bytes(N, no, np_, form, np_stored=None):
     return {
         "angl2proj": O + Ps,
         "fft_forward": 2 * Ps,
-        "oz": Ps + P,
-        "fft_backward": 2 * P,
-        "proj2angl_delta_f": P + 2 * O,
+        "oz": 2 * Ps,
+        "fft_backward": 2 * Ps,
+        "proj2angl_delta_f": Ps + 2 * O,
     }
 
 
@@ -653,9 +652,9 @@ def main():
         "stages_fp64_ms_GBs": stages64,
         "fp64_eval_hbm_frac": round(sum(by64.values()) * fp64_rate / 1e9 / HBM_PEAK_GBS, 4),
         "fft_autotuned": {"x6f": xf["fft"], "mixed_x6": x6["fft"], "fp64": f64["fft"]},
-        # projections angl2proj writes and the forward FFT transforms (the
-        # compact channel order leaves out the mu = 0, mup < 0 ones, implied
-        # by their partners for the real delta_rho)
+        # projections the stages store and transform (the compact channel
+        # order leaves out the mu = 0, mup < 0 ones: implied by their partners
+        # in delta_rho, folded into them on the way back)
         "projections_stored": {"x6f": xf["np_stored"], "mixed_x6": x6["np_stored"], "fp64": f64["np_stored"],
                                "of": np_},
         "p99_solve": p99,

@@ -229,16 +229,17 @@ int mdft_solver_set_mean_fp64(mdft_solver_t h, int on);
 int mdft_solver_stage_ms(mdft_solver_t h, double* ms, long* count);
 
 /* No counterpart in the reference (its path stores every projection).  The
- * compact channel order of the fused evaluation: the fields are real, so a
- * mu = 0 projection of delta_rho with mup < 0 equals (-1)^mup conj of its
- * (m, -mup, 0) partner, before the FFT and (at -q) after it.  In this order
- * angl2proj writes only the other projections (73 of 88 at mmax 5, mrso 2)
- * and the forward FFT transforms those; the OZ pass reads an implied
- * projection from its partner at the mirrored point and writes all outputs
- * (the OZ output obeys the symmetry only for a c(q) that does, which is not
- * assumed), which the inverse FFT and proj2angl take in full.  On by default
- * for MDFT_MIXED_X6 and MDFT_MIXED_X6F where the grid has the compile-time
- * proj2angl and OZ kernels; results change by rounding only.
+ * compact channel order of the fused evaluation: delta_rho is real, so its
+ * mu = 0 projection with mup < 0 equals (-1)^mup conj of its (m, -mup, 0)
+ * partner, before the FFT and (at -q) after it; and proj2angl reads the
+ * mu = 0 column through its real part, to which the pair contributes only as
+ * g_+ + (-1)^mup conj(g_-).  In this order every stage holds the other
+ * projections (73 of 88 at mmax 5, mrso 2): angl2proj writes them, the FFTs
+ * transform them, the OZ pass reads an implied input from its partner at the
+ * mirrored point and stores the pair's combination in the partner's channel,
+ * and proj2angl takes the implied sums as zero.  Nothing is assumed of c(q).
+ * On by default for MDFT_MIXED_X6 and MDFT_MIXED_X6F where the grid has the
+ * compile-time proj2angl and OZ kernels; results change by rounding only.
  * mdft_solver_compact reports whether it is on. */
 int mdft_solver_set_compact(mdft_solver_t h, int on);
 int mdft_solver_compact(mdft_solver_t h, int* on);

@@ -789,10 +789,11 @@ __global__ __launch_bounds__(THREADS) void oz_star_ct_kernel(cplx<T>* __restrict
   T g0x = 0, g0y = 0, g1x = 0, g1y = 0;
   long sq = 0, smq = 0;
   bool singular = false;
-  // the projection's channels in p; in the compact order an implied
+  // the projection's channel in p; in the compact order an implied
   // projection reads its partner's channel at the mirrored point, conjugated
-  // and times (-1)^mup, and writes its own output channel after the stored ones
-  long chan = ip, ochan = ip;
+  // and times (-1)^mup, and hands its outputs to the partner, which stores
+  // g_+(q) + (-1)^mup conj(g_-(-q)) (and the same at -q)
+  long chan = ip;
   bool implied = false;
   if (active) {
     // this pair's phase and its projection's info
@@ -811,7 +812,6 @@ __global__ __launch_bounds__(THREADS) void oz_star_ct_kernel(cplx<T>* __restrict
     if constexpr (CP) {
       const int cc = ccode[ip];
       chan = cc & 0xff;
-      ochan = (cc >> 8) & 0xff;
       implied = (cc & kCompactImplied) != 0;
       odd = (cc & kCompactOdd) != 0;
     }
@@ -937,6 +937,7 @@ __global__ __launch_bounds__(THREADS) void oz_star_ct_kernel(cplx<T>* __restrict
   __syncthreads();
 
   // inverse rotation and stores (:518-541)
+  cplx<T> f0 = mkc<T>(0, 0), f1 = mkc<T>(0, 0);
   if (active) {
     const int mup = mupm - MMAX;
     const cplx<T>* g0 = G + row * W;
@@ -957,8 +958,33 @@ __global__ __launch_bounds__(THREADS) void oz_star_ct_kernel(cplx<T>* __restrict
     const cplx<T> o0 = cmul<T>(mkc<T>(ar, ai), cconj<T>(e));
     cplx<T> o1 = cmul<T>(mkc<T>(br, bi), cconj<T>(e));
     if (m & 1) o1 = mkc<T>(-o1.x, -o1.y);
-    p[ochan * a.pstride + sq] = o0;
-    p[ochan * a.pstride + smq] = singular ? cconj<T>(o1) : o1;
+    // the values the full order leaves at q and -q (one address, the second
+    // store, when the point is its own mirror: q = 0 and the singular points)
+    f1 = singular ? cconj<T>(o1) : o1;
+    f0 = sq == smq ? f1 : o0;
+    if (!CP) {
+      p[chan * a.pstride + sq] = o0;
+      p[chan * a.pstride + smq] = singular ? cconj<T>(o1) : o1;
+    } else if (implied) {
+      Dq[ip] = f0;  // free since the OZ product
+      Dmq[ip] = f1;
+    }
+  }
+  if constexpr (CP) {
+    __syncthreads();
+    if (active && !implied) {
+      const int mup = mupm - MMAX;
+      if (mu2 == 0 && mup > 0) {
+        // c = g_+ + (-1)^mup conj(g_- at the mirrored point)
+        const int ipr = ip - 2 * mup * (m / MRSO + 1);  // the implied (m, -mup, 0)
+        const cplx<T> h0 = Dq[ipr], h1 = Dmq[ipr];
+        const T sg = (mup & 1) ? T(-1) : T(1);
+        f0 = mkc<T>(f0.x + sg * h1.x, f0.y - sg * h1.y);
+        f1 = mkc<T>(f1.x + sg * h0.x, f1.y - sg * h0.y);
+      }
+      p[chan * a.pstride + sq] = f0;
+      p[chan * a.pstride + smq] = f1;
+    }
   }
 }
 
@@ -1505,13 +1531,11 @@ void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch,
 template <typename T>
 void ConvDevice::execute(cplx<T>* p, double chop_eps, hipStream_t s, StageTimer* timer, int nbatch,
                          bool compact) const {
-  // the compact order transforms the stored projections forward, and every
-  // projection of the OZ output backward
-  const long batch = static_cast<long>(tab.ang.np) * nbatch;
-  const long fbatch = static_cast<long>(compact ? compact_projections(tab.ang) : tab.ang.np) * nbatch;
+  // the compact order transforms the stored projections only
+  const long batch = static_cast<long>(compact ? compact_projections(tab.ang) : tab.ang.np) * nbatch;
   if (!fft.tuned<T>()) fft.autotune<T>(batch, s);  // once per element type, before any timing mark
   if (timer) timer->mark(s, kStageConvStart);
-  fft.execute<T>(p, fbatch, -1, s);
+  fft.execute<T>(p, batch, -1, s);
   if (timer) timer->mark(s, kStageFftFwd);
   if (timer && timer->oz_only())
     oz_pass<T>(p, chop_eps, s, nbatch, timer->event(kStageFftFwd), timer->event(kStageOz), compact);

@@ -58,8 +58,10 @@ struct ConvDevice {
   // gamma components below chop_eps in magnitude are set to 0.  With
   // nbatch > 1, p holds nbatch fields as (np, nbatch, nx, ny, nz).  With
   // events, the launch records its own start and end times in ev0 and ev1.
-  // With compact, p holds the compact channel order (compact_projections
-  // channels; FP32 and the star kernel only, compact_supported).
+  // With compact, p holds the stored channels of the compact channel order
+  // (compact_projections; FP32 and the star kernel only, compact_supported):
+  // implied inputs are read from their partners, and each partner's channel
+  // receives the pair's combination g_+ + (-1)^mup conj(g_- at -q).
   template <typename T>
   void oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch = 1, hipEvent_t ev0 = nullptr,
                hipEvent_t ev1 = nullptr, bool compact = false) const;

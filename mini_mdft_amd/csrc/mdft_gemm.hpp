@@ -22,10 +22,9 @@ struct ProjGemm {
   // ~2^-26.  p2a_mfma: proj2angl as a BF16x3 GEMM (needs (2, 2)), else the
   // factorised FP32 VALU kernel.  Grids beyond the MFMA register tiles
   // (2 np > 256 or no > 384) run the factorised FP32 projections instead.
-  // compact: angl2proj writes the stored projections of the compact channel
-  // order (the implied mu = 0, mup < 0 ones left out) and proj2angl reads the
-  // OZ output order (compact_codes); needs the GEMM angl2proj and the
-  // compile-time FP32 proj2angl
+  // compact: the stored projections of the compact channel order (the
+  // implied mu = 0, mup < 0 ones left out; compact_codes), both ways; needs
+  // the GEMM angl2proj and the compile-time FP32 proj2angl
   ProjGemm(const AngularGrid& g, int data_parts, int weight_parts, bool p2a_mfma, bool compact = false);
   ~ProjGemm();
   ProjGemm(const ProjGemm&) = delete;

@@ -407,7 +407,7 @@ __global__ __launch_bounds__(64) void proj2angl_wave_kernel(const cplx<TP>* __re
 #pragma unroll
     for (int b = 0; b < A::kNmu2; ++b) sr[a][b] = si[a][b] = TC(0);
   {
-    constexpr OutputChannels<MMAX, MRSO> kOut{};
+    constexpr CompactMap<MMAX, MRSO> kCm{};
     int ip = 0;
 #pragma unroll
     for (int m = 0; m <= MMAX; ++m)
@@ -415,8 +415,10 @@ __global__ __launch_bounds__(64) void proj2angl_wave_kernel(const cplx<TP>* __re
       for (int mup = -m; mup <= m; ++mup)
 #pragma unroll
         for (int mu = 0; mu <= m; mu += MRSO, ++ip) {
-          // compact order: the OZ output channel of this projection
-          const cplx<TP> v = p[static_cast<long>(CP ? kOut.v[ip] : ip) * N + r];
+          // compact order: the stored channel (mup, 0) holds the pair's
+          // combination, and the implied sums (-mup, 0) stay zero
+          if (CP && implied_projection(mup, mu)) continue;
+          const cplx<TP> v = p[static_cast<long>(CP ? kCm.v[ip] : ip) * N + r];
           const TC wt = w[ip];
           sr[mup + MMAX][mu / MRSO] = fma(static_cast<TC>(v.x), wt, sr[mup + MMAX][mu / MRSO]);
           si[mup + MMAX][mu / MRSO] = fma(static_cast<TC>(v.y), wt, si[mup + MMAX][mu / MRSO]);
@@ -719,17 +721,15 @@ std::vector<int> compact_codes(const AngularGrid& g) {
   int c = 0;
   for (int ip = 0; ip < g.np; ++ip)
     if (!implied_projection(g.mup[ip], g.mu[ip])) idx[ip] = c++;
-  int k = c;  // output channels of the implied projections
   for (int ip = 0; ip < g.np; ++ip) {
     if (idx[ip] >= 0) {
-      code[ip] = idx[ip] | (idx[ip] << 8);
+      code[ip] = idx[ip];
       continue;
     }
     const int partner = g.P(g.m[ip], -g.mup[ip], 0);
     throw_if(partner < 0 || idx[partner] < 0 || g.mu[partner] != 0, "internal error: compact projection partner");
-    code[ip] = idx[partner] | (k++ << 8) | kCompactImplied | ((g.mup[ip] & 1) ? kCompactOdd : 0);
+    code[ip] = idx[partner] | kCompactImplied | ((g.mup[ip] & 1) ? kCompactOdd : 0);
   }
-  throw_if(k != g.np, "internal error: compact output channels");
   return code;
 }
 

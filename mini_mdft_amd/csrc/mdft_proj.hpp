@@ -36,14 +36,17 @@ struct ProjTables {
 // partner: p^m_{-mup,0} = (-1)^mup conj(p^m_{mup,0}) in r space, and after the
 // forward FFT p^m_{-mup,0}(q) = (-1)^mup conj(p^m_{mup,0}(-q)).  angl2proj and
 // the forward FFT hold the other projections only ("stored", in their
-// canonical order: m, mup, mu ascending; at mmax 5, mrso 2, 73 of 88); the OZ
-// pass reads an implied projection from its partner and writes every
-// output, the implied ones after the stored ones (their canonical order).
-// The OZ output keeps all projections because it is real only for a c(q)
-// with the matching symmetry, which the path does not assume.
+// canonical order: m, mup, mu ascending; at mmax 5, mrso 2, 73 of 88), and
+// the OZ pass reads an implied projection from its partner.  On the way back
+// proj2angl uses only the real part of the mu = 0 column, to which the pair
+// (m, +-mup, 0) contributes through c = p^m_{mup,0} + (-1)^mup conj(p^m_{-mup,0})
+// alone (the weights obey d^m_{-mup,0} = (-1)^mup d^m_{mup,0}); so the OZ pass
+// stores that combination, c(q) = g_+(q) + (-1)^mup conj(g_-(-q)), in the
+// partner's channel, the inverse FFT transforms the stored channels, and
+// proj2angl takes the implied sums as zero.  Nothing is assumed of c(q).
 __host__ __device__ constexpr bool implied_projection(int mup, int mu) { return mu == 0 && mup < 0; }
-// stored projections, and the channel of canonical projection ip in the OZ
-// output order (stored ones first), for compile-time grids
+// stored projections, and the stored channel of canonical projection ip (an
+// implied one: its partner's), for compile-time grids
 template <int MMAX, int MRSO>
 __host__ __device__ constexpr int compact_count() {
   int c = 0;
@@ -53,30 +56,28 @@ __host__ __device__ constexpr int compact_count() {
   return c;
 }
 template <int MMAX, int MRSO>
-__host__ __device__ constexpr int output_channel(int ip) {
-  int c = 0, k = compact_count<MMAX, MRSO>(), i = 0;
+__host__ __device__ constexpr int compact_index(int ip) {
+  int c = 0, i = 0;
   for (int m = 0; m <= MMAX; ++m)
     for (int mup = -m; mup <= m; ++mup)
       for (int mu = 0; mu <= m; mu += MRSO, ++i) {
-        const bool imp = implied_projection(mup, mu);
-        if (i == ip) return imp ? k : c;
-        c += imp ? 0 : 1;
-        k += imp ? 1 : 0;
+        if (i == ip && !implied_projection(mup, mu)) return c;
+        if (i == ip) return compact_index<MMAX, MRSO>(ip - 2 * mup * (m / MRSO + 1));
+        c += implied_projection(mup, mu) ? 0 : 1;
       }
   return -1;
 }
 // the same as a table, for kernels that index it with unrolled loop counters
 template <int MMAX, int MRSO>
-struct OutputChannels {
+struct CompactMap {
   int v[(MMAX + 1) * (MMAX + 1) * (MMAX + 2)];
-  __host__ __device__ constexpr OutputChannels() : v() {
-    for (int i = 0; i < (MMAX + 1) * (MMAX + 1) * (MMAX + 2); ++i) v[i] = output_channel<MMAX, MRSO>(i);
+  __host__ __device__ constexpr CompactMap() : v() {
+    for (int i = 0; i < (MMAX + 1) * (MMAX + 1) * (MMAX + 2); ++i) v[i] = compact_index<MMAX, MRSO>(i);
   }
 };
 // host: the number of stored projections of a grid, and per canonical
-// projection a code: bits 0-7 the stored channel it is read from (an implied
-// projection: its partner's), bits 8-15 its OZ output channel, bit 16
-// implied, bit 17 mup odd
+// projection a code: bits 0-7 the stored channel it reads (an implied
+// projection: its partner's), bit 16 implied, bit 17 mup odd
 constexpr int kCompactImplied = 1 << 16, kCompactOdd = 1 << 17;
 int compact_projections(const AngularGrid& g);
 std::vector<int> compact_codes(const AngularGrid& g);
@@ -94,7 +95,7 @@ void angl2proj_factorized(const ProjTables& t, long N, const TI* o, cplx<TP>* p,
 // With colpart != nullptr (and xi, compile-time angular grids only) the
 // epilogue also writes, per block of 64 rows rb, the column sums over its
 // rows: colpart[rb][0][io] = sum_r (xi^2 - 1), colpart[rb][1][io] = sum_r vexc.
-// With compact, p holds the OZ output order of the compact channel order
+// With compact, p holds the stored channels of the compact channel order
 // (compile-time grids only).
 template <typename TP, typename TO, typename TC>
 void proj2angl_factorized(const ProjTables& t, long N, const cplx<TP>* p, TO* o, const TO* xi, TO* df,

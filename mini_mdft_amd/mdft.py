@@ -342,9 +342,9 @@ class Solver:
     def set_compact(self, on: bool) -> None:
         """The compact channel order of the fused evaluation (default on for
         MDFT_MIXED_X6 / MDFT_MIXED_X6F where the grid allows): the mu = 0,
-        mup < 0 projections of delta_rho, implied by their partners for real
-        fields, are not written by angl2proj nor transformed by the forward
-        FFT; results change by rounding only."""
+        mup < 0 projections, implied by their partners in the real delta_rho
+        and folded into them after the OZ pass, are neither stored nor
+        transformed; results change by rounding only."""
         check(lib().mdft_solver_set_compact(self._h, 1 if on else 0))
 
     @property

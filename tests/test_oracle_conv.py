@@ -324,3 +324,33 @@ def test_conv_routing_single_tuple(oracle, tup, why):
     out_a, in_a = routing_blocks(T, tup, mrso)
     out_b, _ = routing_blocks(T, swapped, mrso)
     assert not np.array_equal(out_a, out_b) and np.array_equal(out_b, in_a)
+
+
+def test_proj2angl_folds_implied_projections(oracle):
+    """What the compact channel order of the fused evaluation relies on past the
+    OZ pass, for any c(q): proj2angl reads the mu = 0 column through its real
+    part, so replacing channel (m, mup > 0, 0) by g_+ + (-1)^mup conj(g_-) of the
+    implied (m, -mup, 0) and zeroing the implied one leaves the orientations
+    unchanged (the weights obey d^m_{-mup,0} = (-1)^mup d^m_{mup,0}).  The
+    synthetic c(q) draws its tuples independently, so the OZ output itself does
+    not pair up; the fold without the sign changes the result."""
+    from mini_mdft_amd import luc
+
+    mmax, mrso, n, L = 5, 2, (6, 5, 4), (9.0, 8.0, 7.0)
+    T = oracle.angular_tables(mmax, mrso)
+    tab = luc.synthetic_c(mmax, mrso, luc.nq_for(luc.qmax_of_grid(n, L), luc.DQ_TIP3P), seed=5)
+    conv = oracle.Conv(n, L, mmax, mrso, tab.tuples, tab.c, tab.dq)
+    o = np.random.default_rng(3).uniform(-1, 1, (int(np.prod(n)), T["ntheta"], T["nphi"], T["npsi"]))
+    g = conv.execute(oracle.angl2proj(o, n, mmax, mrso))
+    ref = oracle.proj2angl(g, mmax, mrso)
+    idx = {(int(a), int(b), int(c)): i for i, (a, b, c) in enumerate(zip(T["m"], T["mup"], T["mu"]))}
+    pairs = [(idx[(m, -mup, 0)], i, -mup) for (m, mup, mu), i in idx.items() if mu == 0 and mup < 0]
+    # the OZ output does not pair up by itself here
+    assert max(np.abs(g[i] - (-1) ** k * np.conj(g[j])).max() for j, i, k in pairs) > 1e-3 * np.abs(g).max()
+    for sign in (1, -1):
+        gc = g.copy()
+        for j, i, k in pairs:
+            gc[j] = g[j] + sign * (-1) ** k * np.conj(g[i])
+            gc[i] = 0
+        err = np.abs(oracle.proj2angl(gc, mmax, mrso) - ref).max() / np.abs(ref).max()
+        assert (err < 1e-13) if sign == 1 else (err > 1e-3)
