@@ -115,11 +115,22 @@ struct mdft_solver_s {
 namespace {
 
 // whether the fused evaluation of this form can take the compact channel
-// order: the BF16x6 GEMM angl2proj, the compile-time FP32 proj2angl and the
-// FP32 star OZ kernel
-bool compact_available(int precision, const mdft_conv_s* conv) {
-  if (precision != MDFT_MIXED_X6 && precision != MDFT_MIXED_X6F) return false;
-  return ProjGemm::compact_possible(conv->dev->tab.ang, false) && conv->dev->compact_supported();
+// order: the BF16x6 GEMM angl2proj (X6 forms) or the compile-time angl2proj
+// (FP64, FP32), the compile-time proj2angl and the star OZ kernel
+bool compact_available(int precision, const mdft_conv_s* conv, const ProjTables& tab) {
+  const AngularGrid& g = conv->dev->tab.ang;
+  if (compact_projections(g) == g.np) return false;
+  switch (precision) {
+    case MDFT_MIXED_X6:
+    case MDFT_MIXED_X6F:
+      return ProjGemm::compact_possible(g, false) && conv->dev->compact_supported(false);
+    case MDFT_FP64:
+    case MDFT_FP32:
+      return angl2proj_compiled(tab, precision == MDFT_FP64) && proj2angl_compiled(g.mmax, g.mrso) &&
+             conv->dev->compact_supported(precision == MDFT_FP64);
+    default:
+      return false;
+  }
 }
 
 // whether this solver's proj2angl can write the column sums of the mean mode
@@ -567,7 +578,7 @@ int mdft_solver_create_batch(mdft_conv_t conv, double rho0, double prefactor, in
     h->N = static_cast<long>(t.nx) * t.ny * t.nz;
     throw_if(capacity > 1 && h->N % 64 != 0, "batched evaluation needs nx ny nz to be a multiple of 64");
     h->tab = std::make_unique<ProjTables>(t.ang);
-    h->compact = compact_available(precision, conv);
+    h->compact = compact_available(precision, conv, *h->tab);
     h->gemm = make_gemm(t.ang, precision, h->compact);
     h->field_parts = h->gemm ? h->gemm->p2a_blocks(h->N) : proj2angl_blocks(*h->tab, h->N);
     const size_t pbytes = static_cast<size_t>(h->N) * t.ang.np * capacity * (precision == MDFT_FP64 ? 16 : 8);
@@ -620,18 +631,22 @@ static void energy_impl(mdft_solver_t h, int nb, const void* xi, void* df, doubl
   switch (h->precision) {
     case MDFT_FP64: {
       auto* p = static_cast<cplx<double>*>(h->pbuf);
-      angl2proj_factorized<double, double, double>(*h->tab, Nb, static_cast<const double*>(xi), p, &rho0, s);
-      h->conv->dev->execute<double>(p, eps, s, tm, nb);
+      angl2proj_factorized<double, double, double>(*h->tab, Nb, static_cast<const double*>(xi), p, &rho0, s,
+                                                   h->compact);
+      h->conv->dev->execute<double>(p, eps, s, tm, nb, h->compact);
       proj2angl_factorized<double, double, double>(*h->tab, Nb, p, nullptr, static_cast<const double*>(xi),
-                                                   static_cast<double*>(df), rho0, h->partials, s, colp);
+                                                   static_cast<double*>(df), rho0, h->partials, s, colp,
+                                                   h->compact);
       break;
     }
     case MDFT_FP32: {
       auto* p = static_cast<cplx<float>*>(h->pbuf);
-      angl2proj_factorized<float, float, float>(*h->tab, Nb, static_cast<const float*>(xi), p, &rho0, s);
-      h->conv->dev->execute<float>(p, eps, s, tm, nb);
+      angl2proj_factorized<float, float, float>(*h->tab, Nb, static_cast<const float*>(xi), p, &rho0, s,
+                                                h->compact);
+      h->conv->dev->execute<float>(p, eps, s, tm, nb, h->compact);
       proj2angl_factorized<float, float, float>(*h->tab, Nb, p, nullptr, static_cast<const float*>(xi),
-                                                static_cast<float*>(df), rho0, h->partials, s, colp);
+                                                static_cast<float*>(df), rho0, h->partials, s, colp,
+                                                h->compact);
       break;
     }
     case MDFT_MIXED_X6F: {
@@ -660,7 +675,9 @@ static void energy_impl(mdft_solver_t h, int nb, const void* xi, void* df, doubl
 int mdft_solver_set_compact(mdft_solver_t h, int on) {
   return guarded([&] {
     throw_if(!h, "null handle");
-    if (on) throw_if(!compact_available(h->precision, h->conv), "solver: no compact channel order for this form and grid");
+    if (on)
+      throw_if(!compact_available(h->precision, h->conv, *h->tab),
+               "solver: no compact channel order for this form and grid");
     if ((on != 0) == h->compact) return;
     h->compact = on != 0;
     h->gemm = make_gemm(h->conv->dev->tab.ang, h->precision, h->compact);

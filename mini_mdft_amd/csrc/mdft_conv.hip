@@ -1461,8 +1461,8 @@ void launch_oz(K kernel, unsigned nb, unsigned threads, size_t lds, hipStream_t 
 template <typename T>
 void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch, hipEvent_t ev0,
                          hipEvent_t ev1, bool compact) const {
-  throw_if(compact && (!std::is_same<T, float>::value || !compact_supported()),
-           "OZ pass: the compact channel order needs FP32 projections and the star kernel");
+  throw_if(compact && !compact_supported(std::is_same<T, double>::value),
+           "OZ pass: the compact channel order needs the star kernel");
   if (npairs == 0) {
     if (ev0) MDFT_HIP_CHECK(hipEventRecord(ev0, s));
     if (ev1) MDFT_HIP_CHECK(hipEventRecord(ev1, s));
@@ -1497,14 +1497,12 @@ void ConvDevice::oz_pass(cplx<T>* p, double chop_eps, hipStream_t s, int nbatch,
         const long nb = ag.chunk * kXcds * nbatch;
         throw_if(nb > 0x7fffffffL, "too many q pairs");
         if (!dent) dent = upload(oz_dtable_places(MMAX));
-        if constexpr (std::is_same<T, float>::value) {
-          if (compact) {
-            if (!ccode) ccode = upload(compact_codes(tab.ang));
-            launch_oz(oz_star_ct_kernel<T, MMAX, MRSO, Lay::threads, true>, static_cast<unsigned>(nb),
-                      static_cast<unsigned>(Lay::threads), Lay::dynamic, s, ev0, ev1, p, dt, mt, ph, dmap, dent,
-                      reinterpret_cast<const int4*>(stars), nstars, ag, static_cast<const int*>(ccode));
-            return;
-          }
+        if (compact) {
+          if (!ccode) ccode = upload(compact_codes(tab.ang));
+          launch_oz(oz_star_ct_kernel<T, MMAX, MRSO, Lay::threads, true>, static_cast<unsigned>(nb),
+                    static_cast<unsigned>(Lay::threads), Lay::dynamic, s, ev0, ev1, p, dt, mt, ph, dmap, dent,
+                    reinterpret_cast<const int4*>(stars), nstars, ag, static_cast<const int*>(ccode));
+          return;
         }
         launch_oz(oz_star_ct_kernel<T, MMAX, MRSO, Lay::threads, false>, static_cast<unsigned>(nb),
                   static_cast<unsigned>(Lay::threads), Lay::dynamic, s, ev0, ev1, p, dt, mt, ph, dmap, dent,
@@ -1552,11 +1550,12 @@ template void ConvDevice::oz_pass<float>(cplx<float>*, double, hipStream_t, int,
 template void ConvDevice::execute<double>(cplx<double>*, double, hipStream_t, StageTimer*, int, bool) const;
 template void ConvDevice::execute<float>(cplx<float>*, double, hipStream_t, StageTimer*, int, bool) const;
 
-bool ConvDevice::compact_supported() const {
+bool ConvDevice::compact_supported(bool fp64) const {
   if (npairs == 0 || tab.ang.np > 4 * kWave || !padw) return false;
   bool ok = false;
   dispatch_oz(tab.mmax, tab.mrso, [&](auto mm, auto mr) {
-    ok = StarLayout<float, decltype(mm)::value, decltype(mr)::value>::fits;
+    constexpr int MM = decltype(mm)::value, MR = decltype(mr)::value;
+    ok = fp64 ? StarLayout<double, MM, MR>::fits : StarLayout<float, MM, MR>::fits;
   });
   return ok;
 }

@@ -59,7 +59,7 @@ struct ConvDevice {
   // nbatch > 1, p holds nbatch fields as (np, nbatch, nx, ny, nz).  With
   // events, the launch records its own start and end times in ev0 and ev1.
   // With compact, p holds the stored channels of the compact channel order
-  // (compact_projections; FP32 and the star kernel only, compact_supported):
+  // (compact_projections; the star kernel only, compact_supported):
   // implied inputs are read from their partners, and each partner's channel
   // receives the pair's combination g_+ + (-1)^mup conj(g_- at -q).
   template <typename T>
@@ -70,8 +70,9 @@ struct ConvDevice {
   template <typename T>
   void execute(cplx<T>* p, double chop_eps, hipStream_t s, StageTimer* timer = nullptr, int nbatch = 1,
                bool compact = false) const;
-  // whether oz_pass takes the compact channel order for FP32 projections
-  bool compact_supported() const;
+  // whether oz_pass takes the compact channel order for FP64 (fp64) or FP32
+  // projections
+  bool compact_supported(bool fp64) const;
 };
 
 // (ix, iy, iz) triples of the reference's (q, -q) list with the same q / -q

@@ -163,7 +163,7 @@ constexpr int a2p_ct_rows() {
   return sizeof(TC) == 8 ? 4 : 8;
 }
 
-template <int MMAX, int MRSO, typename TI, typename TP, typename TC>
+template <int MMAX, int MRSO, typename TI, typename TP, typename TC, bool CP>
 __global__ __launch_bounds__(kThreads) void angl2proj_ct_kernel(const TI* __restrict__ o, cplx<TP>* __restrict__ p,
                                                                 long N, TabPtrs<TC> tb, int fuse, double rho0) {
   constexpr int kRT = a2p_ct_rows<TC>();
@@ -244,11 +244,13 @@ __global__ __launch_bounds__(kThreads) void angl2proj_ct_kernel(const TI* __rest
   }
   __syncthreads();
 
-  // p[ip][r] = sum_theta W[theta][ip] Y[r][theta][mup(ip)][mu2(ip)]
+  // p[ip][r] = sum_theta W[theta][ip] Y[r][theta][mup(ip)][mu2(ip)] (the
+  // compact order: the stored projections, at their compact channels)
   for (int it = tid; it < NP * rv; it += kThreads) {
     const int ip = it / rv, r = it - ip * rv;
     const int info = tb.ipinfo[ip];
     const int imup = info & 0xff, mu2 = (info >> 8) & 0xff;
+    if (CP && implied_projection(imup - MMAX, mu2)) continue;
     const cplx<TC>* ys = Y + (r * NTH * NPHI + imup) * NMU2 + mu2;
     TC re = 0, im = 0;
 #pragma unroll
@@ -258,7 +260,8 @@ __global__ __launch_bounds__(kThreads) void angl2proj_ct_kernel(const TI* __rest
       re = fma(wv, y.x, re);
       im = fma(wv, y.y, im);
     }
-    p[static_cast<long>(ip) * N + r0 + r] = mkc<TP>(static_cast<TP>(re), static_cast<TP>(im));
+    const int ch = CP ? compact_of(ip, (info >> 16) & 0xff, imup - MMAX, mu2) : ip;
+    p[static_cast<long>(ch) * N + r0 + r] = mkc<TP>(static_cast<TP>(re), static_cast<TP>(im));
   }
 }
 
@@ -823,7 +826,7 @@ long wave_blocks(const ProjTables& t, long N) {
 
 template <typename TI, typename TP, typename TC>
 void angl2proj_factorized(const ProjTables& t, long N, const TI* o, cplx<TP>* p, const double* drho_rho0,
-                          hipStream_t s) {
+                          hipStream_t s, bool compact) {
   if (N == 0) return;
   const long nblk = (N + kRT - 1) / kRT;
   throw_if(nblk > 0x7fffffffL, "grid too large");
@@ -835,9 +838,14 @@ void angl2proj_factorized(const ProjTables& t, long N, const TI* o, cplx<TP>* p,
     constexpr size_t kLds = (Z::kRegA + Z::kNth * Z::kNp()) * sizeof(TC) +
                             (Z::kCells + Z::kNphi + Z::kNmu2 * Z::kNpsi) * sizeof(cplx<TC>);
     if constexpr (kLds <= 64 * 1024) {  // else the run-time kernel below
-      hipLaunchKernelGGL((angl2proj_ct_kernel<MMAX, MRSO, TI, TP, TC>), dim3(static_cast<unsigned>(nblk_ct)),
-                         dim3(kThreads), 0, s, o, p, N, tabs_of<TC>(t, true), drho_rho0 ? 1 : 0,
-                         drho_rho0 ? *drho_rho0 : 0.0);
+      if (compact)
+        hipLaunchKernelGGL((angl2proj_ct_kernel<MMAX, MRSO, TI, TP, TC, true>), dim3(static_cast<unsigned>(nblk_ct)),
+                           dim3(kThreads), 0, s, o, p, N, tabs_of<TC>(t, true), drho_rho0 ? 1 : 0,
+                           drho_rho0 ? *drho_rho0 : 0.0);
+      else
+        hipLaunchKernelGGL((angl2proj_ct_kernel<MMAX, MRSO, TI, TP, TC, false>), dim3(static_cast<unsigned>(nblk_ct)),
+                           dim3(kThreads), 0, s, o, p, N, tabs_of<TC>(t, true), drho_rho0 ? 1 : 0,
+                           drho_rho0 ? *drho_rho0 : 0.0);
       compiled = true;
     }
   });
@@ -845,6 +853,7 @@ void angl2proj_factorized(const ProjTables& t, long N, const TI* o, cplx<TP>* p,
     MDFT_HIP_CHECK(hipGetLastError());
     return;
   }
+  throw_if(compact, "angl2proj: the compact channel order needs a compile-time angular grid");
   const size_t lds = a2p_lds<TC>(t);
   throw_if(lds > 160 * 1024, "angular grid too large for the projection tile");
   hipLaunchKernelGGL((angl2proj_kernel<TI, TP, TC>), dim3(static_cast<unsigned>(nblk)), dim3(kThreads), lds, s, o, p,
@@ -861,6 +870,25 @@ bool proj2angl_has_column_sums(const ProjTables& t) { return dispatch_angular(t.
 
 bool proj2angl_compiled(int mmax, int mrso) { return dispatch_angular(mmax, mrso, [](auto, auto) {}); }
 
+bool angl2proj_compiled(const ProjTables& t, bool fp64) {
+  bool compiled = false;
+  auto probe = [&](auto tc) {
+    using TC = decltype(tc);
+    dispatch_angular(t.mmax, t.mrso, [&](auto mm, auto mr) {
+      constexpr int MMAX = decltype(mm)::value, MRSO = decltype(mr)::value, RT = a2p_ct_rows<TC>();
+      using Z = A2pSizes<MMAX, MRSO, RT>;
+      constexpr size_t kLds = (Z::kRegA + Z::kNth * Z::kNp()) * sizeof(TC) +
+                              (Z::kCells + Z::kNphi + Z::kNmu2 * Z::kNpsi) * sizeof(cplx<TC>);
+      compiled = kLds <= 64 * 1024;
+    });
+  };
+  if (fp64)
+    probe(double{});
+  else
+    probe(float{});
+  return compiled;
+}
+
 long proj2angl_row_blocks(long N) { return wave_row_blocks(N); }
 
 template <typename TP, typename TO, typename TC>
@@ -869,13 +897,10 @@ void proj2angl_factorized(const ProjTables& t, long N, const cplx<TP>* p, TO* o,
   if (N == 0) return;
   const long nwb = wave_blocks(t, N);
   throw_if(nwb > 0x7fffffffL, "grid too large");
-  // the compact channel order is compiled for FP32 projections (the mixed forms)
-  constexpr bool kCompactOk = std::is_same<TP, float>::value && std::is_same<TC, float>::value;
-  throw_if(compact && !kCompactOk, "proj2angl: the compact channel order takes FP32 projections");
   const bool compiled = dispatch_angular(t.mmax, t.mrso, [&](auto mm, auto mr) {
     constexpr int MM = decltype(mm)::value, MR = decltype(mr)::value;
-    if (kCompactOk && compact)
-      hipLaunchKernelGGL((proj2angl_wave_kernel<MM, MR, TP, TO, TC, kCompactOk>), dim3(static_cast<unsigned>(nwb)),
+    if (compact)
+      hipLaunchKernelGGL((proj2angl_wave_kernel<MM, MR, TP, TO, TC, true>), dim3(static_cast<unsigned>(nwb)),
                          dim3(64), 0, s, p, o, xi, df, t.w, N, wave_row_blocks(N), tabs_of<TC>(t, false), rho0,
                          partials, colpart);
     else
@@ -937,11 +962,11 @@ void fill_uniform(T* x, size_t n, unsigned long long seed, unsigned long long of
 }
 
 template void angl2proj_factorized<double, double, double>(const ProjTables&, long, const double*, cplx<double>*,
-                                                           const double*, hipStream_t);
+                                                           const double*, hipStream_t, bool);
 template void angl2proj_factorized<float, float, float>(const ProjTables&, long, const float*, cplx<float>*,
-                                                        const double*, hipStream_t);
+                                                        const double*, hipStream_t, bool);
 template void angl2proj_factorized<double, float, float>(const ProjTables&, long, const double*, cplx<float>*,
-                                                         const double*, hipStream_t);
+                                                         const double*, hipStream_t, bool);
 template void proj2angl_factorized<double, double, double>(const ProjTables&, long, const cplx<double>*, double*,
                                                            const double*, double*, double, double*, hipStream_t,
                                                            double*, bool);

@@ -67,6 +67,12 @@ __host__ __device__ constexpr int compact_index(int ip) {
       }
   return -1;
 }
+// the compact channel of a stored canonical projection ip = (m, mup, mu2):
+// ip less the implied projections before it (m (m - 1) / 2 in the blocks of
+// smaller m, and those of its own m block that precede it)
+__host__ __device__ constexpr int compact_of(int ip, int m, int mup, int mu2) {
+  return ip - m * (m - 1) / 2 - (mup >= 0 ? m : mup + m + (mu2 > 0 ? 1 : 0));
+}
 // the same as a table, for kernels that index it with unrolled loop counters
 template <int MMAX, int MRSO>
 struct CompactMap {
@@ -85,9 +91,14 @@ std::vector<int> compact_codes(const AngularGrid& g);
 // angl2proj (common/MDFT_OrientationProjectionTransform.hpp:267-380).  With
 // drho_rho0 != nullptr the input is xi and delta_rho = rho0 (xi^2 - 1) is
 // formed on load (common/MDFT_Solvent.hpp:575-596 fused).
+// With compact, only the stored projections of the compact channel order are
+// written, at their compact channels (compile-time grids, angl2proj_compiled).
 template <typename TI, typename TP, typename TC>
 void angl2proj_factorized(const ProjTables& t, long N, const TI* o, cplx<TP>* p, const double* drho_rho0,
-                          hipStream_t s);
+                          hipStream_t s, bool compact = false);
+// whether angl2proj_factorized has a compile-time kernel for this grid with
+// FP64 (fp64) or FP32 arithmetic
+bool angl2proj_compiled(const ProjTables& t, bool fp64);
 
 // proj2angl (:388-484).  With xi != nullptr the orientation output is not
 // stored: get_delta_f (common/MDFT_Solvent.hpp:613-660) runs in the epilogue,
