@@ -86,8 +86,8 @@ struct mdft_solver_s {
   std::unique_ptr<ProjTables> tab;
   std::unique_ptr<ProjGemm> gemm;
   // the fused evaluation in the compact channel order (the implied mu = 0,
-  // mup < 0 projections neither stored nor transformed; MDFT_MIXED_X6 and
-  // MDFT_MIXED_X6F where the grid allows, on by default)
+  // mup < 0 projections neither stored nor transformed; every form but
+  // MDFT_MIXED where the grid allows, on by default)
   bool compact = false;
   void* pbuf = nullptr;  // projections, np x N complex (compact: the first channels)
   double* partials = nullptr;
