@@ -552,7 +552,7 @@ def main():
     over_peak = [k for st in (stages, stages6) for k, v in st.items() if v["GBs"] > HBM_PEAK_GBS]
     if over_peak:
         print(f"[bench] WARNING: stages above the HBM peak by algorithmic bytes: {over_peak}", file=sys.stderr)
-    by64 = stage_bytes(N, no, np_, "fp64")
+    by64 = stage_bytes(N, no, np_, "fp64", f64["np_stored"])
     stages64 = {k: dict(ms=round(ms, 5), GBs=round(by64[k] / (ms * 1e-3) / 1e9, 1))
                 for k, ms in f64["stages"].items()}
     eval_bytes = sum(by.values())
