@@ -238,9 +238,8 @@ int mdft_solver_stage_ms(mdft_solver_t h, double* ms, long* count);
  * transform them, the OZ pass reads an implied input from its partner at the
  * mirrored point and stores the pair's combination in the partner's channel,
  * and proj2angl takes the implied sums as zero.  Nothing is assumed of c(q).
- * On by default where the grid has the compile-time projection and OZ
- * kernels (every form but MDFT_MIXED, whose proj2angl runs on the matrix
- * cores); results change by rounding only.
+ * On by default in every form where the grid has the compile-time projection
+ * (or GEMM) and OZ kernels; results change by rounding only.
  * mdft_solver_compact reports whether it is on. */
 int mdft_solver_set_compact(mdft_solver_t h, int on);
 int mdft_solver_compact(mdft_solver_t h, int* on);

@@ -49,7 +49,7 @@ bool is_mixed(int precision) {
 
 std::unique_ptr<ProjGemm> make_gemm(const AngularGrid& g, int precision, bool compact = false) {
   if (!is_mixed(precision)) return nullptr;
-  return precision == MDFT_MIXED ? std::make_unique<ProjGemm>(g, 2, 2, true)
+  return precision == MDFT_MIXED ? std::make_unique<ProjGemm>(g, 2, 2, true, compact)
                                  : std::make_unique<ProjGemm>(g, 3, 3, false, compact);
 }
 
@@ -86,8 +86,8 @@ struct mdft_solver_s {
   std::unique_ptr<ProjTables> tab;
   std::unique_ptr<ProjGemm> gemm;
   // the fused evaluation in the compact channel order (the implied mu = 0,
-  // mup < 0 projections neither stored nor transformed; every form but
-  // MDFT_MIXED where the grid allows, on by default)
+  // mup < 0 projections neither stored nor transformed; every form where the
+  // grid allows, on by default)
   bool compact = false;
   void* pbuf = nullptr;  // projections, np x N complex (compact: the first channels)
   double* partials = nullptr;
@@ -115,15 +115,16 @@ struct mdft_solver_s {
 namespace {
 
 // whether the fused evaluation of this form can take the compact channel
-// order: the BF16x6 GEMM angl2proj (X6 forms) or the compile-time angl2proj
-// (FP64, FP32), the compile-time proj2angl and the star OZ kernel
+// order: the GEMM angl2proj (mixed forms) or the compile-time angl2proj
+// (FP64, FP32), the GEMM or compile-time proj2angl and the star OZ kernel
 bool compact_available(int precision, const mdft_conv_s* conv, const ProjTables& tab) {
   const AngularGrid& g = conv->dev->tab.ang;
   if (compact_projections(g) == g.np) return false;
   switch (precision) {
+    case MDFT_MIXED:
     case MDFT_MIXED_X6:
     case MDFT_MIXED_X6F:
-      return ProjGemm::compact_possible(g, false) && conv->dev->compact_supported(false);
+      return ProjGemm::compact_possible(g, precision == MDFT_MIXED) && conv->dev->compact_supported(false);
     case MDFT_FP64:
     case MDFT_FP32:
       return angl2proj_compiled(tab, precision == MDFT_FP64) && proj2angl_compiled(g.mmax, g.mrso) &&

@@ -558,6 +558,14 @@ ProjGemm::ProjGemm(const AngularGrid& g, int data_parts, int weight_parts, bool 
         wc.insert(wc.end(), wa.begin() + static_cast<size_t>(2 * ip) * no, wa.begin() + static_cast<size_t>(2 * ip + 2) * no);
     throw_if(wc.size() != static_cast<size_t>(2 * np) * no, "internal error: compact angl2proj rows");
     wa.swap(wc);
+    // proj2angl: the same rows (a stored channel holds the fold of its pair,
+    // which proj2angl weighs with the stored projection's own row)
+    std::vector<double> pc;
+    pc.reserve(static_cast<size_t>(2 * np) * no);
+    for (int ip = 0; ip < g.np; ++ip)
+      if (!(code[ip] & kCompactImplied))
+        pc.insert(pc.end(), wp.begin() + static_cast<size_t>(2 * ip) * no, wp.begin() + static_cast<size_t>(2 * ip + 2) * no);
+    wp.swap(pc);
   }
   if (I.a2p_gemm())
     dispatch_split(I.px, I.pw, [&](auto, auto pwc) {
@@ -623,10 +631,12 @@ bool ProjGemm::has_column_sums() const { return !impl_->p2a_gemm() && proj2angl_
 bool ProjGemm::compact() const { return impl_->compact; }
 
 bool ProjGemm::compact_possible(const AngularGrid& g, bool p2a_mfma) {
-  // the GEMM angl2proj (register tiles) and the compile-time FP32 proj2angl
+  // the GEMM angl2proj (register tiles), and the GEMM or the compile-time
+  // FP32 proj2angl
   const int nc = compact_projections(g);
-  if (p2a_mfma || nc == g.np) return false;
+  if (nc == g.np) return false;
   if (tile_bucket((2 * nc + 15) / 16) > kMaxA2pTiles || g.no < 2) return false;
+  if (p2a_mfma) return tile_bucket((g.no + 15) / 16) <= kMaxP2aTiles;
   return proj2angl_compiled(g.mmax, g.mrso);
 }
 

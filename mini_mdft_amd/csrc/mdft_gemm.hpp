@@ -24,7 +24,7 @@ struct ProjGemm {
   // (2 np > 256 or no > 384) run the factorised FP32 projections instead.
   // compact: the stored projections of the compact channel order (the
   // implied mu = 0, mup < 0 ones left out; compact_codes), both ways; needs
-  // the GEMM angl2proj and the compile-time FP32 proj2angl
+  // the GEMM angl2proj and the GEMM or the compile-time FP32 proj2angl
   ProjGemm(const AngularGrid& g, int data_parts, int weight_parts, bool p2a_mfma, bool compact = false);
   ~ProjGemm();
   ProjGemm(const ProjGemm&) = delete;

@@ -340,8 +340,8 @@ class Solver:
         check(lib().mdft_solver_set_mean_fp64(self._h, 1 if on else 0))
 
     def set_compact(self, on: bool) -> None:
-        """The compact channel order of the fused evaluation (default on where
-        the grid allows, every form but MDFT_MIXED): the mu = 0,
+        """The compact channel order of the fused evaluation (default on in
+        every form where the grid allows): the mu = 0,
         mup < 0 projections, implied by their partners in the real delta_rho
         and folded into them after the OZ pass, are neither stored nor
         transformed; results change by rounding only."""

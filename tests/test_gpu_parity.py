@@ -233,14 +233,14 @@ def test_energy_reduced_precision(oracle, prec, tol, tol_df):
     assert rel_err(_np(df).astype(np.float64), df_ref) < tol_df
 
 
-@pytest.mark.parametrize("prec,tol,tol_ref", [(FP64, 1e-12, 1e-10), (FP32, 1e-4, 1e-3), (MIXED_X6, 1e-5, 1e-5),
-                                              (MIXED_X6F, 1e-5, 1e-5)])
+@pytest.mark.parametrize("prec,tol,tol_ref", [(FP64, 1e-12, 1e-10), (FP32, 1e-4, 1e-3), (MIXED, 1e-4, 1e-3),
+                                              (MIXED_X6, 1e-5, 1e-5), (MIXED_X6F, 1e-5, 1e-5)])
 @pytest.mark.parametrize("n,mmax", [((8, 8, 8), 5), ((6, 5, 4), 3), ((4, 4, 4), 2)])
 def test_compact_channel_order(oracle, prec, tol, tol_ref, n, mmax):
     """The compact channel order (the implied mu = 0, mup < 0 projections
     neither stored nor transformed; on by default) agrees with the full order
-    to rounding, and with the oracle to the form's tolerance; the BF16x3 form,
-    whose proj2angl runs on the matrix cores, has no compact order."""
+    to rounding, and with the oracle to the form's tolerance; a grid without
+    the compile-time kernels (mmax 4, mrso 1) has no compact order."""
     g, ag, mp, tab, ocv = _problem(oracle, n, (9.0, 8.0, 7.0), mmax, seed=5)
     n0 = M.SOLVENTS["spce"]["n0"]
     rho0, pref = M.rho0_of(n0, MRSO), M.prefactor_of(298.15, MRSO, n0)
@@ -260,10 +260,11 @@ def test_compact_channel_order(oracle, prec, tol, tol_ref, n, mmax):
     assert rel_err(out[True][1], out[False][1]) < tol
     assert abs(out[True][0] - ff_ref) <= tol_ref * scale
     assert rel_err(out[True][1], df_ref) < tol_ref
-    s3 = M.Solver(M.Convolution.from_table(tab, g, ag, mp, MIXED), rho0, pref, MIXED)
-    assert not s3.compact
+    g1, ag1, mp1, tab1, _ = _problem(oracle, (4, 4, 4), (5.0,) * 3, 4, seed=5, mrso=1)
+    s1 = M.Solver(M.Convolution.from_table(tab1, g1, ag1, mp1, prec), rho0, pref, prec)
+    assert not s1.compact
     with pytest.raises(M.MDFTError, match="compact"):
-        s3.set_compact(True)
+        s1.set_compact(True)
 
 
 def test_energy_equals_stagewise_api(oracle):
