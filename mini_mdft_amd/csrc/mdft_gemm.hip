@@ -61,11 +61,16 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kWaves = 4;  // waves per block: rows sharing one weight slice
 constexpr int kThreads = 64 * kWaves;
 constexpr int kK = 32;  // K per MFMA
-// angl2proj: row groups of 16 per wave (one group keeps the kernel at 140
-// registers, three waves per SIMD; two groups share each LDS weight fragment
-// over twice the rows but need 228 registers and measured 4% slower)
-constexpr int kA2pGroups = 1;
-constexpr int kA2pRowsPerWave = 16 * kA2pGroups;
+// angl2proj: row groups of 16 per wave.  Two groups share each LDS weight
+// fragment over twice the rows; with the 10 output tiles of the compact
+// channel order (mmax 5) they measured 0.163 against 0.182 ms for one group
+// and 0.171 / 0.209 ms for three / four.  With the full order's 12 tiles two
+// groups needed 228 registers and measured 4% slower, so wider tile counts
+// keep one group.
+template <int MT>
+constexpr int a2p_groups() {
+  return MT <= 10 ? 2 : 1;
+}
 constexpr int kP2aRowsPerWave = 16;
 constexpr int kMaxA2pTiles = 16;  // 2 np <= 256
 constexpr int kMaxP2aTiles = 24;  // no <= 384
@@ -186,8 +191,8 @@ struct Stage {
   }
 };
 
-// P^T = Wa . delta_rho^T.  Block: kA2pRowsPerWave kWaves consecutive r; wave
-// w: r0 + kA2pRowsPerWave w + (0 .. kA2pRowsPerWave - 1), in groups u of 16.
+// P^T = Wa . delta_rho^T.  Block: 16 G kWaves consecutive r, G = a2p_groups<MT>;
+// wave w: r0 + 16 G w + (0 .. 16 G - 1), in groups u of 16.
 // TI: the element type of o / xi (double, or float for the FP32 fields of
 // MDFT_MIXED_X6F; delta_rho is formed in FP64 either way).
 // EVEN: no is even, so a row's K pairs are aligned two-element loads.
@@ -195,9 +200,11 @@ template <typename TI, int PX, int PW, int MT, bool FUSE, bool EVEN>
 __global__ __launch_bounds__(kThreads) void a2p_gemm_kernel(const TI* __restrict__ o, float2* __restrict__ p,
                                                             const u32x4* __restrict__ wsw, GemmDims d, double rho0) {
   using St = Stage<PW, MT>;
+  constexpr int kA2pGroups = a2p_groups<MT>();
+  constexpr int kRows = 16 * kA2pGroups;
   __shared__ u32x4 lds[St::kPer * kThreads];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const long r0 = static_cast<long>(blockIdx.x) * (kWaves * kA2pRowsPerWave) + wave * kA2pRowsPerWave;
+  const long r0 = static_cast<long>(blockIdx.x) * (kWaves * kRows) + wave * kRows;
   const int slice = MT * 64 * PW;  // u32x4 per K-step in global memory
   f32x4 acc[MT][kA2pGroups];
 #pragma unroll
@@ -588,11 +595,9 @@ void ProjGemm::angl2proj_t(long N, const TI* o, cplx<float>* p, const double* rh
     return;
   }
   const GemmDims d{N, I.no, I.np, I.a_kt};
-  const long rows = kWaves * kA2pRowsPerWave;
-  const long nblk = (N + rows - 1) / rows;
-  throw_if(nblk > 0x7fffffffL, "grid too large");
   const double r0 = rho0 ? *rho0 : 0.0;
   const bool even = I.no % 2 == 0;
+  long nblk = 0;
   auto launch = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(nblk)), dim3(kThreads), 0, s, o, p, I.wa, d, r0);
   };
@@ -600,6 +605,9 @@ void ProjGemm::angl2proj_t(long N, const TI* o, cplx<float>* p, const double* rh
     constexpr int PX = decltype(pxc)::value, PW = decltype(pwc)::value;
     dispatch_tiles<kMaxA2pTiles>(I.a_mt, [&](auto mt) {
       constexpr int MT = decltype(mt)::value;
+      const long rows = kWaves * 16 * a2p_groups<MT>();
+      nblk = (N + rows - 1) / rows;
+      throw_if(nblk > 0x7fffffffL, "grid too large");
       if (rho0)
         even ? launch(a2p_gemm_kernel<TI, PX, PW, MT, true, true>)
              : launch(a2p_gemm_kernel<TI, PX, PW, MT, true, false>);
