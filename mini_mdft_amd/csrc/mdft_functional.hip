@@ -16,8 +16,12 @@
 // The minimiser is L-BFGS with the diagonal initial inverse Hessian gamma h0,
 // h0 = 1 / (2 rho0 dv w kT), a backtracking line search that accepts Armijo
 // or the approximate Wolfe test of Hager and Zhang (SIAM J. Optim. 16 (2005)
-// 170-192), and restarts on a non-descent direction.  It runs in the vector-free form of Chen, Wang, Zhou
-// and Gao ("Large-scale L-BFGS using MapReduce", NIPS 2014): the two-loop
+// 170-192), and restarts on a non-descent direction.  After a rejected trial
+// t the next is the minimiser of the quadratic through F(0), F'(0) and F(t),
+// kept within [0.1 t, 0.5 t] (Nocedal and Wright, Numerical Optimization,
+// 2nd ed., section 3.5); a trial where F is not finite halves t.  It runs in
+// the vector-free form of Chen, Wang, Zhou and Gao ("Large-scale L-BFGS using
+// MapReduce", NIPS 2014): the two-loop
 // recursion is carried out on the host in the coefficients of the direction
 // over the basis {s_i, y_i, g}, from the Gram entries s_i.y_j, y_i'H y_j,
 // s_i.g, y_i'H g, g'H g (H = diag(h0)).  Every iteration then makes two
@@ -466,7 +470,7 @@ void LbfgsState::start(const double F[4], double ghg, double gmax) {
 bool LbfgsState::next(Trial& tr) {
   if (phase_ == kDone) return false;
   if (phase_ == kBacktrack) {
-    cur_.t *= 0.5;
+    cur_.t = t_next_;
     tr = cur_;
     return true;
   }
@@ -538,6 +542,12 @@ bool LbfgsState::judge(const double Fn[4], const double* h, int K) {
   const bool ok = std::isfinite(Fn[0]) && (Fn[0] <= F_[0] + o_.c1 * t * gd_ ||
                                            (Fn[0] <= F_[0] + o_.eps_f * fscale && gtd <= (2 * o_.c1 - 1) * gd_));
   if (!ok) {
+    // quadratic interpolation: phi(0) = F, phi'(0) = gd, phi(t) = Fn; a
+    // rejected finite trial fails Armijo, so den > 0 there
+    const double den = 2.0 * (Fn[0] - F_[0] - gd_ * t);
+    double tn = 0.5 * t;
+    if (std::isfinite(Fn[0]) && den > 0) tn = std::min(std::max(-gd_ * t * t / den, 0.1 * t), 0.5 * t);
+    t_next_ = tn;
     if (tries_ >= o_.max_backtrack) {
       r_.status = 2;
       r_.iterations = it_;

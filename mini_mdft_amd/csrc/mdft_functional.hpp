@@ -80,6 +80,7 @@ class LbfgsState {
   enum Phase { kNew, kBacktrack, kDone } phase_ = kNew;
   double F_[4] = {0, 0, 0, 0};
   double ghg_ = 0.0, gmax_ = 0.0, gd_ = 0.0;
+  double t_next_ = 0.5;  // the step of the next trial after a rejected one
   int it_ = 0, tries_ = 0;
   Trial cur_;
   std::vector<int> order_;

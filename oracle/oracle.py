@@ -383,8 +383,11 @@ def lbfgs(fg, x0: np.ndarray, h0: np.ndarray, m: int = 5, max_iter: int = 200, g
     """The L-BFGS of this build (FP64): two-loop recursion with the diagonal
     initial inverse Hessian gamma h0, gamma = s'y / y'(h0 y) of the newest
     pair; a pair is kept when s'y > 1e-12 |s| |y|; a non-descent direction
-    restarts from -h0 g.  Line search: backtracking by halving from t = 1 (at
-    most max_backtrack trials) until the trial point x + t d satisfies Armijo,
+    restarts from -h0 g.  Line search: backtracking from t = 1 (at most
+    max_backtrack trials; after a rejected trial t the next is the minimiser of
+    the quadratic through F(0), F'(0) = g.d and F(t), kept within [0.1 t, 0.5 t]
+    (Nocedal and Wright, Numerical Optimization, 2nd ed., section 3.5), or t/2
+    where F(t) is not finite) until the trial point x + t d satisfies Armijo,
     F(t) <= F + c1 t g.d, or the approximate Wolfe test of Hager and Zhang
     (SIAM J. Optim. 16 (2005) 170-192, their (23) with delta = c1):
     g(t).d <= (2 c1 - 1) g.d and F(t) <= F + eps_f S, S the scale of F.  The
@@ -433,7 +436,13 @@ def lbfgs(fg, x0: np.ndarray, h0: np.ndarray, m: int = 5, max_iter: int = 200, g
             if np.isfinite(Fn) and (Fn <= F + c1 * t * gd or
                                     (Fn <= F + eps_f * scale and gn @ d <= (2 * c1 - 1) * gd)):
                 break
-            t *= 0.5
+            # quadratic through F(0), F'(0) = g.d and F(t), kept in [0.1 t, 0.5 t];
+            # halving where F(t) is not finite
+            den = 2.0 * (Fn - F - gd * t)
+            if np.isfinite(Fn) and den > 0:
+                t = min(max(-gd * t * t / den, 0.1 * t), 0.5 * t)
+            else:
+                t = 0.5 * t
         else:
             return x, F, g, it, nev, hist, 2
         s, y = xn - x, gn - g

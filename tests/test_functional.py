@@ -247,7 +247,9 @@ def test_lbfgs_approximate_wolfe_carries_a_noisy_f(oracle):
     """With F perturbed by a deterministic noise of 1e-9 of its scale (the
     rounding of a lower-precision evaluation) and the gradient exact, the
     slope test keeps the line search decisive and the solve converges; with
-    that test switched off (eps_f = -inf) Armijo alone stalls."""
+    that test switched off (eps_f = -inf) Armijo alone stalls: it ends without
+    converging, out of acceptable steps (status 2) or out of iterations on
+    steps that F's noise accepts (status 1)."""
     s, w, conv, V = _setup(oracle)
     h0 = np.broadcast_to(1.0 / (2 * RHO0 * DV * w * KT), (NPTS, s["no"])).ravel()
 
@@ -261,7 +263,7 @@ def test_lbfgs_approximate_wolfe_carries_a_noisy_f(oracle):
     res = oracle.lbfgs(fg, x0, h0, m=5, max_iter=400, gtol=1e-7)
     assert res[6] == 0
     res = oracle.lbfgs(fg, x0, h0, m=5, max_iter=400, gtol=1e-7, eps_f=-np.inf)
-    assert res[6] == 2 and np.abs(h0 * res[2]).max() > 1e-7
+    assert res[6] in (1, 2) and np.abs(h0 * res[2]).max() > 1e-7
 
 
 # ---- the library (C ABI) against the oracle -----------------------------------
