@@ -291,14 +291,16 @@ int mdft_functional_eval(mdft_functional_t h, const double* xi, double* g, doubl
 /* L-BFGS (FP64 iterate, gradient, dot products and line search) from xi, in
  * place: m stored pairs (1..8; FP32 storage for the mixed forms), diagonal
  * initial inverse Hessian gamma h0 with h0 = 1 / (2 rho0 dv w kT) and
- * gamma = s'y / y'(h0 y) of the newest pair, backtracking from t = 1 (at
- * most 30 trials; after a rejected trial t the next is the minimiser of the
+ * gamma = s'y / y'(h0 y) of the newest pair, backtracking from t = 1, or
+ * from t = min(1, 1 / max |h0 g|) along -h0 g (no stored pair: no xi moves
+ * by more than 1) (at most 30 trials; after a rejected trial t the next is the minimiser of the
  * quadratic through F(0), F'(0) and F(t), kept within [0.1 t, 0.5 t], or t/2
  * where F(t) is not finite) until Armijo (c1 = 1e-4) or the approximate Wolfe
  * test of Hager and Zhang holds (g(t).d <= (2 c1 - 1) g.d with F(t) <= F +
  * 1e-6 (|F_id| + |F_ext| + |F_exc|): Armijo read off the slopes, which stays
  * decisive where F's rounding hides the decrease), a pair kept when
- * s'y > 1e-12 |s| |y|, a restart from -h0 g on a non-descent direction;
+ * s'y > 1e-12 |s| |y| (a full memory whose new pair fails this restarts), a
+ * restart from -h0 g on a non-descent direction;
  * converged when max |h0 g| < gtol.  Outputs (host; history may be
  * NULL, else max_iter + 1 doubles receiving F at the start and after every
  * iteration): F[4] at the returned point, the iterations made, the

@@ -35,16 +35,16 @@ __device__ __forceinline__ void st_nt(T* p, T v) {
   __builtin_nontemporal_store(v, p);
 }
 
-__device__ __forceinline__ double local_grad(double x, double v, double df, double wio, const Elem& e) {
+// the local terms at one element, from one ln x2: accumulates
+// w (x2 ln x2 - x2 + 1) into fid and w x2 V into fext, and returns the
+// gradient dv w (2 rho0 x (kT ln x2 + V) + gscale df)
+__device__ __forceinline__ double local_terms(double x, double v, double df, double wio, const Elem& e, double& fid,
+                                              double& fext) {
   const double x2 = fmax(x * x, 1e-300);
-  return (e.dv * wio) * (2.0 * e.rho0 * x * (e.kT * log(x2) + v) + e.gscale * df);
-}
-
-// accumulates w (x2 ln x2 - x2 + 1) and w x2 V
-__device__ __forceinline__ void local_f(double x, double v, double wio, double& fid, double& fext) {
-  const double x2 = fmax(x * x, 1e-300);
-  fid += wio * (x2 * log(x2) - x2 + 1.0);
+  const double l = log(x2);
+  fid += wio * (x2 * l - x2 + 1.0);
   fext += wio * x2 * v;
+  return (e.dv * wio) * (2.0 * e.rho0 * x * (e.kT * l + v) + e.gscale * df);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {

@@ -52,7 +52,8 @@ constexpr int kSlotFf = kSlotRows;
 template <typename TS>
 struct SlotDesc {
   int mode;
-  int k;  // stored pairs in use (columns k .. KM - 1 read the zero field)
+  int k;   // stored pairs in use (columns k .. KM - 1 read the zero field)
+  int k0;  // the update pass reads pairs k0 .. k0 + KM - 2 (LbfgsState::Trial::k0)
   double cg, gamma, t;
   double cs[kLbfgsMaxPairs], cq[kLbfgsMaxPairs];
   const double* src;  // current point (start: the initial field)
@@ -103,8 +104,9 @@ __global__ __launch_bounds__(kFThreads) void slot_direction_kernel(const SlotDes
 }
 
 // grid (nb, slots): the update pass of Functional's L-BFGS for slot
-// blockIdx.y (rows as in update_kernel with K = KM); at the start point, g
-// and its sums into the same rows (5K+6 g'H g, 5K+7 F_id, 5K+8 F_ext, 5K+9 max)
+// blockIdx.y (rows as in update_kernel with K = KM, KM = m - 1: the pairs
+// from the trial's k0); at the start point, g and its sums into the same rows
+// (5K+6 g'H g, 5K+7 F_id, 5K+8 F_ext, 5K+9 max)
 template <typename TS, int KM, typename TF>
 __global__ __launch_bounds__(kFThreads) void slot_update_kernel(const SlotDesc<TS>* __restrict__ desc,
                                                                 const double* __restrict__ xslab,
@@ -123,8 +125,8 @@ __global__ __launch_bounds__(kFThreads) void slot_update_kernel(const SlotDesc<T
   if (a.mode == kModeStart) {
     MDFT_ELEMENT_LOOP(e) {
       const double xv = ld_nt(&xt[i]), v = ld_nt(&a.V[i]), wio = e.w[io], h = e.h0[io];
-      local_f(xv, v, wio, acc[5 * KM + 7], acc[5 * KM + 8]);
-      const TS gs = static_cast<TS>(local_grad(xv, v, static_cast<double>(ld_nt(&df[i])), wio, e));
+      const double dfv = static_cast<double>(ld_nt(&df[i]));
+      const TS gs = static_cast<TS>(local_terms(xv, v, dfv, wio, e, acc[5 * KM + 7], acc[5 * KM + 8]));
       st_nt(&a.g[i], gs);
       const double gv = static_cast<double>(gs);
       const double hg = h * gv;
@@ -135,8 +137,8 @@ __global__ __launch_bounds__(kFThreads) void slot_update_kernel(const SlotDesc<T
     MDFT_ELEMENT_LOOP(e) {
       const double wio = e.w[io], h = e.h0[io];
       const double xv = ld_nt(&xt[i]), v = ld_nt(&a.V[i]);
-      local_f(xv, v, wio, acc[5 * KM + 7], acc[5 * KM + 8]);
-      const TS gs = static_cast<TS>(local_grad(xv, v, static_cast<double>(ld_nt(&df[i])), wio, e));
+      const double dfv = static_cast<double>(ld_nt(&df[i]));
+      const TS gs = static_cast<TS>(local_terms(xv, v, dfv, wio, e, acc[5 * KM + 7], acc[5 * KM + 8]));
       const double gn = static_cast<double>(gs);
       const TS ys = static_cast<TS>(gn - static_cast<double>(ld_nt(&a.g[i])));
       st_nt(&a.gt[i], gs);
@@ -146,8 +148,8 @@ __global__ __launch_bounds__(kFThreads) void slot_update_kernel(const SlotDesc<T
       double sj[KA], yj[KA];
 #pragma unroll
       for (int j = 0; j < KM; ++j) {
-        sj[j] = static_cast<double>(ld_nt(&a.s[j][i]));
-        yj[j] = static_cast<double>(ld_nt(&a.y[j][i]));
+        sj[j] = static_cast<double>(ld_nt(&a.s[a.k0 + j][i]));
+        yj[j] = static_cast<double>(ld_nt(&a.y[a.k0 + j][i]));
       }
       const double hy = h * yv, hg = h * gn;
 #pragma unroll
@@ -362,7 +364,8 @@ std::vector<Engine::Done> Engine::take_done() {
 
 void Engine::enqueue(int nbatch) {
   const Elem e{tab_, tab_ + p_.no, p_.rho0, p_.kT, p_.dv, 2.0 * p_.prefactor / p_.N, n_, p_.no, step_};
-  const int KM = o_.m, NA = 5 * KM + 10;
+  // the direction reads all m pair columns, the update pass m - 1
+  const int KM = o_.m, KU = o_.m - 1, NA = 5 * KU + 10;
   dispatch_k(KM, [&](auto kc) {
     constexpr int K = decltype(kc)::value;
     if (f32f_) {
@@ -381,7 +384,7 @@ void Engine::enqueue(int nbatch) {
     excess_(nbatch, xfslab_, dfslab_, res_ + kSlotFf, kSlotRes, stream_);
   else
     excess_(nbatch, xslab_, dfslab_, res_ + kSlotFf, kSlotRes, stream_);
-  dispatch_k(KM, [&](auto kc) {
+  dispatch_k(KU, [&](auto kc) {
     constexpr int K = decltype(kc)::value;
     if (f32f_) {
       hipLaunchKernelGGL((slot_update_kernel<float, K, float>), dim3(nb_, nbatch), dim3(kFThreads), 0, stream_,
@@ -398,7 +401,7 @@ void Engine::enqueue(int nbatch) {
     }
   });
   MDFT_HIP_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(slot_reduce_kernel, dim3(NA, nbatch), dim3(kFThreads), 0, stream_, part_, nb_, 5 * KM + 9,
+  hipLaunchKernelGGL(slot_reduce_kernel, dim3(NA, nbatch), dim3(kFThreads), 0, stream_, part_, nb_, 5 * KU + 9,
                      res_);
   MDFT_HIP_CHECK(hipGetLastError());
 }
@@ -421,6 +424,7 @@ void fill_desc(SlotDesc<TS>& d, int mode, const LbfgsState::Trial* tr, const std
   }
   if (!tr) return;
   d.k = tr->k;
+  d.k0 = tr->k0;
   d.cg = tr->cg;
   d.gamma = tr->gamma;
   d.t = tr->t;
@@ -508,8 +512,8 @@ int Engine::end_tick() {
   if (hi == 0) return 0;
   MDFT_HIP_CHECK(hipStreamSynchronize(stream_));
 
-  // the decisions
-  const int KM = o_.m;
+  // the decisions (update-pass rows with m - 1 pair columns)
+  const int KM = o_.m - 1;
   int finished = 0;
   for (int j = 0; j < hi; ++j) {
     Slot* sl = slots_[j].get();

@@ -16,18 +16,23 @@
 // The minimiser is L-BFGS with the diagonal initial inverse Hessian gamma h0,
 // h0 = 1 / (2 rho0 dv w kT), a backtracking line search that accepts Armijo
 // or the approximate Wolfe test of Hager and Zhang (SIAM J. Optim. 16 (2005)
-// 170-192), and restarts on a non-descent direction.  After a rejected trial
-// t the next is the minimiser of the quadratic through F(0), F'(0) and F(t),
-// kept within [0.1 t, 0.5 t] (Nocedal and Wright, Numerical Optimization,
-// 2nd ed., section 3.5); a trial where F is not finite halves t.  It runs in
-// the vector-free form of Chen, Wang, Zhou and Gao ("Large-scale L-BFGS using
-// MapReduce", NIPS 2014): the two-loop
-// recursion is carried out on the host in the coefficients of the direction
-// over the basis {s_i, y_i, g}, from the Gram entries s_i.y_j, y_i'H y_j,
-// s_i.g, y_i'H g, g'H g (H = diag(h0)).  Every iteration then makes two
-// fused passes over HBM: one that forms the trial point x + t d, one (after the
-// excess evaluation there) that forms g, y = g_new - g, the F_id and F_ext sums
-// and all new Gram entries at once.  All sums are FP64 with fixed-order
+// 170-192), and restarts on a non-descent direction.  Trials start at t = 1,
+// or along -h0 g (no stored pair) at t = min(1, 1 / max |h0 g|), so that no
+// xi moves by more than 1.  After a rejected trial t the next is the
+// minimiser of the quadratic through F(0), F'(0) and F(t), kept within
+// [0.1 t, 0.5 t] (Nocedal and Wright, Numerical Optimization, 2nd ed.,
+// section 3.5); a trial where F is not finite halves t.  It runs in the
+// vector-free form of Chen, Wang, Zhou and Gao ("Large-scale L-BFGS using
+// MapReduce", NIPS 2014): the two-loop recursion is carried out on the host
+// in the coefficients of the direction over the basis {s_i, y_i, g}, from the
+// Gram entries s_i.y_j, y_i'H y_j, s_i.g, y_i'H g, g'H g (H = diag(h0)).
+// Every iteration then makes two fused passes over HBM: one that forms the
+// trial point x + t d, one (after the excess evaluation there) that forms g,
+// y = g_new - g, the F_id and F_ext sums and the Gram entries the next
+// direction can use.  With a full memory the oldest pair gets none: a kept
+// candidate drops it, and a candidate that fails the curvature test
+// (s'y <= 1e-12 |s| |y|) restarts the memory, which spares the pass that
+// pair's reads (all of them at m = 1).  All sums are FP64 with fixed-order
 // reductions.
 #include "mdft_functional.hpp"
 #include "mdft_elem.hpp"
@@ -51,8 +56,8 @@ __global__ __launch_bounds__(kFThreads) void local_kernel(const double* __restri
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   MDFT_ELEMENT_LOOP(e) {
     const double xv = ld_nt(&x[i]), v = ld_nt(&V[i]), wio = e.w[io], h = e.h0[io];
-    local_f(xv, v, wio, acc[0], acc[1]);
-    const TG gs = static_cast<TG>(local_grad(xv, v, static_cast<double>(ld_nt(&df[i])), wio, e));
+    const double dfv = static_cast<double>(ld_nt(&df[i]));
+    const TG gs = static_cast<TG>(local_terms(xv, v, dfv, wio, e, acc[0], acc[1]));
     st_nt(&g[i], gs);
     const double gv = static_cast<double>(gs);
     const double hg = h * gv;
@@ -136,8 +141,8 @@ __global__ __launch_bounds__(kFThreads) void update_kernel(UpdArgs<TS, TF> a, El
   MDFT_ELEMENT_LOOP(e) {
     const double wio = e.w[io], h = e.h0[io];
     const double xv = ld_nt(&a.xt[i]), v = ld_nt(&a.V[i]);
-    local_f(xv, v, wio, acc[5 * K + 7], acc[5 * K + 8]);
-    const TS gs = static_cast<TS>(local_grad(xv, v, static_cast<double>(ld_nt(&a.df[i])), wio, e));
+    const double dfv = static_cast<double>(ld_nt(&a.df[i]));
+    const TS gs = static_cast<TS>(local_terms(xv, v, dfv, wio, e, acc[5 * K + 7], acc[5 * K + 8]));
     const double gn = static_cast<double>(gs);
     const TS ys = static_cast<TS>(gn - static_cast<double>(ld_nt(&a.g[i])));
     st_nt(&a.gt[i], gs);
@@ -522,9 +527,11 @@ bool LbfgsState::next(Trial& tr) {
     gd = -ghg_;
   }
   t.k = k;
+  t.k0 = k > 0 && k == o_.m ? 1 : 0;
   t.cand = 0;
   while (std::find(order_.begin(), order_.end(), t.cand) != order_.end()) ++t.cand;
-  t.t = 1.0;
+  // a step along -h0 g (no stored pair) moves no xi by more than 1
+  t.t = k == 0 && gmax_ > 0 ? std::min(1.0, 1.0 / gmax_) : 1.0;
   gd_ = gd;
   tries_ = 0;
   cur_ = t;
@@ -559,19 +566,30 @@ bool LbfgsState::judge(const double Fn[4], const double* h, int K) {
     }
     return false;
   }
-  // the new Gram entries (pair columns beyond the trial's k are not used)
+  // the new Gram entries: pair i >= k0 in column i - k0 (columns beyond are
+  // not used); pairs below k0 have none
   const int m = o_.m;
   int k = cur_.k;
-  std::vector<double> sn_y(h, h + k), s_yn(h + K + 1, h + K + 1 + k), yn_hy(h + 3 * K + 4, h + 3 * K + 4 + k);
+  const int k0 = cur_.k0;
+  const double nan = std::nan("");
+  std::vector<double> sn_y(k, nan), s_yn(k, nan), yn_hy(k, nan);
+  for (int i = k0; i < k; ++i) {
+    const int c = i - k0;
+    sn_y[i] = h[c];
+    s_yn[i] = h[K + 1 + c];
+    yn_hy[i] = h[3 * K + 4 + c];
+    SG_[i] = h[2 * K + 3 + c];
+    YHG_[i] = h[4 * K + 5 + c];
+  }
   const double snyn = h[K], snsn = h[2 * K + 1], ynyn = h[2 * K + 2], sn_g = h[3 * K + 3], ynhyn = h[4 * K + 4],
                yn_hg = h[5 * K + 5];
-  for (int j = 0; j < k; ++j) {
-    SG_[j] = h[2 * K + 3 + j];
-    YHG_[j] = h[4 * K + 5 + j];
-  }
   ghg_ = h[5 * K + 6];
   gmax_ = h[5 * K + 9];
-  if (snyn > 1e-12 * std::sqrt(snsn * ynyn)) {  // keep the pair
+  if (!(snyn > 1e-12 * std::sqrt(snsn * ynyn)) && k0 > 0) {
+    // a full memory whose candidate fails the curvature test: the pairs below
+    // k0 have no entries at the new point, so the memory restarts
+    order_.clear();
+  } else if (snyn > 1e-12 * std::sqrt(snsn * ynyn)) {  // keep the pair
     if (k == m) {  // drop the oldest
       order_.erase(order_.begin());
       for (int i = 0; i + 1 < m; ++i) {
@@ -643,14 +661,19 @@ LbfgsResult Functional::minimize_t(double* xi, const LbfgsOptions& o, hipStream_
   // F_ext sums and the candidate pair's Gram entries (kept if accepted)
   LbfgsState::Trial tr;
   while (st.next(tr)) {
-    const int K = tr.k;
+    // the direction reads the k stored pairs, the update pass the pairs from k0
+    const int K = tr.k, KU = tr.k - tr.k0;
     DirArgs<TS> da{};
     UpdArgs<TS, TF> ua{};
     for (int j = 0; j < K; ++j) {
-      da.s[j] = ua.s[j] = S[st.order()[j]];
-      da.y[j] = ua.y[j] = Y[st.order()[j]];
+      da.s[j] = S[st.order()[j]];
+      da.y[j] = Y[st.order()[j]];
       da.cs[j] = tr.cs[j];
       da.cq[j] = tr.cq[j];
+    }
+    for (int j = 0; j < KU; ++j) {
+      ua.s[j] = S[st.order()[tr.k0 + j]];
+      ua.y[j] = Y[st.order()[tr.k0 + j]];
     }
     da.x = x;
     da.g = g;
@@ -675,16 +698,16 @@ LbfgsResult Functional::minimize_t(double* xi, const LbfgsOptions& o, hipStream_
       excess_(xf_, df, res_ + kFfSlot, s);
     else
       excess_(xt, df, res_ + kFfSlot, s);
-    dispatch_k(K, [&](auto kc) {
+    dispatch_k(KU, [&](auto kc) {
       hipLaunchKernelGGL((update_kernel<TS, TF, decltype(kc)::value>), dim3(nb_), dim3(kFThreads), 0, s, ua, e,
                          part_, nb_);
     });
     MDFT_HIP_CHECK(hipGetLastError());
-    reduce_rows(part_, nb_, 0, 5 * K + 10, 5 * K + 9, res_, s);
+    reduce_rows(part_, nb_, 0, 5 * KU + 10, 5 * KU + 9, res_, s);
     h = read_results(s);
     double Fn[4];
-    fparts(h[5 * K + 7], h[5 * K + 8], h[kFfSlot], Fn);
-    if (st.judge(Fn, h, K)) {
+    fparts(h[5 * KU + 7], h[5 * KU + 8], h[kFfSlot], Fn);
+    if (st.judge(Fn, h, KU)) {
       std::swap(x, xt);
       std::swap(g, gt);
     }

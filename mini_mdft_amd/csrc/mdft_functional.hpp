@@ -57,6 +57,10 @@ class LbfgsState {
   struct Trial {
     int k = 0;     // stored pairs the direction uses: slots order()[0 .. k)
     int cand = 0;  // slot receiving the candidate pair (s = x_t - x, y = g_t - g)
+    // pairs order()[0 .. k0) need no Gram entries at the trial point: with a
+    // full memory (k == m) the oldest is dropped when the candidate is kept,
+    // and the memory restarts when it is not
+    int k0 = 0;
     double cg = 1.0, gamma = 1.0, t = 1.0;
     double cs[kLbfgsMaxPairs] = {}, cq[kLbfgsMaxPairs] = {};
   };
@@ -66,8 +70,9 @@ class LbfgsState {
   // plans the next trial; false once the solve has ended (result() is final)
   bool next(Trial& tr);
   // the trial's F (4 terms) and update-pass results h (layout of a pass
-  // with K pair columns, K >= the trial's k); true when the trial is
-  // accepted (the driver then takes x_t and g_t as the current point)
+  // with K pair columns, column c holding pair order()[k0 + c], K >= the
+  // trial's k - k0); true when the trial is accepted (the driver then takes
+  // x_t and g_t as the current point)
   bool judge(const double Fn[4], const double* h, int K);
   const std::vector<int>& order() const { return order_; }
   const LbfgsResult& result() const { return r_; }

@@ -382,9 +382,12 @@ def lbfgs(fg, x0: np.ndarray, h0: np.ndarray, m: int = 5, max_iter: int = 200, g
           c1: float = 1e-4, max_backtrack: int = 30, eps_f: float = 1e-6):
     """The L-BFGS of this build (FP64): two-loop recursion with the diagonal
     initial inverse Hessian gamma h0, gamma = s'y / y'(h0 y) of the newest
-    pair; a pair is kept when s'y > 1e-12 |s| |y|; a non-descent direction
-    restarts from -h0 g.  Line search: backtracking from t = 1 (at most
-    max_backtrack trials; after a rejected trial t the next is the minimiser of
+    pair; a pair is kept when s'y > 1e-12 |s| |y|, and a full memory whose
+    new pair fails that test restarts; a non-descent direction restarts from
+    -h0 g.  Line search: backtracking from t = 1, or from t = min(1, 1 /
+    max |h0 g|) along -h0 g (no stored pair), so that no xi moves by more
+    than 1 (at most max_backtrack trials; after a rejected trial t the next is
+    the minimiser of
     the quadratic through F(0), F'(0) = g.d and F(t), kept within [0.1 t, 0.5 t]
     (Nocedal and Wright, Numerical Optimization, 2nd ed., section 3.5), or t/2
     where F(t) is not finite) until the trial point x + t d satisfies Armijo,
@@ -429,6 +432,10 @@ def lbfgs(fg, x0: np.ndarray, h0: np.ndarray, m: int = 5, max_iter: int = 200, g
             d = -h0 * g
             gd = g @ d
         t = 1.0
+        if not S:  # along -h0 g: no xi moves by more than 1
+            gmax = np.abs(h0 * g).max()
+            if gmax > 0:
+                t = min(1.0, 1.0 / gmax)
         for _ in range(max_backtrack):
             xn = x + t * d
             Fn, gn, sn = call(xn)
@@ -452,6 +459,12 @@ def lbfgs(fg, x0: np.ndarray, h0: np.ndarray, m: int = 5, max_iter: int = 200, g
             if len(S) > m:
                 S.pop(0)
                 Y.pop(0)
+        elif len(S) == m:
+            # a full memory whose new pair fails the curvature test restarts
+            # (the GPU update pass forms no Gram entries for the oldest pair
+            # of a full memory, which a kept new pair drops)
+            S.clear()
+            Y.clear()
         x, F, g, scale = xn, Fn, gn, sn
         hist.append(F)
     if np.abs(h0 * g).max() < gtol:
