@@ -134,40 +134,59 @@ __global__ __launch_bounds__(kFThreads) void slot_update_kernel(const SlotDesc<T
       acc[5 * KM + 9] = fmax(acc[5 * KM + 9], fabs(hg));
     }
   } else {
-    MDFT_ELEMENT_LOOP(e) {
-      const double wio = e.w[io], h = e.h0[io];
-      const double xv = ld_nt(&xt[i]), v = ld_nt(&a.V[i]);
-      const double dfv = static_cast<double>(ld_nt(&df[i]));
-      const TS gs = static_cast<TS>(local_terms(xv, v, dfv, wio, e, acc[5 * KM + 7], acc[5 * KM + 8]));
-      const double gn = static_cast<double>(gs);
-      const TS ys = static_cast<TS>(gn - static_cast<double>(ld_nt(&a.g[i])));
-      st_nt(&a.gt[i], gs);
-      st_nt(&a.yn[i], ys);
-      const double yv = static_cast<double>(ys);
-      const double scv = static_cast<double>(ld_nt(&a.sc[i]));
-      double sj[KA], yj[KA];
+    // two elements per trip, both elements' loads first (update_kernel's
+    // loop; the sums take the elements in the one-element order)
+    const long stride_ = static_cast<long>(gridDim.x) * kFThreads;
+    long i = static_cast<long>(blockIdx.x) * kFThreads + threadIdx.x;
+    int io = static_cast<int>(i % e.no);
+    auto nio = [&](int x) { return x + e.step >= e.no ? x + e.step - e.no : x + e.step; };
+    for (; i < e.n; i += 2 * stride_, io = nio(nio(io))) {
+      const bool two = i + stride_ < e.n;
+      const long ix[2] = {i, two ? i + stride_ : i};
+      const int iox[2] = {io, nio(io)};
+      double xv[2], v[2], dfv[2], gv[2], scv[2], sj[2][KA], yj[2][KA];
 #pragma unroll
-      for (int j = 0; j < KM; ++j) {
-        sj[j] = static_cast<double>(ld_nt(&a.s[a.k0 + j][i]));
-        yj[j] = static_cast<double>(ld_nt(&a.y[a.k0 + j][i]));
-      }
-      const double hy = h * yv, hg = h * gn;
+      for (int u = 0; u < 2; ++u) {
+        xv[u] = ld_nt(&xt[ix[u]]);
+        v[u] = ld_nt(&a.V[ix[u]]);
+        dfv[u] = static_cast<double>(ld_nt(&df[ix[u]]));
+        gv[u] = static_cast<double>(ld_nt(&a.g[ix[u]]));
+        scv[u] = static_cast<double>(ld_nt(&a.sc[ix[u]]));
 #pragma unroll
-      for (int j = 0; j < KM; ++j) {
-        acc[j] += scv * yj[j];
-        acc[KM + 1 + j] += sj[j] * yv;
-        acc[2 * KM + 3 + j] += sj[j] * gn;
-        acc[3 * KM + 4 + j] += hy * yj[j];
-        acc[4 * KM + 5 + j] += yj[j] * hg;
+        for (int j = 0; j < KM; ++j) {
+          sj[u][j] = static_cast<double>(ld_nt(&a.s[a.k0 + j][ix[u]]));
+          yj[u][j] = static_cast<double>(ld_nt(&a.y[a.k0 + j][ix[u]]));
+        }
       }
-      acc[KM] += scv * yv;
-      acc[2 * KM + 1] += scv * scv;
-      acc[2 * KM + 2] += yv * yv;
-      acc[3 * KM + 3] += scv * gn;
-      acc[4 * KM + 4] += hy * yv;
-      acc[5 * KM + 5] += yv * hg;
-      acc[5 * KM + 6] += gn * hg;
-      acc[5 * KM + 9] = fmax(acc[5 * KM + 9], fabs(hg));
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        if (u == 1 && !two) break;
+        const double wio = e.w[iox[u]], h = e.h0[iox[u]];
+        const TS gs =
+            static_cast<TS>(local_terms(xv[u], v[u], dfv[u], wio, e, acc[5 * KM + 7], acc[5 * KM + 8]));
+        const double gn = static_cast<double>(gs);
+        const TS ys = static_cast<TS>(gn - gv[u]);
+        st_nt(&a.gt[ix[u]], gs);
+        st_nt(&a.yn[ix[u]], ys);
+        const double yv = static_cast<double>(ys);
+        const double hy = h * yv, hg = h * gn;
+#pragma unroll
+        for (int j = 0; j < KM; ++j) {
+          acc[j] += scv[u] * yj[u][j];
+          acc[KM + 1 + j] += sj[u][j] * yv;
+          acc[2 * KM + 3 + j] += sj[u][j] * gn;
+          acc[3 * KM + 4 + j] += hy * yj[u][j];
+          acc[4 * KM + 5 + j] += yj[u][j] * hg;
+        }
+        acc[KM] += scv[u] * yv;
+        acc[2 * KM + 1] += scv[u] * scv[u];
+        acc[2 * KM + 2] += yv * yv;
+        acc[3 * KM + 3] += scv[u] * gn;
+        acc[4 * KM + 4] += hy * yv;
+        acc[5 * KM + 5] += yv * hg;
+        acc[5 * KM + 6] += gn * hg;
+        acc[5 * KM + 9] = fmax(acc[5 * KM + 9], fabs(hg));
+      }
     }
   }
   store_partials<NA, true>(acc, part, nb);
