@@ -283,6 +283,13 @@ int mdft_vext(int nx, int ny, int nz, double lx, double ly, double lz, int mmax,
  * doubles, and MDFT_MIXED_X6F evaluates the excess on an FP32 copy of xi and
  * reads its df in FP32); vext (device, N x no) is borrowed and must outlive
  * the handle.  F (host) receives F, F_id, F_ext, F_exc.  Both calls synchronise. */
+/* xi = exp(-V / (2 kT)) for n device values (V / kT below -600 taken as -600):
+ * the minimum of F_id + F_ext alone, the density guess from V_ext of the
+ * original MDFT, as a start point for mdft_lbfgs_minimize.  The reference
+ * fills xi with 1 because it has no V_ext (common/MDFT_Solvent.hpp:287-296).
+ * Synchronises. */
+int mdft_xi_from_vext(const double* vext, long long n, double kT, double* xi, void* stream);
+
 typedef struct mdft_functional_s* mdft_functional_t;
 int mdft_functional_create(mdft_solver_t solver, const double* vext, double kT, mdft_functional_t* out);
 int mdft_functional_destroy(mdft_functional_t h);
@@ -349,6 +356,11 @@ int mdft_server_create(int np_luc, const int* tuples, const double* c, int nq_ta
                        double hbm_budget_bytes, mdft_server_t* out);
 int mdft_server_destroy(mdft_server_t h);
 int mdft_server_use_graphs(mdft_server_t h, int on);
+/* Start points of the served solves (before the first step): on = 0, xi = 1
+ * (the default, Solvent::init_density, common/MDFT_Solvent.hpp:287-296);
+ * on = 1, xi = exp(-V / (2 kT)) from each solute's V_ext (mdft_xi_from_vext).
+ * No counterpart in the reference, which has no V_ext to start from. */
+int mdft_server_start_from_vext(mdft_server_t h, int on);
 int mdft_server_submit(mdft_server_t h, int nx, int ny, int nz, double lx, double ly, double lz, int mmax,
                        int nsolute, const double* solute, int nsolvent, const double* solvent, double vmax,
                        double* out_xi, long long* id);

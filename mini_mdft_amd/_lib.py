@@ -71,6 +71,7 @@ _SIGS = {
     "mdft_energy_cproj_batch_async": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     "mdft_rotation_matrices": (_i, [_i, _i, _pd]),
     "mdft_vext": (_i, [_i, _i, _i, _d, _d, _d, _i, _i, _i, _pd, _i, _pd, _d, _vp, _vp]),
+    "mdft_xi_from_vext": (_i, [_vp, ctypes.c_longlong, _d, _vp, _vp]),
     "mdft_functional_create": (_i, [_vp, _vp, _d, ctypes.POINTER(_vp)]),
     "mdft_functional_destroy": (_i, [_vp]),
     "mdft_functional_eval": (_i, [_vp, _vp, _vp, _pd, _vp]),
@@ -78,6 +79,7 @@ _SIGS = {
     "mdft_server_create": (_i, [_i, _pi, _pd, _i, _d, _i, _d, _d, _i, _i, _i, _i, _d, _d, ctypes.POINTER(_vp)]),
     "mdft_server_destroy": (_i, [_vp]),
     "mdft_server_use_graphs": (_i, [_vp, _i]),
+    "mdft_server_start_from_vext": (_i, [_vp, _i]),
     "mdft_server_submit": (_i, [_vp, _i, _i, _i, _d, _d, _d, _i, _i, _pd, _i, _pd, _d, _vp,
                                 ctypes.POINTER(ctypes.c_longlong)]),
     "mdft_server_step": (_i, [_vp, _pi, _pi]),

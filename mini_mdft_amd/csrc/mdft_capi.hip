@@ -787,6 +787,13 @@ int mdft_vext(int nx, int ny, int nz, double lx, double ly, double lz, int mmax,
   });
 }
 
+int mdft_xi_from_vext(const double* vext, long long n, double kT, double* xi, void* stream) {
+  return guarded([&] {
+    xi_from_vext(vext, static_cast<long>(n), kT, xi, as_stream(stream));
+    MDFT_HIP_CHECK(hipStreamSynchronize(as_stream(stream)));
+  });
+}
+
 int mdft_functional_create(mdft_solver_t solver, const double* vext, double kT, mdft_functional_t* out) {
   return guarded([&] {
     throw_if(!out || !solver || !vext, "functional_create: null argument");
@@ -862,6 +869,7 @@ struct mdft_server_s {
   double kT = 0, rho0 = 0, prefactor = 0;
   LbfgsOptions opt;
   bool graphs = true;
+  bool start_vext = false;
   std::unique_ptr<Server> server;
   std::string last_poll_error;  // of the result mdft_server_poll returned last
 
@@ -949,6 +957,7 @@ int mdft_server_create(int np_luc, const int* tuples, const double* c, int nq_ta
           },
           sp->precision == MDFT_MIXED_X6F);
       eng->use_graphs(sp->graphs);
+      eng->start_from_vext(sp->start_vext);
       table_bytes = sp->footprint(sh).second;
       return eng;
     };
@@ -967,6 +976,14 @@ int mdft_server_use_graphs(mdft_server_t h, int on) {
     throw_if(!h, "null handle");
     throw_if(h->server->engines() > 0, "server: set the graph mode before the first step");
     h->graphs = on != 0;
+  });
+}
+
+int mdft_server_start_from_vext(mdft_server_t h, int on) {
+  return guarded([&] {
+    throw_if(!h, "null handle");
+    throw_if(h->server->engines() > 0, "server: set the start point before the first step");
+    h->start_vext = on != 0;
   });
 }
 

@@ -365,6 +365,11 @@ void Engine::cancel(int slot) {
   release_slot(slot);
 }
 
+void Engine::vext_ready(int slot) {
+  throw_if(slot < 0 || slot >= capacity_ || !slots_[slot], "engine: no such slot");
+  if (start_vext_) xi_from_vext(slots_[slot]->V, n_, p_.kT, slots_[slot]->x, stream_);
+}
+
 double* Engine::vext_of(int slot) {
   throw_if(slot < 0 || slot >= capacity_ || !slots_[slot], "engine: no such slot");
   return slots_[slot]->V;
@@ -645,6 +650,7 @@ void Server::admit() {
                    make_angular_grid(rq.shape.mmax, mrso_), rq.solute.data(), static_cast<int>(rq.solute.size() / 6),
                    rq.solvent.data(), static_cast<int>(rq.solvent.size() / 6), rq.vmax, eng->vext_of(slot),
                    eng->stream());
+      eng->vext_ready(slot);
     } catch (const std::exception& ex) {
       // a failed runtime call (an allocation) stays the thread's last HIP
       // error; clear it so that the next launch check does not report it

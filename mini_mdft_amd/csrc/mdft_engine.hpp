@@ -51,6 +51,10 @@ class Engine {
   // gives an admitted slot back before its first tick (its set-up failed)
   void cancel(int slot);
   double* vext_of(int slot);
+  // called once the slot's V_ext is in place (on stream()): with
+  // start_from_vext on, the slot starts from xi = exp(-V / (2 kT))
+  // (xi_from_vext) instead of xi = 1
+  void vext_ready(int slot);
   // one trial (or start-point evaluation) for every active slot; returns the
   // number of slots that finished in this tick.  begin_tick enqueues the
   // tick on stream(), end_tick waits for it and makes the decisions.
@@ -65,6 +69,8 @@ class Engine {
   std::vector<Done> take_done();
   // graphs on (default) or off: results are the same either way
   void use_graphs(bool on) { graphs_ = on; }
+  // start points: xi = 1 (default, Solvent::init_density) or exp(-V / (2 kT))
+  void start_from_vext(bool on) { start_vext_ = on; }
 
  private:
   struct Slot;
@@ -80,6 +86,7 @@ class Engine {
   bool fp32_;
   bool f32f_;
   bool graphs_ = true;
+  bool start_vext_ = false;
   long n_ = 0;
   int nb_ = 0, step_ = 0;
   hipStream_t stream_ = nullptr;

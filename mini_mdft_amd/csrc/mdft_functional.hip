@@ -266,6 +266,13 @@ __global__ __launch_bounds__(kFThreads) void vext_kernel(VextArgs a, const doubl
   }
 }
 
+__global__ __launch_bounds__(kFThreads) void xi_from_vext_kernel(const double* __restrict__ V, long n, double ikT,
+                                                                 double* __restrict__ xi) {
+  for (long i = static_cast<long>(blockIdx.x) * kFThreads + threadIdx.x; i < n;
+       i += static_cast<long>(gridDim.x) * kFThreads)
+    xi[i] = exp(-0.5 * fmax(V[i] * ikT, -600.0));
+}
+
 // Constants::qfact (common/MDFT_Constants.hpp:36-47): qfact q q / r in kJ/mol
 constexpr double kQunit = 1.602176565e-19, kNavo = 6.02214129e23, kEps0 = 8.854187817e-12;
 constexpr double kPi = 3.14159265358979323846;
@@ -351,6 +358,14 @@ void compute_vext(int nx, int ny, int nz, double lx, double ly, double lz, const
   MDFT_HIP_CHECK(hipGetLastError());
   // the tables (host and device) are released on return
   MDFT_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void xi_from_vext(const double* V, long n, double kT, double* xi, hipStream_t s) {
+  throw_if(n < 0 || (n > 0 && (!V || !xi)) || !(kT > 0), "xi_from_vext: bad arguments");
+  if (n == 0) return;
+  hipLaunchKernelGGL(xi_from_vext_kernel, dim3(grid_for(n, kFThreads, 65536)), dim3(kFThreads), 0, s, V, n,
+                     1.0 / kT, xi);
+  MDFT_HIP_CHECK(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------

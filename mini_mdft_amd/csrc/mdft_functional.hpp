@@ -20,6 +20,13 @@ void compute_vext(int nx, int ny, int nz, double lx, double ly, double lz, const
                   const double* solute, int nsolute, const double* solvent, int nsolvent, double vmax, double* vext,
                   hipStream_t s);
 
+// xi = exp(-V / (2 kT)) on the device (n values, on stream s): the minimum of
+// F_id + F_ext alone (xi^2 = exp(-V / kT)), the density guess from V_ext of
+// the original MDFT (the reference fills xi with 1 only because it has no
+// V_ext, common/MDFT_Solvent.hpp:287-296).  V / kT below -600 is taken as
+// -600, so that xi^2 stays finite.
+void xi_from_vext(const double* V, long n, double kT, double* xi, hipStream_t s);
+
 struct FunctionalParams {
   long N;  // grid points
   int no;  // orientations

@@ -435,6 +435,21 @@ def vext(spatial_grid: SpatialGrid, angular_grid: AngularGrid, solute, solvent, 
     return out
 
 
+def xi_from_vext(V, kT: float, out=None, stream=None):
+    """xi = exp(-V / (2 kT)) on the GPU (V / kT below -600 taken as -600): the
+    minimum of F_id + F_ext alone, the density guess from V_ext of the original
+    MDFT, as a start point for Functional.minimize.  The reference fills xi
+    with 1 (common/MDFT_Solvent.hpp:287-296) because it has no V_ext."""
+    torch = _torch()
+    if out is None:
+        out = torch.empty_like(V)
+    _require(V, torch.float64, out.numel(), "xi_from_vext")
+    _require(out, torch.float64, V.numel(), "xi_from_vext")
+    check(lib().mdft_xi_from_vext(ptr(V), ctypes.c_longlong(V.numel()), ctypes.c_double(kT), ptr(out),
+                                  stream_handle(stream)))
+    return out
+
+
 class Functional:
     """F = F_id + F_ext + F_exc and its gradient on the GPU, with an FP64
     L-BFGS minimiser (the reference's Solver makes one excess evaluation,
@@ -513,6 +528,11 @@ class Server:
 
     def use_graphs(self, on: bool) -> None:
         check(lib().mdft_server_use_graphs(self._h, 1 if on else 0))
+
+    def start_from_vext(self, on: bool) -> None:
+        """Start points of the solves (set before the first step): xi = 1 (the
+        default) or xi = exp(-V / (2 kT)) from each solute's V_ext (xi_from_vext)."""
+        check(lib().mdft_server_start_from_vext(self._h, 1 if on else 0))
 
     def submit(self, n_nodes, length, mmax: int, solute, solvent, vmax: float, out=None) -> int:
         """Queues one solute; `out` (a float64 CUDA tensor of N x no) receives

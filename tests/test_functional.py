@@ -354,6 +354,40 @@ def test_gpu_lbfgs_trajectory_matches_oracle(oracle):
 
 
 @pytest.mark.gpu
+def test_gpu_solve_from_vext_guess_matches_oracle(oracle):
+    """xi = exp(-V / (2 kT)) (the minimum of F_id + F_ext alone) against
+    numpy, and the FP64 solve from it against the oracle's L-BFGS from the
+    same start: the same minimum as the solve from xi = 1, in fewer
+    iterations."""
+    P = _gpu_problem(oracle)
+    torch = P["torch"]
+    w = P["ag"].m_w
+    Vh = P["V"].cpu().numpy()
+    no = P["ag"].m_no
+    xg = M.xi_from_vext(P["V"], KT)
+    ref_x = np.exp(-0.5 * Vh / KT)
+    assert np.abs(xg.cpu().numpy() - ref_x).max() <= 4e-16 * np.abs(ref_x).max()
+    # F_id + F_ext is stationary there: its gradient 2 rho0 xi (kT ln xi^2 + V) vanishes
+    assert np.abs(KT * np.log(ref_x ** 2) + Vh).max() <= 1e-12 * np.abs(Vh).max()
+    h0 = np.broadcast_to(1.0 / (2 * RHO0 * P["dv"] * w * KT), (P["N"], no)).ravel()
+
+    def fg(x):
+        (F, *parts), g = oracle.functional(P["oconv"], x.reshape(P["N"], -1), Vh, w, RHO0, PREF, KT, P["dv"])
+        return F, g.ravel(), sum(abs(p) for p in parts)
+
+    ref = oracle.lbfgs(fg, ref_x.ravel().copy(), h0, m=5, max_iter=300, gtol=1e-6)
+    xi = xg.clone()
+    r = P["fun"].minimize(xi, m=5, max_iter=300, gtol=1e-6)
+    assert r["status"] == ref[6] == 0 and r["residual"] < 1e-6
+    assert abs(r["iterations"] - ref[3]) <= 2
+    assert abs(r["F"][0] - ref[1]) <= 1e-10 * abs(ref[1])
+    one = torch.ones((P["N"], no), dtype=torch.float64, device="cuda")
+    r1 = P["fun"].minimize(one, m=5, max_iter=300, gtol=1e-6)
+    assert r1["status"] == 0 and r["iterations"] < r1["iterations"]
+    assert abs(r["F"][0] - r1["F"][0]) <= 1e-7 * abs(r1["F"][0])
+
+
+@pytest.mark.gpu
 def test_gpu_lbfgs_c0_limit(oracle):
     P = _gpu_problem(oracle, amp=0.0, vmax_kt=10.0)
     torch = P["torch"]

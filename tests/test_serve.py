@@ -96,6 +96,39 @@ def test_server_matches_functional_minimize(prec):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("prec", [FP64, MIXED_X6F])
+def test_server_start_from_vext_matches_functional(prec):
+    """Served solves started from xi = exp(-V / (2 kT)): the same decisions
+    and numbers as Functional.minimize from M.xi_from_vext of the same V_ext,
+    and the same minimum as from xi = 1."""
+    import torch
+
+    tab = _table()
+    sol = _solutes(1)[0]
+    sg, ag = M.SpatialGrid(N3, L3), M.AngularGrid(MMAX, MRSO)
+    out = torch.zeros((sg.N, ag.m_no), dtype=torch.float64, device="cuda")
+    srv = M.Server(tab, MRSO, N0, TEMP, prec, capacity=2)
+    srv.start_from_vext(True)
+    r_srv = _run(srv, [(sol, out)])[0]
+    conv = M.Convolution.from_table(tab, sg, ag, M.OrientationProjectionMap(ag), FP64 if prec == FP64 else FP32)
+    solver = M.Solver(conv, M.rho0_of(N0, MRSO), M.prefactor_of(TEMP, MRSO, N0), prec)
+    V = M.vext(sg, ag, sol, WATER, VMAX)
+    fun = M.Functional(solver, V, KT)
+    xi = M.xi_from_vext(V, KT)
+    r = fun.minimize(xi, m=5, max_iter=400, gtol=1e-5)
+    assert r["status"] == r_srv["status"] == 0
+    assert r["iterations"] == r_srv["iterations"] and r["evaluations"] == r_srv["evaluations"]
+    assert r["F"] == r_srv["F"]
+    assert torch.equal(xi, out)
+    one = torch.ones_like(out)
+    r1 = fun.minimize(one, m=5, max_iter=400, gtol=1e-5)
+    assert r1["status"] == 0 and abs(r["F"][0] - r1["F"][0]) <= 1e-6 * abs(r1["F"][0])
+    # the start point is set before the first step only
+    with pytest.raises(M.MDFTError):
+        srv.start_from_vext(False)
+
+
+@pytest.mark.gpu
 def test_graph_replay_changes_nothing():
     import torch
 
