@@ -38,6 +38,17 @@ void compute_vext(const ExecutionSpace& exec, const SpatialGrid<ExecutionSpace, 
                         vext.data(), exec.handle()));
 }
 
+// xi = exp(-V / (2 kT)) from vext (both (nx, ny, nz, no) doubles): the
+// minimum of F_id + F_ext alone, a start point for Functional::minimize (the
+// reference starts from xi = 1, common/MDFT_Solvent.hpp:287-296, having no
+// V_ext).  Waits for the stream.
+template <typename ExecutionSpace, typename View4DType>
+void xi_from_vext(const ExecutionSpace& exec, const View4DType& vext, double kT, const View4DType& xi) {
+  static_assert(std::is_same<typename View4DType::non_const_value_type, double>::value, "xi is a double view");
+  Impl::Throw_If(vext.size() != xi.size(), "xi_from_vext: vext and xi differ in size");
+  Impl::check(mdft_xi_from_vext(vext.data(), static_cast<long long>(vext.size()), kT, xi.data(), exec.handle()));
+}
+
 struct MinimizeResult {
   std::array<double, 4> F;  // F, F_id, F_ext, F_exc (kJ/mol)
   int iterations, evaluations;

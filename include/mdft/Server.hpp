@@ -44,6 +44,10 @@ class Server {
   Server(const Server&) = delete;
   Server& operator=(const Server&) = delete;
 
+  // start points of the solves (before the first step): xi = 1 (default) or
+  // xi = exp(-V / (2 kT)) from each solute's V_ext
+  void start_from_vext(bool on) { Impl::check(mdft_server_start_from_vext(m_h, on ? 1 : 0)); }
+
   // queues one solute; out_xi (device, N no doubles) receives the final xi
   template <typename ExecutionSpace, typename ScalarType>
   long long submit(const SpatialGrid<ExecutionSpace, ScalarType>& grid, int mmax, const std::vector<Site>& solute,
