@@ -232,35 +232,73 @@ struct VextArgs {
 // and solute site j, floored at 1e-6 A; pair[(k nsolute + j) * 3] =
 // (sigma, 4 eps, qfact q q), sigma = 0 where either site has no LJ.  1/d is
 // one rsqrt of d^2, clamped at 1e6 (the floor): against a square root and a
-// division, 8.4 -> 5.3 ms at 64^3 / mmax 5 with 20 x 3 site pairs.
-__global__ __launch_bounds__(kFThreads) void vext_kernel(VextArgs a, const double* __restrict__ off,
+// division, 8.4 -> 5.3 ms at 64^3 / mmax 5 with 20 x 3 site pairs.  Sites at
+// the molecular origin are summed once per node (vext_node_kernel); when they
+// lead the site list, as the oxygen does, the sum keeps its order.
+
+// v += sum_j [LJ_kj(d) + qq_kj / d] for solvent site k at (px, py, pz)
+__device__ __forceinline__ void vext_site(const VextArgs& a, double px, double py, double pz, int k,
+                                          const double* __restrict__ pair, const double* __restrict__ solute,
+                                          double& v) {
+  for (int j = 0; j < a.nsolute; ++j) {
+    double ddx = px - solute[3 * j], ddy = py - solute[3 * j + 1], ddz = pz - solute[3 * j + 2];
+    ddx -= a.lx * rint(ddx * a.ilx);
+    ddy -= a.ly * rint(ddy * a.ily);
+    ddz -= a.lz * rint(ddz * a.ilz);
+    const double inv = fmin(rsqrt(ddx * ddx + ddy * ddy + ddz * ddz), 1e6);
+    const double* pp = pair + (static_cast<long>(k) * a.nsolute + j) * 3;
+    if (pp[0] > 0.0) {
+      const double sr = pp[0] * inv, sr2 = sr * sr, sr6 = sr2 * sr2 * sr2;
+      v += pp[1] * (sr6 * sr6 - sr6);
+    }
+    v += pp[2] * inv;
+  }
+}
+
+__device__ __forceinline__ void node_position(const VextArgs& a, long r, double& gx, double& gy, double& gz) {
+  const int ix = static_cast<int>(r / a.nyz);
+  const int rem = static_cast<int>(r - static_cast<long>(ix) * a.nyz);
+  const int iy = rem / a.nz, iz = rem - iy * a.nz;
+  gx = ix * a.dx;
+  gy = iy * a.dy;
+  gz = iz * a.dz;
+}
+
+// Vr(r) = the terms of the solvent sites at the molecular origin (the bits
+// of `origin`, such as the oxygen of tip3p or spce), which no orientation
+// moves: once per node instead of once per (node, orientation)
+__global__ __launch_bounds__(kFThreads) void vext_node_kernel(VextArgs a, unsigned origin,
+                                                              const double* __restrict__ pair,
+                                                              const double* __restrict__ solute,
+                                                              double* __restrict__ Vr) {
+  const long nr = a.n / a.no;
+  for (long r = static_cast<long>(blockIdx.x) * kFThreads + threadIdx.x; r < nr;
+       r += static_cast<long>(gridDim.x) * kFThreads) {
+    double gx, gy, gz;
+    node_position(a, r, gx, gy, gz);
+    double v = 0.0;
+    for (int k = 0; k < a.nsolv; ++k)
+      if ((origin >> k) & 1u) vext_site(a, gx, gy, gz, k, pair, solute, v);
+    Vr[r] = v;
+  }
+}
+
+// V(r, o): Vr(r) (or 0 without origin sites), then the other sites in order
+__global__ __launch_bounds__(kFThreads) void vext_kernel(VextArgs a, unsigned origin, const double* __restrict__ Vr,
+                                                         const double* __restrict__ off,
                                                          const double* __restrict__ pair,
                                                          const double* __restrict__ solute, double* __restrict__ V) {
   for (long idx = static_cast<long>(blockIdx.x) * kFThreads + threadIdx.x; idx < a.n;
        idx += static_cast<long>(gridDim.x) * kFThreads) {
     const long r = idx / a.no;
     const int io = static_cast<int>(idx - r * a.no);
-    const int ix = static_cast<int>(r / a.nyz);
-    const int rem = static_cast<int>(r - static_cast<long>(ix) * a.nyz);
-    const int iy = rem / a.nz, iz = rem - iy * a.nz;
-    const double gx = ix * a.dx, gy = iy * a.dy, gz = iz * a.dz;
-    double v = 0.0;
+    double gx, gy, gz;
+    node_position(a, r, gx, gy, gz);
+    double v = Vr ? Vr[r] : 0.0;
     for (int k = 0; k < a.nsolv; ++k) {
+      if ((origin >> k) & 1u) continue;
       const double* o = off + (static_cast<long>(io) * a.nsolv + k) * 3;
-      const double px = gx + o[0], py = gy + o[1], pz = gz + o[2];
-      for (int j = 0; j < a.nsolute; ++j) {
-        double ddx = px - solute[3 * j], ddy = py - solute[3 * j + 1], ddz = pz - solute[3 * j + 2];
-        ddx -= a.lx * rint(ddx * a.ilx);
-        ddy -= a.ly * rint(ddy * a.ily);
-        ddz -= a.lz * rint(ddz * a.ilz);
-        const double inv = fmin(rsqrt(ddx * ddx + ddy * ddy + ddz * ddz), 1e6);
-        const double* pp = pair + (static_cast<long>(k) * a.nsolute + j) * 3;
-        if (pp[0] > 0.0) {
-          const double sr = pp[0] * inv, sr2 = sr * sr, sr6 = sr2 * sr2 * sr2;
-          v += pp[1] * (sr6 * sr6 - sr6);
-        }
-        v += pp[2] * inv;
-      }
+      vext_site(a, gx + o[0], gy + o[1], gz + o[2], k, pair, solute, v);
     }
     V[idx] = fmin(v, a.vmax);
   }
@@ -344,7 +382,7 @@ void compute_vext(int nx, int ny, int nz, double lx, double ly, double lz, const
     ~Buf() {
       if (p) (void)hipFree(p);
     }
-  } doff, dpair, dpos;
+  } doff, dpair, dpos, dvr;
   MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&doff.p), off.size() * sizeof(double)));
   MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&dpair.p), pair.size() * sizeof(double)));
   MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&dpos.p), pos.size() * sizeof(double)));
@@ -353,8 +391,20 @@ void compute_vext(int nx, int ny, int nz, double lx, double ly, double lz, const
   MDFT_HIP_CHECK(hipMemcpyAsync(dpos.p, pos.data(), pos.size() * sizeof(double), hipMemcpyHostToDevice, s));
   VextArgs a{n, g.no, ny * nz, nz, nsolvent, nsolute, lx / nx, ly / ny, lz / nz, lx, ly, lz,
              1.0 / lx, 1.0 / ly, 1.0 / lz, vmax};
-  hipLaunchKernelGGL(vext_kernel, dim3(grid_for(n, kFThreads, 65536)), dim3(kFThreads), 0, s, a, doff.p, dpair.p,
-                     dpos.p, vext);
+  // solvent sites at the molecular origin (all of the site's offsets zero)
+  unsigned origin = 0;
+  if (nsolvent <= 32)
+    for (int k = 0; k < nsolvent; ++k)
+      if (solvent[6 * k + 3] == 0.0 && solvent[6 * k + 4] == 0.0 && solvent[6 * k + 5] == 0.0) origin |= 1u << k;
+  const long nr = static_cast<long>(nx) * ny * nz;
+  if (origin) {
+    MDFT_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&dvr.p), nr * sizeof(double)));
+    hipLaunchKernelGGL(vext_node_kernel, dim3(grid_for(nr, kFThreads, 65536)), dim3(kFThreads), 0, s, a, origin,
+                       dpair.p, dpos.p, dvr.p);
+    MDFT_HIP_CHECK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(vext_kernel, dim3(grid_for(n, kFThreads, 65536)), dim3(kFThreads), 0, s, a, origin, dvr.p,
+                     doff.p, dpair.p, dpos.p, vext);
   MDFT_HIP_CHECK(hipGetLastError());
   // the tables (host and device) are released on return
   MDFT_HIP_CHECK(hipStreamSynchronize(s));
